@@ -1,0 +1,164 @@
+/* polar_amd.h -- C-ABI of the MI355X (gfx950) pole-shared Krylov core.
+ *
+ * Every entry point takes raw device pointers owned by the caller, sizes as int64_t, and an
+ * explicit hipStream_t (passed as void*).  The library allocates no device memory: temporaries
+ * live in a caller-provided workspace whose size is returned by the matching *_ws_bytes query.
+ * Return value: 0 on success, a positive hipError_t, or a negative POLAR_E_* code; the text of
+ * the last error is returned by polar_last_error().  No entry point synchronises the stream
+ * except polar_msolve (a 4-byte convergence-flag read every 8 iterations, as the reference's
+ * host check at realworld/polar_gpu.py:126) and polar_select_topb / polar_rerank_pick, whose
+ * results are the host-side shortlist/winner the reference copies back (polar_gpu.py:221, :236).
+ *
+ * Reference interfaces replaced (chenty2333/POLAR):
+ *   realworld/spmm.py:3-25          csr_spmm(n, R, ptr, idx, val, X, Y)   -> polar_spmm
+ *   realworld/polar_gpu.py:63-80    Op.__call__(x, shift)                  -> polar_spmm (patch)
+ *   realworld/polar_gpu.py:88-133   msolve(op, W, tau, active, tol, ...)   -> polar_msolve
+ *   realworld/polar_gpu.py:205      bank gather-reduce h, t                -> polar_pair_sqnorm
+ *   realworld/polar_gpu.py:208-211  proxy(h, t) + dead mask (:219)         -> polar_proxy
+ *   realworld/polar_gpu.py:221      argsort(-sc)[:Br]                      -> polar_select_topb
+ *   realworld/polar_gpu.py:225-236  Wb build, bu gather, re-rank argmax    -> polar_pair_rhs,
+ *                                                                             polar_rerank_pick
+ *   realworld/polar_gpu.py:243-249  Sherman-Morrison maintenance          -> polar_maint_prep,
+ *                                                                             polar_maint_rank1,
+ *                                                                             polar_maint_cache
+ *   realworld/polar_gpu.py:250      F[a] += Gc[e]; F[b] -= Gc[e]           -> polar_rows_axpy
+ *   realworld/polar_gpu.py:148-165  sketches (counter-hash Rademacher kind) -> polar_rademacher
+ */
+#ifndef POLAR_AMD_H
+#define POLAR_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define POLAR_ABI_VERSION 1
+
+enum polar_dtype { POLAR_F64 = 0, POLAR_F32 = 1 };
+
+enum polar_status {
+  POLAR_OK = 0,
+  POLAR_E_ITERCAP = -2,   /* multi-shift CG iteration cap (polar_gpu.py:128-129) */
+  POLAR_E_ARG = -3,       /* invalid argument (shape, dtype, alignment, workspace size) */
+};
+
+/* msolve flags */
+#define POLAR_TRACK 1     /* track ||u_j||^2 with three scalars per pole (polar_gpu.py:114-117) */
+#define POLAR_TRACE 2     /* track w^T (M + tau_j)^{-1} w with two scalars per pole */
+#define POLAR_NOFREEZE 4  /* never freeze converged poles (synthetic/scalability/benchmark.py:72) */
+
+/* Operator y = (diag + shift) x - A_w x + patch(x): the graph Laplacian M0 = diag(deg+sigma) - A
+ * of realworld/polar_gpu.py:47-55 stored as diagonal + off-diagonal adjacency (value-free when
+ * every weight is 1), plus the unit-weight added edges of polar_gpu.py:68-79 as a row patch.
+ * Long rows (degree > POLAR_CHUNK) are split into chunks; their schedule is built by the host. */
+#define POLAR_CHUNK 256
+typedef struct polar_op {
+  int64_t n;
+  int32_t dtype;              /* polar_dtype of weights, diag and all vectors */
+  int32_t weighted;           /* 0: all off-diagonal weights are 1 and `weights` is NULL */
+  const int32_t* indptr;      /* [n+1] */
+  const int32_t* indices;     /* [nnz]  off-diagonal columns */
+  const void* weights;        /* [nnz]  off-diagonal weights (positive), or NULL */
+  const void* diag;           /* [n]    weighted degree + sigma */
+  int32_t n_long;             /* rows with degree > POLAR_CHUNK */
+  int32_t n_extra;            /* chunks of long rows beyond their first */
+  int32_t n_partial;          /* total chunks of long rows */
+  int32_t pad0;
+  const int32_t* long_rows;   /* [n_long] ascending row ids */
+  const int32_t* long_base;   /* [n_long] first partial slot of each long row */
+  const int32_t* extra;       /* [n_extra][2]: (long index, chunk index >= 1) */
+  int32_t n_patch;            /* rows touched by added edges */
+  int32_t pad1;
+  const int32_t* patch_rows;  /* [n_patch] ascending */
+  const int32_t* patch_ptr;   /* [n_patch+1] */
+  const int32_t* patch_cols;  /* [patch_ptr[n_patch]] */
+} polar_op;
+
+int polar_abi_version(void);
+const char* polar_last_error(void);
+
+/* Y[:, :R] = (M0 + shift I + sum_S b b^T) X[:, :R]; X, Y row-major with leading dims ldx, ldy. */
+int64_t polar_spmm_ws_bytes(const polar_op* op, int64_t R);
+int polar_spmm(const polar_op* op, double shift, const void* X, int64_t ldx, void* Y, int64_t ldy,
+               int64_t R, void* ws, int64_t ws_bytes, void* stream);
+
+/* Block multi-shift CG for (M + tau_j I) U = W[:, :R] on all poles (realworld/polar_gpu.py:88-133).
+ * tau: device double[p], ascending (tau[0] is the seed shift).  active: device int32[nA].
+ * Xout: element (j, i, c) at Xout + j*x_sj + i*x_si + c (dtype of op).
+ * h_out (TRACK), te_out (TRACE): device double[p*R] or NULL.  iters_out: device int64[R].
+ * Returns POLAR_E_ITERCAP when a column is still active after the last 8-iteration check within
+ * maxit iterations (the reference's for/else at polar_gpu.py:100-129). */
+int64_t polar_msolve_ws_bytes(const polar_op* op, int64_t R, int32_t p, int64_t nA, int32_t flags);
+int polar_msolve(const polar_op* op, const void* W, int64_t ldw, int64_t R, const double* tau,
+                 int32_t p, const int32_t* active, int64_t nA, double tol, int64_t maxit,
+                 int32_t flags, void* Xout, int64_t x_sj, int64_t x_si, double* h_out,
+                 double* te_out, int64_t* iters_out, void* ws, int64_t ws_bytes, void* stream);
+
+/* out[j*N + e] = sum_c (X[j, ea[e], c] - X[j, eb[e], c])^2, double (polar_gpu.py:205). */
+int polar_pair_sqnorm(int32_t dtype, const void* X, int64_t x_sj, int64_t x_si, int32_t p,
+                      int64_t q, const int32_t* ea, const int32_t* eb, int64_t N, double* out,
+                      void* stream);
+
+/* score[e] = sum_j om_j h/(1 + t + (sigma + tau_j) h)  (obj 0 = frac/er), or
+ * log1p(t[0,e] + sigma h[0,e]) (obj 2 = logdet); dead[e] != 0 -> -inf (polar_gpu.py:208-219). */
+int polar_proxy(int32_t obj, const double* h, const double* t, const double* tau, const double* om,
+                int32_t p, int64_t N, double sigma, const uint8_t* dead, double* score, void* stream);
+
+/* First Br indices of the stable descending order of score (ties: lower index first), i.e.
+ * argsort(-score, stable)[:Br] (polar_gpu.py:221).  Writes device int32[Br] to out and copies them
+ * to host_out (synchronising the stream).  ws: polar_select_ws_bytes(N) bytes. */
+int64_t polar_select_ws_bytes(int64_t N);
+int polar_select_topb(const double* score, int64_t N, int32_t Br, int32_t* out, int32_t* host_out,
+                      void* ws, int64_t ws_bytes, void* stream);
+
+/* W[:, c] = e_a - e_b for pairs (a[c], b[c]), c < Br; W is n x Br row-major, zeroed first. */
+int polar_pair_rhs(int32_t dtype, int64_t n, int32_t Br, const int32_t* a, const int32_t* b,
+                   void* W, void* stream);
+
+/* Re-rank (polar_gpu.py:229-236): bu[j,c] = U[j, sa[c], c] - U[j, sb[c], c];
+ * g[c] = sum_j om_j hh[j,c]/(1+bu[j,c]) (obj 0) or log1p(bu[0,c]) (obj 2);
+ * winner = first argmax.  Writes g (device double[Br]), copies (winner, g[winner]) to host. */
+int polar_rerank_pick(int32_t obj, int32_t dtype, const void* U, int64_t u_sj, int64_t u_si,
+                      int32_t p, int32_t Br, const int32_t* sa, const int32_t* sb, const double* hh,
+                      const double* om, double* g, int32_t* host_winner, double* host_value,
+                      void* stream);
+
+/* Maintenance (polar_gpu.py:243-249), applied to the caches after choosing candidate e with
+ * endpoint rows (aa, bb) in the active set and solution column u (element (j,i) at
+ * u + j*u_sj + i*u_si):
+ *   prep:  xi[j,:] = X[j,aa]-X[j,bb]; eta[j,:] = gc - (Y[j,aa]-Y[j,bb]); D[j] = 1+u[j,aa]-u[j,bb];
+ *          nxi[j] = |xi_j|^2; neta[j] = |eta_j|^2   (coef: double[p*(2q+3)] laid out xi|eta|D|nxi|neta)
+ *   rank1: wX[j,i] = X[j,i,:].xi_j then X[j,i,:] -= u[j,i] xi_j / D_j   (one pass, old rows)
+ *          wY[j,i] = Y[j,i,:].eta_j then Y[j,i,:] += u[j,i] eta_j / D_j
+ *   cache: cc = (u[j,ea]-u[j,eb])/D_j; h = max(0, h - 2cc(wX[ea]-wX[eb]) + cc^2 nxi);
+ *          t = max(0, t + 2cc(wY[ea]-wY[eb]) + cc^2 neta). */
+int polar_maint_prep(int32_t dtype, const void* X, const void* Y, int64_t sj, int64_t si,
+                     int32_t p, int64_t q, const void* u, int64_t u_sj, int64_t u_si, int32_t aa,
+                     int32_t bb, const double* gc, double* coef, void* stream);
+int polar_maint_rank1(int32_t dtype, void* X, void* Y, int64_t sj, int64_t si, int32_t p,
+                      int64_t nA, int64_t q, const void* u, int64_t u_sj, int64_t u_si,
+                      const double* coef, double* wX, double* wY, void* stream);
+int polar_maint_cache(int32_t dtype, const void* u, int64_t u_sj, int64_t u_si, int32_t p,
+                      int64_t nA, int64_t N, const int32_t* ea, const int32_t* eb,
+                      const double* coef, int64_t q, const double* wX, const double* wY,
+                      double* h, double* t, void* stream);
+
+/* F[a, :] += g; F[b, :] -= g for one row-vector g (double[q]); F row-major, leading dim ldf. */
+int polar_rows_axpy(int32_t dtype, void* F, int64_t ldf, int64_t q, int32_t a, int32_t b,
+                    const double* g, void* stream);
+
+/* Counter-hash Rademacher rows: out[i, c] = scale * (+-1), sign = top bit of
+ * splitmix64(splitmix64(seed*4 + stream) + key_i*q + c), key_i = keys ? keys[i] : i. */
+int polar_rademacher(int32_t dtype, uint64_t seed, int32_t stream_id, const int64_t* keys,
+                     int64_t rows, int64_t q, double scale, void* out, int64_t ld, void* stream);
+
+/* F = C0^T U for the Rademacher edge sketch, from the adjacency of op: for row i and neighbour j
+ * with weight w, F[i,:] += sign * sqrt(w) * U(min(i,j)*n + max(i,j)), sign = +1 if i < j else -1. */
+int polar_edge_sketch(const polar_op* op, uint64_t seed, int64_t q, double scale, void* F,
+                      int64_t ldf, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* POLAR_AMD_H */

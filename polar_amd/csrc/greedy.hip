@@ -1,0 +1,585 @@
+// Greedy-round kernels: endpoint-sketch bank gathers, proxy scores, top-B shortlist, re-rank,
+// Sherman-Morrison cache maintenance and the counter-hash Rademacher sketches (gfx950, wave64).
+//
+// Reference semantics: realworld/polar_gpu.py:200-251 (bank, proxy, shortlist, re-rank,
+// maintenance) and :148-165 (sketches).  All of these are HBM-bound streams or gathers over
+// (p x N) or (p x nA x q) arrays; rows of q sketch columns are read by a group of lanes with
+// 16-byte loads and reduced across the group with ds_swizzle/DPP (__shfl_xor).
+#include "common.hpp"
+
+namespace polar {
+
+// Sum over the L lanes of a row-group (L power of two); result valid in every lane of the group.
+__device__ __forceinline__ double group_sum(double v, int L) {
+  for (int o = L >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <typename T, int CPL>
+__device__ __forceinline__ void load_cols(const T* __restrict__ row, int c0, int64_t q, double (&o)[CPL]) {
+  if (c0 + CPL <= q) {
+    T t[CPL];
+    vload<T, CPL>(row + c0, t);
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) o[e] = (double)t[e];
+  } else {
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) o[e] = (c0 + e < q) ? (double)row[c0 + e] : 0.0;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// out[j*N + e] = |X[j, ea[e], :] - X[j, eb[e], :]|^2
+template <typename T, int CPL>
+__global__ __launch_bounds__(kBlock) void k_pair_sqnorm(const T* __restrict__ X, int64_t sj, int64_t si,
+                                                        int p, int64_t q, const int32_t* __restrict__ ea,
+                                                        const int32_t* __restrict__ eb, int64_t N,
+                                                        double* __restrict__ out, int L) {
+  Geo g(L);
+  const int c0 = g.li * CPL;
+  const int64_t gpb = (int64_t)kWaves * g.G;
+  const int64_t total = (int64_t)p * N;
+  for (int64_t item = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi; item < total;
+       item += (int64_t)gridDim.x * gpb) {
+    const int j = (int)(item / N);
+    const int64_t e = item - (int64_t)j * N;
+    const T* xa = X + (int64_t)j * sj + (int64_t)ea[e] * si;
+    const T* xb = X + (int64_t)j * sj + (int64_t)eb[e] * si;
+    double s = 0.0;
+    if (c0 < q) {
+      double va[CPL], vb[CPL];
+      load_cols<T, CPL>(xa, c0, q, va);
+      load_cols<T, CPL>(xb, c0, q, vb);
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) {
+        const double dd = va[k] - vb[k];
+        s += dd * dd;
+      }
+    }
+    s = group_sum(s, L);
+    if (g.li == 0) out[item] = s;
+  }
+}
+
+// score[e] = sum_j om_j h / (1 + t + (sigma + tau_j) h)  | log1p(t0 + sigma h0); dead -> -inf
+__global__ __launch_bounds__(kBlock) void k_proxy(int obj, const double* __restrict__ h,
+                                                  const double* __restrict__ t, const double* __restrict__ tau,
+                                                  const double* __restrict__ om, int p, int64_t N, double sigma,
+                                                  const uint8_t* __restrict__ dead, double* __restrict__ score) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (int64_t)gridDim.x * blockDim.x) {
+    double sc;
+    if (obj == 2) {
+      sc = log1p(t[e] + sigma * h[e]);
+    } else {
+      sc = 0.0;
+      for (int j = 0; j < p; ++j) {
+        const double hv = h[(int64_t)j * N + e], tv = t[(int64_t)j * N + e];
+        sc += om[j] * hv / (1.0 + tv + (sigma + tau[j]) * hv);
+      }
+    }
+    if (dead && dead[e]) sc = -INFINITY;
+    score[e] = sc;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Top-B in the order (score desc, index asc): pass b selects the maximum among elements that come
+// strictly after the pass b-1 pick.  sel_s/sel_i: device slots of previous picks.
+struct Best {
+  double s;
+  int64_t i;
+};
+
+__device__ __forceinline__ bool better(double s1, int64_t i1, double s2, int64_t i2) {
+  // true if (s1, i1) precedes (s2, i2) in (score desc, index asc)
+  return (s1 > s2) || (s1 == s2 && i1 < i2);
+}
+
+__global__ __launch_bounds__(kBlock) void k_select_pass(const double* __restrict__ score, int64_t N, int pass,
+                                                        double* __restrict__ sel_s, int32_t* __restrict__ out,
+                                                        double* __restrict__ bs, int64_t* __restrict__ bi,
+                                                        unsigned* __restrict__ counter) {
+  const bool has_prev = pass > 0;
+  const double ps = has_prev ? sel_s[pass - 1] : 0.0;
+  const int64_t pi = has_prev ? (int64_t)out[pass - 1] : -1;
+  double best_s = -INFINITY;
+  int64_t best_i = -1;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (int64_t)gridDim.x * blockDim.x) {
+    const double s = score[e];
+    const bool after = !has_prev || better(ps, pi, s, e);
+    if (after && (best_i < 0 || better(s, e, best_s, best_i))) {
+      best_s = s;
+      best_i = e;
+    }
+  }
+  __shared__ double ss[kBlock];
+  __shared__ int64_t si[kBlock];
+  ss[threadIdx.x] = best_s;
+  si[threadIdx.x] = best_i;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const int64_t i2 = si[threadIdx.x + o];
+      if (i2 >= 0 && (si[threadIdx.x] < 0 || better(ss[threadIdx.x + o], i2, ss[threadIdx.x], si[threadIdx.x]))) {
+        ss[threadIdx.x] = ss[threadIdx.x + o];
+        si[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    bs[blockIdx.x] = ss[0];
+    bi[blockIdx.x] = si[0];
+  }
+  __threadfence();
+  __syncthreads();
+  __shared__ int last;
+  if (threadIdx.x == 0) last = (atomicAdd(counter, 1u) == gridDim.x - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x == 0) {
+    double b_s = -INFINITY;
+    int64_t b_i = -1;
+    for (unsigned b = 0; b < gridDim.x; ++b) {
+      const int64_t i2 = bi[b];
+      if (i2 >= 0 && (b_i < 0 || better(bs[b], i2, b_s, b_i))) {
+        b_s = bs[b];
+        b_i = i2;
+      }
+    }
+    sel_s[pass] = b_s;
+    out[pass] = (int32_t)b_i;
+    *counter = 0;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void k_pair_rhs(int Br, const int32_t* __restrict__ a, const int32_t* __restrict__ b, T* __restrict__ W) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < Br) {
+    W[(int64_t)a[c] * Br + c] = T(1);
+    W[(int64_t)b[c] * Br + c] = T(-1);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rerank(int obj, const T* __restrict__ U, int64_t u_sj, int64_t u_si,
+                                                   int p, int Br, const int32_t* __restrict__ sa,
+                                                   const int32_t* __restrict__ sb, const double* __restrict__ hh,
+                                                   const double* __restrict__ om, double* __restrict__ g,
+                                                   double* __restrict__ res) {
+  for (int c = threadIdx.x; c < Br; c += blockDim.x) {
+    double acc = 0.0;
+    if (obj == 2) {
+      const double bu = (double)U[(int64_t)sa[c] * u_si + c] - (double)U[(int64_t)sb[c] * u_si + c];
+      acc = log1p(bu);
+    } else {
+      for (int j = 0; j < p; ++j) {
+        const T* Uj = U + (int64_t)j * u_sj;
+        const double bu = (double)Uj[(int64_t)sa[c] * u_si + c] - (double)Uj[(int64_t)sb[c] * u_si + c];
+        acc += om[j] * hh[(int64_t)j * Br + c] / (1.0 + bu);
+      }
+    }
+    g[c] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int best = 0;
+    for (int c = 1; c < Br; ++c)
+      if (g[c] > g[best]) best = c;
+    res[0] = (double)best;
+    res[1] = g[best];
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Maintenance.  coef per pole j: [xi(q) | eta(q) | D | nxi | neta]
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_maint_prep(const T* __restrict__ X, const T* __restrict__ Y, int64_t sj,
+                                                       int64_t si, int64_t q, const T* __restrict__ u, int64_t u_sj,
+                                                       int64_t u_si, int aa, int bb, const double* __restrict__ gc,
+                                                       double* __restrict__ coef) {
+  const int j = blockIdx.x;
+  const int64_t stride = 2 * q + 3;
+  double* cj = coef + (int64_t)j * stride;
+  const T* Xa = X + (int64_t)j * sj + (int64_t)aa * si;
+  const T* Xb = X + (int64_t)j * sj + (int64_t)bb * si;
+  const T* Ya = Y + (int64_t)j * sj + (int64_t)aa * si;
+  const T* Yb = Y + (int64_t)j * sj + (int64_t)bb * si;
+  __shared__ double sx[kBlock], se[kBlock];
+  double ax = 0.0, ae = 0.0;
+  for (int64_t c = threadIdx.x; c < q; c += blockDim.x) {
+    const double xi = (double)Xa[c] - (double)Xb[c];
+    const double eta = gc[c] - ((double)Ya[c] - (double)Yb[c]);
+    cj[c] = xi;
+    cj[q + c] = eta;
+    ax += xi * xi;
+    ae += eta * eta;
+  }
+  sx[threadIdx.x] = ax;
+  se[threadIdx.x] = ae;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tx = 0.0, te = 0.0;
+    for (int k = 0; k < (int)blockDim.x; ++k) {
+      tx += sx[k];
+      te += se[k];
+    }
+    const T* uj = u + (int64_t)j * u_sj;
+    cj[2 * q] = 1.0 + ((double)uj[(int64_t)aa * u_si] - (double)uj[(int64_t)bb * u_si]);
+    cj[2 * q + 1] = tx;
+    cj[2 * q + 2] = te;
+  }
+}
+
+// One pass over the old rows: w = row . coef, then row -/+= u * coef / D (X with xi, Y with eta).
+template <typename T, int CPL>
+__global__ __launch_bounds__(kBlock) void k_maint_rank1(T* __restrict__ X, T* __restrict__ Y, int64_t sj, int64_t si,
+                                                        int p, int64_t nA, int64_t q, const T* __restrict__ u,
+                                                        int64_t u_sj, int64_t u_si, const double* __restrict__ coef,
+                                                        double* __restrict__ wX, double* __restrict__ wY, int L) {
+  Geo g(L);
+  const int c0 = g.li * CPL;
+  const int64_t gpb = (int64_t)kWaves * g.G;
+  const int64_t total = (int64_t)p * nA;
+  const int64_t stride = 2 * q + 3;
+  for (int64_t item = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi; item < total;
+       item += (int64_t)gridDim.x * gpb) {
+    const int j = (int)(item / nA);
+    const int64_t i = item - (int64_t)j * nA;
+    const double* cj = coef + (int64_t)j * stride;
+    const double D = cj[2 * q];
+    const double ui = (double)u[(int64_t)j * u_sj + i * u_si];
+    T* xr = X + (int64_t)j * sj + i * si;
+    T* yr = Y + (int64_t)j * sj + i * si;
+    double sx = 0.0, sy = 0.0;
+    const bool ok = c0 < q;
+    T xv[CPL], yv[CPL];
+    if (ok) {
+      if (c0 + CPL <= q) {
+        vload<T, CPL>(xr + c0, xv);
+        vload<T, CPL>(yr + c0, yv);
+      } else {
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) {
+          xv[e] = (c0 + e < q) ? xr[c0 + e] : T(0);
+          yv[e] = (c0 + e < q) ? yr[c0 + e] : T(0);
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        if (c0 + e < q) {
+          sx += (double)xv[e] * cj[c0 + e];
+          sy += (double)yv[e] * cj[q + c0 + e];
+        }
+      }
+    }
+    sx = group_sum(sx, L);
+    sy = group_sum(sy, L);
+    if (g.li == 0) {
+      wX[item] = sx;
+      wY[item] = sy;
+    }
+    if (ok) {
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        if (c0 + e < q) {
+          xv[e] = (T)((double)xv[e] - ui * cj[c0 + e] / D);
+          yv[e] = (T)((double)yv[e] + ui * cj[q + c0 + e] / D);
+        }
+      }
+      if (c0 + CPL <= q) {
+        vstore<T, CPL>(xr + c0, xv);
+        vstore<T, CPL>(yr + c0, yv);
+      } else {
+#pragma unroll
+        for (int e = 0; e < CPL; ++e)
+          if (c0 + e < q) {
+            xr[c0 + e] = xv[e];
+            yr[c0 + e] = yv[e];
+          }
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_maint_cache(const T* __restrict__ u, int64_t u_sj, int64_t u_si, int p,
+                                                        int64_t nA, int64_t N, const int32_t* __restrict__ ea,
+                                                        const int32_t* __restrict__ eb, const double* __restrict__ coef,
+                                                        int64_t q, const double* __restrict__ wX,
+                                                        const double* __restrict__ wY, double* __restrict__ h,
+                                                        double* __restrict__ t) {
+  const int64_t total = (int64_t)p * N;
+  const int64_t stride = 2 * q + 3;
+  for (int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; item < total;
+       item += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(item / N);
+    const int64_t e = item - (int64_t)j * N;
+    const double* cj = coef + (int64_t)j * stride;
+    const double D = cj[2 * q], nxi = cj[2 * q + 1], neta = cj[2 * q + 2];
+    const int ia = ea[e], ib = eb[e];
+    const T* uj = u + (int64_t)j * u_sj;
+    const double cc = ((double)uj[(int64_t)ia * u_si] - (double)uj[(int64_t)ib * u_si]) / D;
+    const double dwx = wX[(int64_t)j * nA + ia] - wX[(int64_t)j * nA + ib];
+    const double dwy = wY[(int64_t)j * nA + ia] - wY[(int64_t)j * nA + ib];
+    const double hn = h[item] - 2.0 * cc * dwx + cc * cc * nxi;
+    const double tn = t[item] + 2.0 * cc * dwy + cc * cc * neta;
+    h[item] = hn > 0.0 ? hn : 0.0;
+    t[item] = tn > 0.0 ? tn : 0.0;
+  }
+}
+
+template <typename T>
+__global__ void k_rows_axpy(T* __restrict__ F, int64_t ldf, int64_t q, int a, int b, const double* __restrict__ g) {
+  for (int64_t c = blockIdx.x * blockDim.x + threadIdx.x; c < q; c += (int64_t)gridDim.x * blockDim.x) {
+    F[(int64_t)a * ldf + c] = (T)((double)F[(int64_t)a * ldf + c] + g[c]);
+    F[(int64_t)b * ldf + c] = (T)((double)F[(int64_t)b * ldf + c] - g[c]);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Counter-hash Rademacher generator (splitmix64 finaliser).
+__device__ __host__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rademacher(uint64_t base, const int64_t* __restrict__ keys, int64_t rows,
+                                                       int64_t q, double scale, T* __restrict__ out, int64_t ld) {
+  const int64_t total = rows * q;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx / q, c = idx - i * q;
+    const uint64_t key = keys ? (uint64_t)keys[i] : (uint64_t)i;
+    const uint64_t hsh = splitmix64(base + key * (uint64_t)q + (uint64_t)c);
+    out[i * ld + c] = (T)((hsh >> 63) ? -scale : scale);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_edge_sketch(polar_op op, uint64_t base, int64_t q, double scale,
+                                                        T* __restrict__ F, int64_t ldf) {
+  // one thread per (row, column); loops over the row's neighbours in CSR order
+  const int64_t total = op.n * q;
+  const T* wt = (const T*)op.weights;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx / q, c = idx - i * q;
+    double acc = 0.0;
+    for (int k = op.indptr[i]; k < op.indptr[i + 1]; ++k) {
+      const int64_t j = op.indices[k];
+      const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+      const uint64_t hsh = splitmix64(base + (uint64_t)(lo * op.n + hi) * (uint64_t)q + (uint64_t)c);
+      const double w = op.weighted ? sqrt((double)wt[k]) : 1.0;
+      const double sgn = (i < j) ? 1.0 : -1.0;
+      acc += sgn * w * ((hsh >> 63) ? -scale : scale);
+    }
+    F[i * ldf + c] = (T)acc;
+  }
+}
+
+static int grid_for(int64_t items, int per_block, int cap = 65536) {
+  return (int)std::min<int64_t>(cap, std::max<int64_t>(1, (items + per_block - 1) / per_block));
+}
+
+static int pick_cpl_rows(int dtype, int64_t q, const void* base, int64_t sj, int64_t si) {
+  const int esz = elem_size(dtype);
+  for (int cpl = 16 / esz; cpl > 1; cpl /= 2) {
+    if (q % cpl == 0 && ((uintptr_t)base % (cpl * esz)) == 0 && (sj * esz) % (cpl * esz) == 0 &&
+        (si * esz) % (cpl * esz) == 0 && (q + cpl - 1) / cpl <= 64)
+      return cpl;
+  }
+  return 1;
+}
+
+}  // namespace polar
+
+using namespace polar;
+
+extern "C" {
+
+int polar_pair_sqnorm(int32_t dtype, const void* X, int64_t x_sj, int64_t x_si, int32_t p, int64_t q,
+                      const int32_t* ea, const int32_t* eb, int64_t N, double* out, void* stream) {
+  if (!X || !ea || !eb || !out || p <= 0 || q <= 0 || N < 0) return set_error(POLAR_E_ARG, "bad pair_sqnorm args");
+  if (N == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  int cpl = pick_cpl_rows(dtype, q, X, x_sj, x_si);
+  int L = lanes_for(q, cpl);
+  if (L > 64) {
+    cpl = (dtype == POLAR_F64) ? 2 : 4;
+    L = 64;
+    if ((q + cpl - 1) / cpl > 64) return set_error(POLAR_E_ARG, "q=%lld too wide", (long long)q);
+    return set_error(POLAR_E_ARG, "q=%lld needs aligned 16-byte rows", (long long)q);
+  }
+  const int gpb = kWaves * (64 / L);
+  const int grid = grid_for((int64_t)p * N, gpb);
+  if (dtype == POLAR_F64) {
+    if (cpl == 2) hipLaunchKernelGGL((k_pair_sqnorm<double, 2>), dim3(grid), dim3(kBlock), 0, st, (const double*)X, x_sj, x_si, p, q, ea, eb, N, out, L);
+    else hipLaunchKernelGGL((k_pair_sqnorm<double, 1>), dim3(grid), dim3(kBlock), 0, st, (const double*)X, x_sj, x_si, p, q, ea, eb, N, out, L);
+  } else {
+    if (cpl == 4) hipLaunchKernelGGL((k_pair_sqnorm<float, 4>), dim3(grid), dim3(kBlock), 0, st, (const float*)X, x_sj, x_si, p, q, ea, eb, N, out, L);
+    else if (cpl == 2) hipLaunchKernelGGL((k_pair_sqnorm<float, 2>), dim3(grid), dim3(kBlock), 0, st, (const float*)X, x_sj, x_si, p, q, ea, eb, N, out, L);
+    else hipLaunchKernelGGL((k_pair_sqnorm<float, 1>), dim3(grid), dim3(kBlock), 0, st, (const float*)X, x_sj, x_si, p, q, ea, eb, N, out, L);
+  }
+  POLAR_LAUNCH_CHECK();
+  return 0;
+}
+
+int polar_proxy(int32_t obj, const double* h, const double* t, const double* tau, const double* om, int32_t p,
+                int64_t N, double sigma, const uint8_t* dead, double* score, void* stream) {
+  if (!h || !t || !tau || !om || !score || p <= 0) return set_error(POLAR_E_ARG, "bad proxy args");
+  if (N == 0) return 0;
+  hipLaunchKernelGGL(k_proxy, dim3(grid_for(N, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, obj, h, t, tau,
+                     om, p, N, sigma, dead, score);
+  POLAR_LAUNCH_CHECK();
+  return 0;
+}
+
+int64_t polar_select_ws_bytes(int64_t N) {
+  (void)N;
+  return (int64_t)kMaxGrid * 16 + 64 * 8 + 256 + 512;
+}
+
+int polar_select_topb(const double* score, int64_t N, int32_t Br, int32_t* out, int32_t* host_out, void* ws,
+                      int64_t ws_bytes, void* stream) {
+  if (!score || !out || Br <= 0 || Br > 64 || Br > N || !ws) return set_error(POLAR_E_ARG, "bad select args (Br=%d N=%lld)", Br, (long long)N);
+  if (ws_bytes < polar_select_ws_bytes(N)) return set_error(POLAR_E_ARG, "select workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  char* w = (char*)ws;
+  double* bs = (double*)w;
+  int64_t* bi = (int64_t*)(w + kMaxGrid * 8);
+  double* sel_s = (double*)(w + kMaxGrid * 16);
+  unsigned* counter = (unsigned*)(w + kMaxGrid * 16 + 64 * 8);
+  POLAR_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned), st));
+  const int grid = grid_for(N, kBlock, kMaxGrid);
+  for (int b = 0; b < Br; ++b) {
+    hipLaunchKernelGGL(k_select_pass, dim3(grid), dim3(kBlock), 0, st, score, N, b, sel_s, out, bs, bi, counter);
+    POLAR_LAUNCH_CHECK();
+  }
+  if (host_out) {
+    POLAR_HIP(hipMemcpyAsync(host_out, out, sizeof(int32_t) * Br, hipMemcpyDeviceToHost, st));
+    POLAR_HIP(hipStreamSynchronize(st));
+  }
+  return 0;
+}
+
+int polar_pair_rhs(int32_t dtype, int64_t n, int32_t Br, const int32_t* a, const int32_t* b, void* W, void* stream) {
+  if (!a || !b || !W || Br <= 0 || n <= 0) return set_error(POLAR_E_ARG, "bad pair_rhs args");
+  hipStream_t st = (hipStream_t)stream;
+  POLAR_HIP(hipMemsetAsync(W, 0, (size_t)n * Br * elem_size(dtype), st));
+  if (dtype == POLAR_F64) hipLaunchKernelGGL(k_pair_rhs<double>, dim3((Br + 63) / 64), dim3(64), 0, st, Br, a, b, (double*)W);
+  else hipLaunchKernelGGL(k_pair_rhs<float>, dim3((Br + 63) / 64), dim3(64), 0, st, Br, a, b, (float*)W);
+  POLAR_LAUNCH_CHECK();
+  return 0;
+}
+
+int polar_rerank_pick(int32_t obj, int32_t dtype, const void* U, int64_t u_sj, int64_t u_si, int32_t p, int32_t Br,
+                      const int32_t* sa, const int32_t* sb, const double* hh, const double* om, double* g,
+                      int32_t* host_winner, double* host_value, void* stream) {
+  if (!U || !sa || !sb || !om || !g || Br <= 0 || p <= 0) return set_error(POLAR_E_ARG, "bad rerank args");
+  if (obj != 2 && !hh) return set_error(POLAR_E_ARG, "rerank needs tracked norms");
+  hipStream_t st = (hipStream_t)stream;
+  double* res = g + Br;  // caller provides Br + 2 doubles
+  if (dtype == POLAR_F64)
+    hipLaunchKernelGGL(k_rerank<double>, dim3(1), dim3(kBlock), 0, st, obj, (const double*)U, u_sj, u_si, p, Br, sa, sb, hh, om, g, res);
+  else
+    hipLaunchKernelGGL(k_rerank<float>, dim3(1), dim3(kBlock), 0, st, obj, (const float*)U, u_sj, u_si, p, Br, sa, sb, hh, om, g, res);
+  POLAR_LAUNCH_CHECK();
+  double hr[2];
+  POLAR_HIP(hipMemcpyAsync(hr, res, sizeof(hr), hipMemcpyDeviceToHost, st));
+  POLAR_HIP(hipStreamSynchronize(st));
+  if (host_winner) *host_winner = (int32_t)hr[0];
+  if (host_value) *host_value = hr[1];
+  return 0;
+}
+
+int polar_maint_prep(int32_t dtype, const void* X, const void* Y, int64_t sj, int64_t si, int32_t p, int64_t q,
+                     const void* u, int64_t u_sj, int64_t u_si, int32_t aa, int32_t bb, const double* gc,
+                     double* coef, void* stream) {
+  if (!X || !Y || !u || !gc || !coef || p <= 0 || q <= 0) return set_error(POLAR_E_ARG, "bad maint_prep args");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == POLAR_F64)
+    hipLaunchKernelGGL(k_maint_prep<double>, dim3(p), dim3(kBlock), 0, st, (const double*)X, (const double*)Y, sj, si, q, (const double*)u, u_sj, u_si, aa, bb, gc, coef);
+  else
+    hipLaunchKernelGGL(k_maint_prep<float>, dim3(p), dim3(kBlock), 0, st, (const float*)X, (const float*)Y, sj, si, q, (const float*)u, u_sj, u_si, aa, bb, gc, coef);
+  POLAR_LAUNCH_CHECK();
+  return 0;
+}
+
+int polar_maint_rank1(int32_t dtype, void* X, void* Y, int64_t sj, int64_t si, int32_t p, int64_t nA, int64_t q,
+                      const void* u, int64_t u_sj, int64_t u_si, const double* coef, double* wX, double* wY,
+                      void* stream) {
+  if (!X || !Y || !u || !coef || !wX || !wY || p <= 0 || q <= 0) return set_error(POLAR_E_ARG, "bad maint_rank1 args");
+  hipStream_t st = (hipStream_t)stream;
+  int cpl = std::min(pick_cpl_rows(dtype, q, X, sj, si), pick_cpl_rows(dtype, q, Y, sj, si));
+  int L = lanes_for(q, cpl);
+  if (L > 64) return set_error(POLAR_E_ARG, "q=%lld too wide for maint_rank1 at this alignment", (long long)q);
+  const int gpb = kWaves * (64 / L);
+  const int grid = grid_for((int64_t)p * nA, gpb);
+  if (dtype == POLAR_F64) {
+    if (cpl == 2) hipLaunchKernelGGL((k_maint_rank1<double, 2>), dim3(grid), dim3(kBlock), 0, st, (double*)X, (double*)Y, sj, si, p, nA, q, (const double*)u, u_sj, u_si, coef, wX, wY, L);
+    else hipLaunchKernelGGL((k_maint_rank1<double, 1>), dim3(grid), dim3(kBlock), 0, st, (double*)X, (double*)Y, sj, si, p, nA, q, (const double*)u, u_sj, u_si, coef, wX, wY, L);
+  } else {
+    if (cpl == 4) hipLaunchKernelGGL((k_maint_rank1<float, 4>), dim3(grid), dim3(kBlock), 0, st, (float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, L);
+    else if (cpl == 2) hipLaunchKernelGGL((k_maint_rank1<float, 2>), dim3(grid), dim3(kBlock), 0, st, (float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, L);
+    else hipLaunchKernelGGL((k_maint_rank1<float, 1>), dim3(grid), dim3(kBlock), 0, st, (float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, L);
+  }
+  POLAR_LAUNCH_CHECK();
+  return 0;
+}
+
+int polar_maint_cache(int32_t dtype, const void* u, int64_t u_sj, int64_t u_si, int32_t p, int64_t nA, int64_t N,
+                      const int32_t* ea, const int32_t* eb, const double* coef, int64_t q, const double* wX,
+                      const double* wY, double* h, double* t, void* stream) {
+  if (!u || !ea || !eb || !coef || !wX || !wY || !h || !t || p <= 0) return set_error(POLAR_E_ARG, "bad maint_cache args");
+  if (N == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  const int grid = grid_for((int64_t)p * N, kBlock);
+  if (dtype == POLAR_F64)
+    hipLaunchKernelGGL(k_maint_cache<double>, dim3(grid), dim3(kBlock), 0, st, (const double*)u, u_sj, u_si, p, nA, N, ea, eb, coef, q, wX, wY, h, t);
+  else
+    hipLaunchKernelGGL(k_maint_cache<float>, dim3(grid), dim3(kBlock), 0, st, (const float*)u, u_sj, u_si, p, nA, N, ea, eb, coef, q, wX, wY, h, t);
+  POLAR_LAUNCH_CHECK();
+  return 0;
+}
+
+int polar_rows_axpy(int32_t dtype, void* F, int64_t ldf, int64_t q, int32_t a, int32_t b, const double* g, void* stream) {
+  if (!F || !g || q <= 0) return set_error(POLAR_E_ARG, "bad rows_axpy args");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == POLAR_F64) hipLaunchKernelGGL(k_rows_axpy<double>, dim3(1), dim3(256), 0, st, (double*)F, ldf, q, a, b, g);
+  else hipLaunchKernelGGL(k_rows_axpy<float>, dim3(1), dim3(256), 0, st, (float*)F, ldf, q, a, b, g);
+  POLAR_LAUNCH_CHECK();
+  return 0;
+}
+
+int polar_rademacher(int32_t dtype, uint64_t seed, int32_t stream_id, const int64_t* keys, int64_t rows, int64_t q,
+                     double scale, void* out, int64_t ld, void* stream) {
+  if (!out || rows < 0 || q <= 0 || ld < q) return set_error(POLAR_E_ARG, "bad rademacher args");
+  if (rows == 0) return 0;
+  const uint64_t base = splitmix64(seed * 4 + (uint64_t)stream_id);
+  hipStream_t st = (hipStream_t)stream;
+  const int grid = grid_for(rows * q, kBlock);
+  if (dtype == POLAR_F64) hipLaunchKernelGGL(k_rademacher<double>, dim3(grid), dim3(kBlock), 0, st, base, keys, rows, q, scale, (double*)out, ld);
+  else hipLaunchKernelGGL(k_rademacher<float>, dim3(grid), dim3(kBlock), 0, st, base, keys, rows, q, scale, (float*)out, ld);
+  POLAR_LAUNCH_CHECK();
+  return 0;
+}
+
+int polar_edge_sketch(const polar_op* op, uint64_t seed, int64_t q, double scale, void* F, int64_t ldf, void* stream) {
+  if (!op || !F || q <= 0 || ldf < q) return set_error(POLAR_E_ARG, "bad edge_sketch args");
+  const uint64_t base = splitmix64(seed * 4 + 1);
+  hipStream_t st = (hipStream_t)stream;
+  const int grid = grid_for(op->n * q, kBlock);
+  if (op->dtype == POLAR_F64) hipLaunchKernelGGL(k_edge_sketch<double>, dim3(grid), dim3(kBlock), 0, st, *op, base, q, scale, (double*)F, ldf);
+  else hipLaunchKernelGGL(k_edge_sketch<float>, dim3(grid), dim3(kBlock), 0, st, *op, base, q, scale, (float*)F, ldf);
+  POLAR_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // extern "C"
