@@ -1,0 +1,230 @@
+#!/usr/bin/env python
+"""Benchmark of the pole-shared Krylov core on one MI355X.
+
+Metric (k-round greedy wall time + SpMM GB/s at n=4M; FP32-mixed vs FP64 speedup):
+  * step = one full POLAR greedy workload of the c2 configuration: Chung-Lu n=100,000 (gamma 2.5,
+    mean degree 20, seed 1), sigma=0.05, q=64 Rademacher, 24 poles, N=1,000,000 candidates,
+    k=20 rounds, B=8, for each s in {0.25, 0.5, 0.75}; inputs resident in HBM before timing.
+    `value` = wall seconds of one step in the FP32-mixed form (higher_is_better: false).
+  * the plain FP64 form is timed on the same inputs (fp64_seconds; speedup = fp64/fp32) and its
+    selections and marginals are compared with the FP32-mixed ones.
+  * SpMM (L + sigma I) X at the c5 shape (Chung-Lu n=4,000,000, m=34.7M, R=128) in FP64 and FP32,
+    GB/s on compulsory bytes.
+  * roofline: dominant CG kernel over the timed region, HIP-event durations from the library.
+  * cpu_baseline: the oracle (NumPy/SciPy FP64, one core) on a bounded sample at the c2 shape,
+    extrapolated to the step's column-iterations.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'k-round greedy wall time + SpMM GB/s at n=4M; FP32-mixed vs FP64 speedup'
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print('[bench]', *a, file=sys.stderr, flush=True)
+
+
+def run_step(pg, G, inp, sk, precision):
+    outs = []
+    for s in inp['s_all']:
+        tau, om, _ = pg.fixed_rule(s, inp['p'], inp['sigma'], G.Lam(inp['k']))
+        outs.append(pg.polar(G, inp['cand'], inp['k'], s, q=inp['q'], B=inp['B'], tol=inp['tol'], sketch=sk,
+                             poles=(tau, om), precision=precision))
+    return outs
+
+
+def spmm_bench(pg, torch, lib, L, n_scale, reps=10):
+    from polar_amd import configs, graphs
+    spec = configs.SPECS['c5']
+    n = int(spec['n'] * n_scale)
+    t = time.time()
+    e = graphs.chung_lu(n, spec['gamma'], spec['mean_deg'], spec['seed'], m=int(spec['m'] * n_scale))
+    log(f'c5 graph n={n} m={len(e)} built in {time.time() - t:.1f}s')
+    G = pg.Graph(n, e, spec['sigma'])
+    R = 128
+    res = {}
+    for prec, dt, esz in (('fp64', torch.float64, 8), ('fp32', torch.float32, 4)):
+        op = pg.Op(G, precision=prec)
+        X = torch.randn((n, R), dtype=dt, device='cuda')
+        Y = torch.empty_like(X)
+        s = op.c_op(dt)
+        ws = torch.empty(max(lib.polar_spmm_ws_bytes(s, R), 256), dtype=torch.uint8, device='cuda')
+        st = torch.cuda.current_stream()
+
+        def go():
+            L.check(lib.polar_spmm(s, 0.0, L.ptr(X), R, L.ptr(Y), R, R, L.ptr(ws), ws.numel(), L.stream_ptr()))
+        go()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            go()
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        nnz = G.nnz
+        ours = 4 * (n + 1) + 4 * nnz + esz * n + 2 * n * R * esz            # value-free format
+        refb = 4 * (n + 1) + (n + nnz) * (4 + esz) + 2 * n * R * esz        # CSR with diagonal
+        res[prec] = dict(ms=ms, gbps=ours / ms / 1e6, bytes=ours, gbps_ref_format=refb / ms / 1e6,
+                         frac=ours / ms / 1e6 / HBM_PEAK_GBPS)
+        del X, Y, ws
+        torch.cuda.empty_cache()
+    return dict(n=n, m=len(e), nnz_offdiag=G.nnz, R=R, dmax=G.dmax, **res)
+
+
+def cpu_baseline(inp, step_colits, seconds_budget=20.0):
+    """Oracle msolve at the c2 shape on an 8-column block, bounded by iteration count; the cost
+    per column-iteration is scaled by the GPU step's column-iterations."""
+    from oracle import polar_oracle as O
+    Go = O.Graph(inp['n'], inp['edges'], inp['sigma'])
+    Z, F, _ = O.sketches(Go, inp['cand'][:16], inp['q'], inp['seed'], kind='rademacher')
+    active = np.unique(inp['cand'])
+    tau, om, _ = O.fixed_rule(0.5, inp['p'], inp['sigma'], Go.Lam(inp['k']))
+    W = Z[:, :8]
+    its = 8
+    t0 = time.perf_counter()
+    while True:
+        try:
+            O.msolve(O.Op(Go), W, tau, active, inp['tol'], maxit=its)
+        except RuntimeError:
+            pass
+        el = time.perf_counter() - t0
+        if el > seconds_budget / 4 or its >= 4096:
+            break
+        its *= 2
+        t0 = time.perf_counter()
+    per_colit = el / (its * W.shape[1])
+    return dict(value=per_colit * step_colits, unit='s', cores=1, kind='port',
+                sample=f'oracle msolve on 8 columns x {its} CG iterations at the c2 shape '
+                       f'(n={inp["n"]}, p={inp["p"]}, nA={len(active)}): {per_colit * 1e3:.3f} ms per '
+                       f'column-iteration, scaled by the step\'s {step_colits} column-iterations',
+                cpu_model=_cpu_model(), os_cpu_count=os.cpu_count(),
+                affinity=len(os.sched_getaffinity(0)))
+
+
+def _cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=1)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--config', default='c2')
+    ap.add_argument('--scale', type=float, default=1.0)
+    ap.add_argument('--skip-fp64', action='store_true')
+    ap.add_argument('--skip-cpu', action='store_true')
+    ap.add_argument('--skip-spmm', action='store_true')
+    ap.add_argument('--spmm-scale', type=float, default=1.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f'bench.py measures one GPU only (asked for --gpus {a.gpus}); the pole-sharded multi-GPU '
+              'layout is a design note, not built.', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from polar_amd import _lib as L
+    from polar_amd import configs
+    from polar_amd import polar_gpu as pg
+    lib = L.require_gpu()
+    torch.cuda.set_device(0)
+
+    t = time.time()
+    inp = configs.make(a.config, a.scale)
+    G = pg.Graph(inp['n'], inp['edges'], inp['sigma'])
+    sk = pg.sketches(G, inp['cand'], inp['q'], inp['seed'], kind='rademacher', on_device=True, precision='fp64')
+    torch.cuda.synchronize()
+    log(f'{a.config}: n={inp["n"]} m={inp["m"]} N={inp["N"]} nA={len(np.unique(inp["cand"]))} '
+        f'dmax={inp["dmax"]} inputs in {time.time() - t:.1f}s')
+
+    for w in range(a.warmup):
+        run_step(pg, G, inp, sk, 'fp32')
+        log(f'warmup {w} done')
+    lib.polar_prof_set(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    outs32 = None
+    for i in range(a.steps):
+        outs32 = run_step(pg, G, inp, sk, 'fp32')
+        log(f'step {i} done')
+    torch.cuda.synchronize()
+    el32 = (time.perf_counter() - t0) / a.steps
+    prof = {}
+    for cls, name in enumerate(['opmul', 'resid', 'update', 'solves']):
+        ms, la, by = (np.zeros(1), np.zeros(1, np.int64), np.zeros(1))
+        lib.polar_prof_get(cls, ms.ctypes.data_as(L.vp), la.ctypes.data_as(L.vp), by.ctypes.data_as(L.vp))
+        prof[name] = dict(ms=float(ms[0]), launches=int(la[0]), bytes=float(by[0]))
+    lib.polar_prof_set(0)
+    log('fp32 step %.3fs' % el32, json.dumps(prof))
+
+    el64 = None
+    agree = None
+    if not a.skip_fp64:
+        run_step(pg, G, inp, sk, 'fp64') if a.warmup else None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outs64 = run_step(pg, G, inp, sk, 'fp64')
+        torch.cuda.synchronize()
+        el64 = time.perf_counter() - t0
+        same = all(o32['sequence'] == o64['sequence'] for o32, o64 in zip(outs32, outs64))
+        mrel = max(float(np.max(np.abs(np.array(o32['marginals']) - o64['marginals']) / np.abs(o64['marginals'])))
+                   for o32, o64 in zip(outs32, outs64))
+        trel = max(float(abs(o32['trace_estimate'][0] - o64['trace_estimate'][0]) / abs(o64['trace_estimate'][0]))
+                   for o32, o64 in zip(outs32, outs64))
+        agree = dict(selections_identical=same, max_rel_marginal_err=mrel, max_rel_trace_err=trel)
+        log('fp64 step %.3fs' % el64, json.dumps(agree))
+
+    spmm = None
+    if not a.skip_spmm:
+        spmm = spmm_bench(pg, torch, lib, L, a.spmm_scale)
+        log('spmm', json.dumps(spmm))
+
+    colits = int(sum(o['spmv'] for o in outs32))
+    dom = max(('opmul', 'resid', 'update'), key=lambda k: prof[k]['ms'])
+    d = prof[dom]
+    achieved = d['bytes'] / (d['ms'] * 1e-3) / 1e9 if d['ms'] > 0 else 0.0
+    roofline = dict(kernel={'opmul': 'k_opmul', 'resid': 'k_resid', 'update': 'k_update'}[dom], bound='hbm',
+                    achieved=achieved, peak=HBM_PEAK_GBPS, unit='GB/s', frac=achieved / HBM_PEAK_GBPS,
+                    traffic=None, avg_launch_us=d['ms'] * 1e3 / max(d['launches'], 1),
+                    algorithmic_bytes_per_launch=d['bytes'] / max(d['launches'], 1),
+                    share_of_step=d['ms'] * 1e-3 / el32 / a.steps * a.steps)
+    cpu = None
+    if not a.skip_cpu:
+        cpu = cpu_baseline(inp, colits)
+        log('cpu', json.dumps(cpu))
+    out = dict(metric=METRIC, value=el32, unit='s', n_gpus=1, steps=a.steps, warmup=a.warmup,
+               ms_per_step=el32 * 1e3, higher_is_better=False, scaling='weak', vs_baseline=None,
+               dtype='fp32 vectors / fp64 recurrences', data='synthetic',
+               config=dict(workload=f'{a.config}: Chung-Lu n={inp["n"]}, m={inp["m"]}, sigma={inp["sigma"]}, '
+                                    f's={list(inp["s_all"])}, q={inp["q"]}, p={inp["p"]}, N={inp["N"]}, '
+                                    f'k={inp["k"]}, B={inp["B"]}, tol={inp["tol"]}',
+                           precision='fp32-mixed', column_iterations=colits,
+                           rhs=int(sum(o['rhs'] for o in outs32))),
+               fp64_seconds=el64, fp32_vs_fp64_speedup=(el64 / el32) if el64 else None,
+               fp32_vs_fp64=agree, spmm_n4m=spmm, roofline=roofline, cpu_baseline=cpu,
+               kernel_profile=prof)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -78,6 +78,14 @@ typedef struct polar_op {
 int polar_abi_version(void);
 const char* polar_last_error(void);
 
+/* Live profiling of the CG iteration kernels with HIP events (no counterpart in the reference;
+ * used by the benchmark).  polar_prof_set(1) resets and enables it.  Class 0: operator product,
+ * 1: residual update, 2: direction + shifted x/p update; per class the summed event time (ms),
+ * launches that did work, and algorithmic HBM bytes.  Class 3: solves, and (bytes) the number of
+ * live (pole, column) pairs summed over iterations. */
+int polar_prof_set(int32_t on);
+int polar_prof_get(int32_t cls, double* ms, int64_t* launches, double* bytes);
+
 /* Y[:, :R] = (M0 + shift I + sum_S b b^T) X[:, :R]; X, Y row-major with leading dims ldx, ldy. */
 int64_t polar_spmm_ws_bytes(const polar_op* op, int64_t R);
 int polar_spmm(const polar_op* op, double shift, const void* X, int64_t ldx, void* Y, int64_t ldy,

@@ -11,6 +11,14 @@ It restates, in NumPy/SciPy FP64, the semantics of the reference's ``realworld/p
   of ``synthetic/scalability/benchmark.py:72-87`` (no pole freezing); ``trace=True`` adds a
   2-scalar per-pole tracker ``w^T (M+tau_j)^{-1} w`` (Gauss-quadrature identity of CG).
 * ``polar(..., poles=(tau, om))`` pole override and per-round marginal / trace-estimate outputs.
+* ``arith='cr'``: the same algorithm with a pinned arithmetic: every reduction (inner products,
+  squared norms, row dots) is summed in double-double and rounded once (order-independent), the
+  operator is applied as ((diag + shift) * x) - (sequential neighbour sum, rows longer than 256
+  split in 256-entry chunks added in order) + added-edge terms in addition order, no fused
+  multiply-add anywhere, and scalar sums over poles run in pole order.  The HIP path implements
+  exactly this arithmetic, so its FP64 results equal the oracle's bit for bit (up to ties of the
+  double-double rounding).  ``arith='numpy'`` (default) keeps NumPy's native reduction order,
+  which reproduces the reference's own NumPy runs (fixture test).
 
 Parity pins (what checks each function; tests/test_oracle.py):
 * composite      -- closed form sum w/(x+t) vs x^-s within 256*4^-r, pole counts p=(K+2)r vs the
@@ -95,6 +103,57 @@ def poles(obj, s, sigma, Lam, eps):
 
 
 # ------------------------------------------------------------------------------------------------
+# Pinned-arithmetic helpers (arith='cr').
+def ddsum(P, axis=0):
+    """Double-double pairwise sum along `axis`, rounded once: the correctly rounded sum of the
+    entries except in double-double rounding ties."""
+    hi = np.moveaxis(np.array(P, dtype=np.float64), axis, 0)
+    lo = np.zeros_like(hi)
+    while hi.shape[0] > 1:
+        if hi.shape[0] % 2:
+            z = np.zeros((1,) + hi.shape[1:])
+            hi = np.concatenate([hi, z])
+            lo = np.concatenate([lo, z])
+        ah, bh, al, bl = hi[0::2], hi[1::2], lo[0::2], lo[1::2]
+        s_ = ah + bh
+        bb = s_ - ah
+        e = (ah - (s_ - bb)) + (bh - bb)
+        e = e + (al + bl)
+        hi = s_ + e
+        lo = e - (hi - s_)
+    return hi[0] + lo[0]
+
+
+CHUNK = 256
+
+
+class _Adjacency:
+    """Off-diagonal adjacency in sorted CSR, for the pinned operator order."""
+
+    def __init__(self, G):
+        A = (sp.diags(G.M0.diagonal()) - G.M0).tocsr()
+        A.eliminate_zeros()
+        A.sort_indices()
+        self.A = A
+        self.diag = G.M0.diagonal().copy()
+        deg = np.diff(A.indptr)
+        self.long_rows = np.flatnonzero(deg > CHUNK)
+
+    def neighbour_sum(self, x):
+        acc = self.A @ x
+        ip, ix, w = self.A.indptr, self.A.indices, self.A.data
+        for r in self.long_rows:
+            tot = None
+            for c0 in range(ip[r], ip[r + 1], CHUNK):
+                part = np.zeros(x.shape[1:])
+                for k in range(c0, min(c0 + CHUNK, ip[r + 1])):
+                    part = part + (w[k] * x[ix[k]] if w[k] != 1.0 else x[ix[k]])
+                tot = part if tot is None else tot + part
+            acc[r] = tot
+        return acc
+
+
+# ------------------------------------------------------------------------------------------------
 class Graph:
     """M0 = diag(deg + sigma) - A as sorted CSR (polar_gpu.py:47-60)."""
 
@@ -121,16 +180,25 @@ class Graph:
 class Op:
     """y = (M0 + shift I + sum_S b b^T) x with unit-weight added edges (polar_gpu.py:63-80)."""
 
-    def __init__(self, G):
+    def __init__(self, G, arith='numpy'):
         self.M0 = G.M0
         self.A = []
         self.B = []
+        self.arith = arith
+        self._adj = _Adjacency(G) if arith == 'cr' else None
 
     def add(self, a, b):
         self.A.append(int(a))
         self.B.append(int(b))
 
     def __call__(self, x, shift):
+        if self.arith == 'cr':
+            xx = x if x.ndim == 2 else x[:, None]
+            y = (self._adj.diag + shift)[:, None] * xx - self._adj.neighbour_sum(xx)
+            for a, b in zip(self.A, self.B):
+                y[a] = y[a] + (xx[a] - xx[b])
+                y[b] = y[b] + (xx[b] - xx[a])
+            return y if x.ndim == 2 else y[:, 0]
         y = self.M0 @ x
         if shift:
             y = y + shift * x
@@ -154,6 +222,7 @@ class Counter:
 
 def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, freeze=True,
            trace=False):
+    colsum = ddsum if getattr(op, 'arith', 'numpy') == 'cr' else (lambda P: np.sum(P, axis=0))
     """Block multi-shift CG (Jegerlehner) for (M + tau_j I) U = W, all columns, all poles
     (polar_gpu.py:88-133).  Shifted iterates live only on the `active` rows.
 
@@ -170,7 +239,7 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
     shift = (tau - tau[0])[:, None]                                   # :92
     r = W.copy()
     d = r.copy()
-    rr = np.sum(r * r, axis=0)
+    rr = colsum(r * r)
     stop = tol * np.sqrt(rr)                                          # :93
     act = rr > 0                                                      # :94
     X = np.zeros((p, len(active), R))
@@ -191,10 +260,10 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
     with np.errstate(divide='ignore', invalid='ignore'):
         for it in range(maxit):                                       # :100
             v = op(d, float(tau[0]))
-            dv = np.sum(d * v, axis=0)
+            dv = colsum(d * v)
             a = np.where(act, rr / np.where(act, dv, 1.0), 0.0)
             rn = r - a * v
-            rrn = np.sum(rn * rn, axis=0)
+            rrn = colsum(rn * rn)
             be = np.where(act, rrn / np.where(act, rr, 1.0), 0.0)
             c = np.where(act, a * bp / ap, 0.0)                       # :107
             zn = z / (1 + a * shift + c * (1 - z / zp))               # :108
@@ -321,7 +390,7 @@ def chunk_width(n, p, nA, budget=10e9):
 # ------------------------------------------------------------------------------------------------
 def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=None, naive=False,
           seed=2026, kind='block', trace=None, time_limit=None, poles=None, width=None,
-          freeze=True):
+          freeze=True, arith='numpy'):
     """POLAR greedy with cache backend (polar_gpu.py:180-254).
 
     Extensions: `poles=(tau, om)` overrides the rule; `width` overrides chunk_width; the result
@@ -346,10 +415,22 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     ea, eb = es[:, 0], es[:, 1]
     tau_c = tau[:, None]
     om_c = om[:, None]
-    op = Op(G)
+    op = Op(G, arith=arith)
+    cr = arith == 'cr'
     cnt = Counter()
     live = np.ones(N, bool)
     seq, marg, traces = [], [], []
+
+    def qsum(P):                      # reduction over the sketch axis (last)
+        return ddsum(P, axis=-1) if cr else np.sum(P, axis=-1)
+
+    def polesum(T):                   # sum over poles (axis 0), in pole order
+        if not cr:
+            return np.sum(T, axis=0)
+        acc = np.zeros(T.shape[1:])
+        for j in range(T.shape[0]):
+            acc = acc + T[j]
+        return acc
     t0 = time.perf_counter()
     if width is None:
         width = chunk_width(n, p, len(active))                        # :198
@@ -357,24 +438,25 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
 
     def bank():                                                       # :200-206
         RHS = np.concatenate([Z, F], axis=1)
-        parts, tr = [], 0.0
+        parts, terms = [], []
         for lo in range(0, 2 * q, width):
             out = msolve(op, RHS[:, lo:lo + width], tau, active, tol, counter=cnt,
                          freeze=freeze, trace=want_trace)
             parts.append(out[0])
             if want_trace:
                 ncol = min(max(q - lo, 0), out[0].shape[2])
-                tr += float(np.sum(om_c * out[3][:, :ncol]))
+                terms.append((om_c * out[3][:, :ncol]).ravel())
         XY = np.concatenate(parts, axis=2)
         X, Y = XY[:, :, :q].copy(), XY[:, :, q:].copy()
-        h = np.sum((X[:, ea] - X[:, eb]) ** 2, axis=-1)
-        t = np.sum((Y[:, ea] - Y[:, eb]) ** 2, axis=-1)
+        tr = math.fsum(np.concatenate(terms)) if terms else 0.0
+        h = qsum((X[:, ea] - X[:, eb]) ** 2)
+        t = qsum((Y[:, ea] - Y[:, eb]) ** 2)
         return X, Y, h, t, tr
 
     def proxy(h, t):                                                  # :208-211
         if obj == 'logdet':
             return np.log1p(t[0] + sigma * h[0])
-        return np.sum(om_c * h / (1 + t + (sigma + tau_c) * h), axis=0)
+        return polesum(om_c * h / (1 + t + (sigma + tau_c) * h))
 
     X, Y, h, t, tr = bank()
     traces.append(tr if want_trace else None)
@@ -403,7 +485,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             if obj == 'logdet':
                 gsc = np.log1p(bu[0])
             else:
-                gsc = np.sum(om_c * hh / (1 + bu), axis=0)            # :236
+                gsc = polesum(om_c * hh / (1 + bu))                   # :236
             j = 0 if Br == 1 else int(np.argmax(gsc))
             e = int(short[j])
             u = U[:, :, j]
@@ -419,14 +501,20 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             eta = Gc[e][None] - (Y[:, aa] - Y[:, bb])
             D = 1 + u[:, aa] - u[:, bb]
             cc = (u[:, ea] - u[:, eb]) / D[:, None]
-            wX = np.einsum('pnq,pq->pn', X, xi)
-            wY = np.einsum('pnq,pq->pn', Y, eta)
-            h = np.maximum(0, h - 2 * cc * (wX[:, ea] - wX[:, eb]) + cc * cc * np.sum(xi * xi, 1)[:, None])
-            t = np.maximum(0, t + 2 * cc * (wY[:, ea] - wY[:, eb]) + cc * cc * np.sum(eta * eta, 1)[:, None])
+            if cr:
+                wX = ddsum(X * xi[:, None, :], axis=-1)
+                wY = ddsum(Y * eta[:, None, :], axis=-1)
+            else:
+                wX = np.einsum('pnq,pq->pn', X, xi)
+                wY = np.einsum('pnq,pq->pn', Y, eta)
+            nxi = qsum(xi * xi)
+            neta = qsum(eta * eta)
+            h = np.maximum(0, h - 2 * cc * (wX[:, ea] - wX[:, eb]) + cc * cc * nxi[:, None])
+            t = np.maximum(0, t + 2 * cc * (wY[:, ea] - wY[:, eb]) + cc * cc * neta[:, None])
             X -= u[:, :, None] * xi[:, None, :] / D[:, None, None]
             Y += u[:, :, None] * eta[:, None, :] / D[:, None, None]
             if want_trace:
-                traces.append(traces[-1] - float(np.sum(om * np.sum(xi * xi, 1) / D)))
+                traces.append(traces[-1] - math.fsum(om * nxi / D))
         elif naive and want_trace:
             traces.append(None)
         F[a] += Gc[e]                                                 # :250
@@ -460,7 +548,8 @@ def random_set(cand, k, seed):
     return np.random.default_rng(seed).choice(len(cand), k, replace=False).tolist()
 
 
-def true_marginals(G, cand, seq, s, obj='frac', eps=1e-9, tol=1e-10, kmax=None, poles=None):
+def true_marginals(G, cand, seq, s, obj='frac', eps=1e-9, tol=1e-10, kmax=None, poles=None,
+                   arith='numpy'):
     """Certified marginals along a sequence (polar_gpu.py:273-291)."""
     kmax = kmax or len(seq)
     if poles is None:
@@ -468,7 +557,7 @@ def true_marginals(G, cand, seq, s, obj='frac', eps=1e-9, tol=1e-10, kmax=None, 
     else:
         tau, om = np.asarray(poles[0], float), np.asarray(poles[1], float)
         qerr = float('nan')
-    op = Op(G)
+    op = Op(G, arith=arith)
     cnt = Counter()
     out = []
     for e in seq:

@@ -21,7 +21,7 @@ EXPORTS = [
     'polar_msolve_ws_bytes', 'polar_msolve', 'polar_pair_sqnorm', 'polar_proxy',
     'polar_select_ws_bytes', 'polar_select_topb', 'polar_pair_rhs', 'polar_rerank_pick',
     'polar_maint_prep', 'polar_maint_rank1', 'polar_maint_cache', 'polar_rows_axpy',
-    'polar_rademacher', 'polar_edge_sketch',
+    'polar_rademacher', 'polar_edge_sketch', 'polar_prof_set', 'polar_prof_get',
 ]
 
 vp = ctypes.c_void_p
@@ -42,13 +42,15 @@ class PolarOp(ctypes.Structure):
     ]
 
 
+OPP = ctypes.POINTER(PolarOp)
+
 _SIGS = {
     'polar_abi_version': (i32, []),
     'polar_last_error': (ctypes.c_char_p, []),
-    'polar_spmm_ws_bytes': (i64, [vp, i64]),
-    'polar_spmm': (i32, [vp, f64, vp, i64, vp, i64, i64, vp, i64, vp]),
-    'polar_msolve_ws_bytes': (i64, [vp, i64, i32, i64, i32]),
-    'polar_msolve': (i32, [vp, vp, i64, i64, vp, i32, vp, i64, f64, i64, i32, vp, i64, i64, vp, vp,
+    'polar_spmm_ws_bytes': (i64, [OPP, i64]),
+    'polar_spmm': (i32, [OPP, f64, vp, i64, vp, i64, i64, vp, i64, vp]),
+    'polar_msolve_ws_bytes': (i64, [OPP, i64, i32, i64, i32]),
+    'polar_msolve': (i32, [OPP, vp, i64, i64, vp, i32, vp, i64, f64, i64, i32, vp, i64, i64, vp, vp,
                            vp, vp, i64, vp]),
     'polar_pair_sqnorm': (i32, [i32, vp, i64, i64, i32, i64, vp, vp, i64, vp, vp]),
     'polar_proxy': (i32, [i32, vp, vp, vp, vp, i32, i64, f64, vp, vp, vp]),
@@ -61,7 +63,9 @@ _SIGS = {
     'polar_maint_cache': (i32, [i32, vp, i64, i64, i32, i64, i64, vp, vp, vp, i64, vp, vp, vp, vp, vp]),
     'polar_rows_axpy': (i32, [i32, vp, i64, i64, i32, i32, vp, vp]),
     'polar_rademacher': (i32, [i32, u64, i32, vp, i64, i64, f64, vp, i64, vp]),
-    'polar_edge_sketch': (i32, [vp, u64, i64, f64, vp, i64, vp]),
+    'polar_edge_sketch': (i32, [OPP, u64, i64, f64, vp, i64, vp]),
+    'polar_prof_set': (i32, [i32]),
+    'polar_prof_get': (i32, [i32, vp, vp, vp]),
 }
 
 _lib = None

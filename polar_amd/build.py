@@ -14,7 +14,7 @@ SRC = [HERE / 'csrc' / 'krylov.hip', HERE / 'csrc' / 'greedy.hip']
 OUT = HERE / 'libpolar_amd.so'
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 
-FLAGS = ['-O3', '--offload-arch=gfx950', '-fPIC', '-shared', '-std=c++17',
+FLAGS = ['-O3', '--offload-arch=gfx950', '-fPIC', '-shared', '-std=c++17', '-ffp-contract=off',
          # packed-FP32 ops stay off in every HIP build of this project
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
          '-I', str(HERE.parent / 'include'), '-Wno-unused-command-line-argument']

@@ -82,6 +82,43 @@ template <typename T> struct DT;
 template <> struct DT<double> { static constexpr int code = POLAR_F64; };
 template <> struct DT<float> { static constexpr int code = POLAR_F32; };
 
+// Double-double accumulation (error-free TwoSum; the error terms are carried in `lo`).
+struct DD {
+  double hi, lo;
+};
+
+__device__ __forceinline__ void dd_add(DD& a, double b) {
+#pragma clang fp contract(off)
+  const double s = a.hi + b;
+  const double bb = s - a.hi;
+  const double err = (a.hi - (s - bb)) + (b - bb);
+  a.hi = s;
+  a.lo += err;
+}
+
+__device__ __forceinline__ DD dd_merge(DD a, DD b) {
+#pragma clang fp contract(off)
+  const double s = a.hi + b.hi;
+  const double bb = s - a.hi;
+  double e = (a.hi - (s - bb)) + (b.hi - bb);
+  e += a.lo + b.lo;
+  DD r;
+  r.hi = s + e;
+  r.lo = e - (r.hi - s);
+  return r;
+}
+
+// Merge a double-double across the L lanes of a row-group (L a power of two); all lanes get it.
+__device__ __forceinline__ DD dd_group_sum(DD v, int L) {
+  for (int o = L >> 1; o > 0; o >>= 1) {
+    DD w;
+    w.hi = __shfl_xor(v.hi, o, 64);
+    w.lo = __shfl_xor(v.lo, o, 64);
+    v = dd_merge(v, w);
+  }
+  return v;
+}
+
 inline int elem_size(int dtype) { return dtype == POLAR_F64 ? 8 : 4; }
 
 }  // namespace polar

@@ -45,19 +45,19 @@ __global__ __launch_bounds__(kBlock) void k_pair_sqnorm(const T* __restrict__ X,
     const int64_t e = item - (int64_t)j * N;
     const T* xa = X + (int64_t)j * sj + (int64_t)ea[e] * si;
     const T* xb = X + (int64_t)j * sj + (int64_t)eb[e] * si;
-    double s = 0.0;
+    DD s{0.0, 0.0};
     if (c0 < q) {
       double va[CPL], vb[CPL];
       load_cols<T, CPL>(xa, c0, q, va);
       load_cols<T, CPL>(xb, c0, q, vb);
 #pragma unroll
       for (int k = 0; k < CPL; ++k) {
-        const double dd = va[k] - vb[k];
-        s += dd * dd;
+        const double df = va[k] - vb[k];
+        dd_add(s, df * df);
       }
     }
-    s = group_sum(s, L);
-    if (g.li == 0) out[item] = s;
+    s = dd_group_sum(s, L);
+    if (g.li == 0) out[item] = s.hi + s.lo;
   }
 }
 
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_proxy(int obj, const double* __restr
       sc = 0.0;
       for (int j = 0; j < p; ++j) {
         const double hv = h[(int64_t)j * N + e], tv = t[(int64_t)j * N + e];
-        sc += om[j] * hv / (1.0 + tv + (sigma + tau[j]) * hv);
+        sc = sc + ((om[j] * hv) / ((1.0 + tv) + ((sigma + tau[j]) * hv)));
       }
     }
     if (dead && dead[e]) sc = -INFINITY;
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_rerank(int obj, const T* __restrict_
       for (int j = 0; j < p; ++j) {
         const T* Uj = U + (int64_t)j * u_sj;
         const double bu = (double)Uj[(int64_t)sa[c] * u_si + c] - (double)Uj[(int64_t)sb[c] * u_si + c];
-        acc += om[j] * hh[(int64_t)j * Br + c] / (1.0 + bu);
+        acc = acc + ((om[j] * hh[(int64_t)j * Br + c]) / (1.0 + bu));
       }
     }
     g[c] = acc;
@@ -208,29 +208,29 @@ __global__ __launch_bounds__(kBlock) void k_maint_prep(const T* __restrict__ X, 
   const T* Xb = X + (int64_t)j * sj + (int64_t)bb * si;
   const T* Ya = Y + (int64_t)j * sj + (int64_t)aa * si;
   const T* Yb = Y + (int64_t)j * sj + (int64_t)bb * si;
-  __shared__ double sx[kBlock], se[kBlock];
-  double ax = 0.0, ae = 0.0;
+  __shared__ DD sx[kBlock], se[kBlock];
+  DD ax{0.0, 0.0}, ae{0.0, 0.0};
   for (int64_t c = threadIdx.x; c < q; c += blockDim.x) {
     const double xi = (double)Xa[c] - (double)Xb[c];
     const double eta = gc[c] - ((double)Ya[c] - (double)Yb[c]);
     cj[c] = xi;
     cj[q + c] = eta;
-    ax += xi * xi;
-    ae += eta * eta;
+    dd_add(ax, xi * xi);
+    dd_add(ae, eta * eta);
   }
   sx[threadIdx.x] = ax;
   se[threadIdx.x] = ae;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double tx = 0.0, te = 0.0;
+    DD tx{0.0, 0.0}, te{0.0, 0.0};
     for (int k = 0; k < (int)blockDim.x; ++k) {
-      tx += sx[k];
-      te += se[k];
+      tx = dd_merge(tx, sx[k]);
+      te = dd_merge(te, se[k]);
     }
     const T* uj = u + (int64_t)j * u_sj;
-    cj[2 * q] = 1.0 + ((double)uj[(int64_t)aa * u_si] - (double)uj[(int64_t)bb * u_si]);
-    cj[2 * q + 1] = tx;
-    cj[2 * q + 2] = te;
+    cj[2 * q] = (1.0 + (double)uj[(int64_t)aa * u_si]) - (double)uj[(int64_t)bb * u_si];
+    cj[2 * q + 1] = tx.hi + tx.lo;
+    cj[2 * q + 2] = te.hi + te.lo;
   }
 }
 
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_maint_rank1(T* __restrict__ X, T* __
     const double ui = (double)u[(int64_t)j * u_sj + i * u_si];
     T* xr = X + (int64_t)j * sj + i * si;
     T* yr = Y + (int64_t)j * sj + i * si;
-    double sx = 0.0, sy = 0.0;
+    DD sx{0.0, 0.0}, sy{0.0, 0.0};
     const bool ok = c0 < q;
     T xv[CPL], yv[CPL];
     if (ok) {
@@ -271,23 +271,23 @@ __global__ __launch_bounds__(kBlock) void k_maint_rank1(T* __restrict__ X, T* __
 #pragma unroll
       for (int e = 0; e < CPL; ++e) {
         if (c0 + e < q) {
-          sx += (double)xv[e] * cj[c0 + e];
-          sy += (double)yv[e] * cj[q + c0 + e];
+          dd_add(sx, (double)xv[e] * cj[c0 + e]);
+          dd_add(sy, (double)yv[e] * cj[q + c0 + e]);
         }
       }
     }
-    sx = group_sum(sx, L);
-    sy = group_sum(sy, L);
+    sx = dd_group_sum(sx, L);
+    sy = dd_group_sum(sy, L);
     if (g.li == 0) {
-      wX[item] = sx;
-      wY[item] = sy;
+      wX[item] = sx.hi + sx.lo;
+      wY[item] = sy.hi + sy.lo;
     }
     if (ok) {
 #pragma unroll
       for (int e = 0; e < CPL; ++e) {
         if (c0 + e < q) {
-          xv[e] = (T)((double)xv[e] - ui * cj[c0 + e] / D);
-          yv[e] = (T)((double)yv[e] + ui * cj[q + c0 + e] / D);
+          xv[e] = (T)((double)xv[e] - ((ui * cj[c0 + e]) / D));
+          yv[e] = (T)((double)yv[e] + ((ui * cj[q + c0 + e]) / D));
         }
       }
       if (c0 + CPL <= q) {
@@ -325,8 +325,8 @@ __global__ __launch_bounds__(kBlock) void k_maint_cache(const T* __restrict__ u,
     const double cc = ((double)uj[(int64_t)ia * u_si] - (double)uj[(int64_t)ib * u_si]) / D;
     const double dwx = wX[(int64_t)j * nA + ia] - wX[(int64_t)j * nA + ib];
     const double dwy = wY[(int64_t)j * nA + ia] - wY[(int64_t)j * nA + ib];
-    const double hn = h[item] - 2.0 * cc * dwx + cc * cc * nxi;
-    const double tn = t[item] + 2.0 * cc * dwy + cc * cc * neta;
+    const double hn = (h[item] - ((2.0 * cc) * dwx)) + ((cc * cc) * nxi);
+    const double tn = (t[item] + ((2.0 * cc) * dwy)) + ((cc * cc) * neta);
     h[item] = hn > 0.0 ? hn : 0.0;
     t[item] = tn > 0.0 ? tn : 0.0;
   }

@@ -15,6 +15,8 @@
 //    convergence no-ops, so the host reads it only every 8 iterations.
 #include "common.hpp"
 
+#include <vector>
+
 namespace polar {
 
 // ------------------------------------------------------------------------------------------------
@@ -44,21 +46,33 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t* __restrict__ a, in
 }
 
 // ------------------------------------------------------------------------------------------------
-// Deterministic block reduction of per-lane column partials acc[CPL] (lanes with equal li own the
-// same columns) into out[c], c < R.  Sums groups in (wave, group) order.
+// Order-independent reductions.  Every inner product and squared norm of the iteration is summed
+// in double-double (error-free TwoSum with the error terms carried in a second double) and rounded
+// once at the end, so its value is the correctly rounded sum of the (rounded) products whatever
+// the grid shape or summation tree -- the CPU oracle computes the same numbers.
+// Block reduction of per-lane column partials acc[CPL] (lanes with equal li own the same columns)
+// into out[c] (hi) and out[R + c] (lo), c < R.
 template <int CPL>
-__device__ __forceinline__ void block_colsum(const Geo& g, const double (&acc)[CPL], int R,
+__device__ __forceinline__ void block_colsum(const Geo& g, const DD (&acc)[CPL], int R,
                                              double* __restrict__ out) {
-  __shared__ double red[kBlock * 4];
+  __shared__ double red_hi[kBlock * 4];
+  __shared__ double red_lo[kBlock * 4];
 #pragma unroll
-  for (int k = 0; k < CPL; ++k) red[threadIdx.x * CPL + k] = acc[k];
+  for (int k = 0; k < CPL; ++k) {
+    red_hi[threadIdx.x * CPL + k] = acc[k].hi;
+    red_lo[threadIdx.x * CPL + k] = acc[k].lo;
+  }
   __syncthreads();
   for (int c = threadIdx.x; c < R; c += kBlock) {
     int li = c / CPL, k = c % CPL;
-    double s = 0.0;
+    DD s{0.0, 0.0};
     for (int w = 0; w < kWaves; ++w)
-      for (int gi = 0; gi < g.G; ++gi) s += red[((w * 64) + gi * g.L + li) * CPL + k];
-    out[c] = s;
+      for (int gi = 0; gi < g.G; ++gi) {
+        const int ix = ((w * 64) + gi * g.L + li) * CPL + k;
+        s = dd_merge(s, DD{red_hi[ix], red_lo[ix]});
+      }
+    out[c] = s.hi;
+    out[R + c] = s.lo;
   }
   __syncthreads();
 }
@@ -78,31 +92,33 @@ __device__ __forceinline__ bool last_block(unsigned* counter) {
   return s_last != 0;
 }
 
-// Sum parts[b*R + c] over b < nparts in block order; thread layout: segments of R threads.
+// Sum the double-double partial rows parts[b*2R + (0|R) + c], b < nparts, into sh_out[c].
 __device__ __forceinline__ void sum_parts(const double* __restrict__ parts, int nparts, int R,
                                           double* __restrict__ sh_out /* shared [R] */) {
-  __shared__ double seg_sum[kBlock];
-  int nseg = kBlock / R;
-  if (nseg < 1) nseg = 1;
+  __shared__ double seg_hi[kBlock];
+  __shared__ double seg_lo[kBlock];
   for (int c0 = 0; c0 < R; c0 += kBlock) {
-    int t = threadIdx.x;
-    int cw = min(R - c0, kBlock);
-    int ns = max(1, kBlock / cw);
-    int col = t % cw, seg = t / cw;
-    double s = 0.0;
+    const int t = threadIdx.x;
+    const int cw = min(R - c0, kBlock);
+    const int ns = max(1, kBlock / cw);
+    const int col = t % cw, seg = t / cw;
     if (seg < ns) {
-      for (int b = seg; b < nparts; b += ns) s += parts[(int64_t)b * R + c0 + col];
-      seg_sum[seg * cw + col] = s;
+      DD s{0.0, 0.0};
+      for (int b = seg; b < nparts; b += ns) {
+        const double* pb = parts + (int64_t)b * 2 * R;
+        s = dd_merge(s, DD{pb[c0 + col], pb[R + c0 + col]});
+      }
+      seg_hi[seg * cw + col] = s.hi;
+      seg_lo[seg * cw + col] = s.lo;
     }
     __syncthreads();
     if (t < cw) {
-      double tot = 0.0;
-      for (int sgi = 0; sgi < ns; ++sgi) tot += seg_sum[sgi * cw + t];
-      sh_out[c0 + t] = tot;
+      DD tot{0.0, 0.0};
+      for (int sgi = 0; sgi < ns; ++sgi) tot = dd_merge(tot, DD{seg_hi[sgi * cw + t], seg_lo[sgi * cw + t]});
+      sh_out[c0 + t] = tot.hi + tot.lo;
     }
     __syncthreads();
   }
-  (void)nseg;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -158,14 +174,14 @@ __device__ __forceinline__ void finish_row(const polar_op& op, double shift, con
   vload<T, CPL>(X + (int64_t)row * ldx + c0, xv);
   const T dg = (T)((double)((const T*)op.diag)[row] + shift);
 #pragma unroll
-  for (int e = 0; e < CPL; ++e) yv[e] = dg * xv[e] - acc[e];
+  for (int e = 0; e < CPL; ++e) yv[e] = (dg * xv[e]) - acc[e];
   if (pidx >= 0) {
     const int pb = op.patch_ptr[pidx], pe = op.patch_ptr[pidx + 1];
     for (int q = pb; q < pe; ++q) {
       T xj[CPL];
       vload<T, CPL>(X + (int64_t)op.patch_cols[q] * ldx + c0, xj);
 #pragma unroll
-      for (int e = 0; e < CPL; ++e) yv[e] += xv[e] - xj[e];
+      for (int e = 0; e < CPL; ++e) yv[e] = yv[e] + (xv[e] - xj[e]);
     }
   }
 }
@@ -182,7 +198,8 @@ struct CGState {
   int *live, *m;
   int* flag;           // [2] any-active flags, indexed by iteration parity
   unsigned* counter;   // [4] last-block counters
-  double* parts;       // [2*kMaxGrid][R] block partials
+  unsigned long long* stats;  // [2] sum over iterations of active columns / live (pole, column) pairs
+  double* parts;       // [2*kMaxGrid][2][R] double-double block partials
   int R, p;
   double tol;
   int freeze, track, trace;
@@ -203,13 +220,19 @@ __device__ void phase_scalars(const CGState& s, int nparts, int cur) {
   __shared__ int any_s;
   sum_parts(s.parts, nparts, s.R, rrn);
   const int R = s.R;
-  if (threadIdx.x == 0) any_s = 0;
+  __shared__ int nlive_s, nact_s;
+  if (threadIdx.x == 0) {
+    any_s = 0;
+    nlive_s = 0;
+    nact_s = 0;
+  }
   for (int c = threadIdx.x; c < R; c += kBlock) {
     const bool act = s.act[c] != 0;
     bec[c] = act ? rrn[c] / s.rr[c] : 0.0;
     ccs[c] = act ? s.a[c] * s.bp[c] / s.ap[c] : 0.0;
   }
   __syncthreads();
+  int nlive = 0;
   for (int idx = threadIdx.x; idx < s.p * R; idx += kBlock) {
     const int c = idx % R;
     const bool act = s.act[c] != 0;
@@ -224,6 +247,7 @@ __device__ void phase_scalars(const CGState& s, int nparts, int cur) {
     s.aj[idx] = aj;
     s.bj[idx] = bj;
     s.m[idx] = m ? 1 : 0;
+    nlive += m ? 1 : 0;
     if (s.track) {
       const double ip = s.ip[idx], pn = s.pn[idx];
       s.h[idx] += 2.0 * aj * ip + aj * aj * pn;
@@ -244,10 +268,12 @@ __device__ void phase_scalars(const CGState& s, int nparts, int cur) {
     const double nr = sqrt(rrn[c]);
     if (s.freeze && m && fabs(zn) * nr <= s.stop[c]) s.live[idx] = 0;
   }
+  if (nlive) atomicAdd(&nlive_s, nlive);
   __syncthreads();
-  int any = 0;
+  int any = 0, nact = 0;
   for (int c = threadIdx.x; c < R; c += kBlock) {
     const bool act = s.act[c] != 0;
+    nact += act ? 1 : 0;
     s.dmask[c] = act ? 1 : 0;
     s.be[c] = bec[c];
     if (act) {
@@ -261,8 +287,13 @@ __device__ void phase_scalars(const CGState& s, int nparts, int cur) {
     }
   }
   if (any) atomicOr(&any_s, 1);
+  if (nact) atomicAdd(&nact_s, nact);
   __syncthreads();
-  if (threadIdx.x == 0) s.flag[cur ^ 1] = any_s;
+  if (threadIdx.x == 0) {
+    s.flag[cur ^ 1] = any_s;
+    s.stats[0] += (unsigned long long)nact_s;
+    s.stats[1] += (unsigned long long)nlive_s;
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -281,9 +312,9 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
   const int gpb = kWaves * g.G;
   int64_t u = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi;
   const int64_t stride = (int64_t)gridDim.x * gpb;
-  double dacc[CPL];
+  DD dacc[CPL];
 #pragma unroll
-  for (int e = 0; e < CPL; ++e) dacc[e] = 0.0;
+  for (int e = 0; e < CPL; ++e) dacc[e] = DD{0.0, 0.0};
   int pc = (op.n_patch > 0 && u < n) ? lower_bound_i32(op.patch_rows, op.n_patch, (int)u) : 0;
   for (; u < nunits; u += stride) {
     int row, beg, end, lidx = -1, chunk = 0;
@@ -320,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
         vstore<T, CPL>(Y + (int64_t)row * ldy + c0, yv);
         if (DOT) {
 #pragma unroll
-          for (int e = 0; e < CPL; ++e) dacc[e] += (double)xv[e] * (double)yv[e];
+          for (int e = 0; e < CPL; ++e) dd_add(dacc[e], (double)xv[e] * (double)yv[e]);
         }
       }
     } else if (colok) {
@@ -328,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
     }
   }
   if (DOT) {
-    block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)blockIdx.x * R);
+    block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)blockIdx.x * 2 * R);
     if (finish && last_block(&s.counter[0])) {
       phase_alpha(s, gridDim.x);
       __syncthreads();
@@ -348,9 +379,9 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
   const int c0 = g.li * CPL;
   const bool colok = c0 < R;
   const int gpb = kWaves * g.G;
-  double dacc[CPL];
+  DD dacc[CPL];
 #pragma unroll
-  for (int e = 0; e < CPL; ++e) dacc[e] = 0.0;
+  for (int e = 0; e < CPL; ++e) dacc[e] = DD{0.0, 0.0};
   for (int64_t l = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi; l < op.n_long;
        l += (int64_t)gridDim.x * gpb) {
     const int row = op.long_rows[l];
@@ -377,12 +408,12 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
       vstore<T, CPL>(Y + (int64_t)row * ldy + c0, yv);
       if (DOT) {
 #pragma unroll
-        for (int e = 0; e < CPL; ++e) dacc[e] += (double)xv[e] * (double)yv[e];
+        for (int e = 0; e < CPL; ++e) dd_add(dacc[e], (double)xv[e] * (double)yv[e]);
       }
     }
   }
   if (DOT) {
-    block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)(parts_off + blockIdx.x) * R);
+    block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)(parts_off + blockIdx.x) * 2 * R);
     if (last_block(&s.counter[1])) {
       phase_alpha(s, parts_off + gridDim.x);
       __syncthreads();
@@ -395,13 +426,17 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
 template <typename T, int CPL>
 __global__ __launch_bounds__(kBlock) void k_resid(int64_t n, int R, int L, T* __restrict__ r,
                                                   const T* __restrict__ v, CGState s, int cur) {
-  if (s.flag[cur] == 0) return;
+  if (s.flag[cur] == 0) {
+    // converged: keep both parities at 0 so every later launch is a no-op
+    if (blockIdx.x == 0 && threadIdx.x == 0) s.flag[cur ^ 1] = 0;
+    return;
+  }
   Geo g(L);
   const int c0 = g.li * CPL;
   const bool colok = c0 < R;
-  double dacc[CPL];
+  DD dacc[CPL];
 #pragma unroll
-  for (int e = 0; e < CPL; ++e) dacc[e] = 0.0;
+  for (int e = 0; e < CPL; ++e) dacc[e] = DD{0.0, 0.0};
   if (colok) {
     T a[CPL];
     bool act[CPL];
@@ -420,14 +455,14 @@ __global__ __launch_bounds__(kBlock) void k_resid(int64_t n, int R, int L, T* __
         vload<T, CPL>(v + i * R + c0, vv);
 #pragma unroll
         for (int e = 0; e < CPL; ++e)
-          if (act[e]) rv[e] = rv[e] - a[e] * vv[e];
+          if (act[e]) rv[e] = rv[e] - (a[e] * vv[e]);
         vstore<T, CPL>(r + i * R + c0, rv);
       }
 #pragma unroll
-      for (int e = 0; e < CPL; ++e) dacc[e] += (double)rv[e] * (double)rv[e];
+      for (int e = 0; e < CPL; ++e) dd_add(dacc[e], (double)rv[e] * (double)rv[e]);
     }
   }
-  block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)blockIdx.x * R);
+  block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)blockIdx.x * 2 * R);
   if (last_block(&s.counter[2])) {
     phase_scalars(s, gridDim.x, cur);
     __syncthreads();
@@ -465,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
       vload<T, CPL>(d + i * R + c0, dv);
 #pragma unroll
       for (int e = 0; e < CPL; ++e)
-        if (msk[e]) dv[e] = rv[e] + be[e] * dv[e];
+        if (msk[e]) dv[e] = rv[e] + (be[e] * dv[e]);
       vstore<T, CPL>(d + i * R + c0, dv);
     }
     return;
@@ -500,8 +535,8 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
 #pragma unroll
       for (int e = 0; e < CPL; ++e) {
         if (mj[e]) {
-          xv[e] = xv[e] + aj[e] * pv[e];
-          pv[e] = zn[e] * rn[e] + bj[e] * pv[e];
+          xv[e] = xv[e] + (aj[e] * pv[e]);
+          pv[e] = (zn[e] * rn[e]) + (bj[e] * pv[e]);
         }
       }
       vstore<T, CPL>(xp, xv);
@@ -518,9 +553,9 @@ __global__ __launch_bounds__(kBlock) void k_init(int64_t n, int R, int L, const 
   Geo g(L);
   const int c0 = g.li * CPL;
   const bool colok = c0 < R;
-  double dacc[CPL];
+  DD dacc[CPL];
 #pragma unroll
-  for (int e = 0; e < CPL; ++e) dacc[e] = 0.0;
+  for (int e = 0; e < CPL; ++e) dacc[e] = DD{0.0, 0.0};
   if (colok) {
     const int gpb = kWaves * g.G;
     for (int64_t i = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi; i < n; i += (int64_t)gridDim.x * gpb) {
@@ -529,10 +564,10 @@ __global__ __launch_bounds__(kBlock) void k_init(int64_t n, int R, int L, const 
       vstore<T, CPL>(r + i * R + c0, wv);
       vstore<T, CPL>(d + i * R + c0, wv);
 #pragma unroll
-      for (int e = 0; e < CPL; ++e) dacc[e] += (double)wv[e] * (double)wv[e];
+      for (int e = 0; e < CPL; ++e) dd_add(dacc[e], (double)wv[e] * (double)wv[e]);
     }
   }
-  block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)blockIdx.x * R);
+  block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)blockIdx.x * 2 * R);
   if (last_block(&s.counter[3])) {
     __shared__ double rr[512];
     __shared__ int any_s;
@@ -608,6 +643,17 @@ __global__ void k_shift(const double* __restrict__ tau, int p, double* __restric
 
 // ------------------------------------------------------------------------------------------------
 // Host side
+// Live kernel profiling (HIP events around each launch of the CG iteration), off by default.
+// Class 0: opmul (+ long-row finish), 1: resid, 2: update; class 3 counts solves and, in `bytes`,
+// the sum over iterations of live (pole, column) pairs.
+struct ProfClass {
+  double ms = 0.0;
+  int64_t launches = 0;
+  double bytes = 0.0;
+};
+static ProfClass g_prof[4];
+static int g_prof_on = 0;
+
 struct Plan {
   int cpl, L, G;
   int grid_a, grid_l, grid_rows;
@@ -680,7 +726,8 @@ static void carve_state(Carver& cv, CGState& s, int64_t R, int p) {
   s.live = cv.take<int>(pr); s.m = cv.take<int>(pr);
   s.flag = cv.take<int>(2);
   s.counter = cv.take<unsigned>(4);
-  s.parts = cv.take<double>((int64_t)2 * kMaxGrid * R);
+  s.stats = cv.take<unsigned long long>(2);
+  s.parts = cv.take<double>((int64_t)4 * kMaxGrid * R);
 }
 
 template <typename T, int CPL>
@@ -748,6 +795,11 @@ static int msolve_impl(const polar_op* op, const T* W, int64_t ldw, int64_t R, c
   const int64_t gpb = (int64_t)kWaves * pl.G;
 
   POLAR_HIP(hipMemsetAsync(s.counter, 0, 4 * sizeof(unsigned), st));
+  POLAR_HIP(hipMemsetAsync(s.stats, 0, 2 * sizeof(unsigned long long), st));
+  const bool prof = g_prof_on != 0;
+  std::vector<hipEvent_t> ev;
+  std::vector<float> kms;  // per iteration: opmul, resid, update milliseconds
+  if (prof) ev.reserve(4 * 8);
   hipLaunchKernelGGL(k_shift, dim3((p + 255) / 256), dim3(256), 0, st, tau, p, s.shift);
   POLAR_LAUNCH_CHECK();
   hipLaunchKernelGGL((k_init<T, CPL>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, L, W, ldw, r, d, s);
@@ -767,22 +819,77 @@ static int msolve_impl(const polar_op* op, const T* W, int64_t ldw, int64_t R, c
   int64_t it = 0;
   for (; it < maxit; ++it) {
     const int cur = (int)(it & 1);
+    const int slot = (int)(it % 8) * 4;
+    if (prof) {
+      while ((int)ev.size() < slot + 4) {
+        hipEvent_t e;
+        POLAR_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+      }
+      POLAR_HIP(hipEventRecord(ev[slot], st));
+    }
     int rc = launch_opmul<T, CPL>(op, pl, tau0, d, R, v, R, (int)R, lpart, s, cur, true, st);
     if (rc) return rc;
+    if (prof) POLAR_HIP(hipEventRecord(ev[slot + 1], st));
     hipLaunchKernelGGL((k_resid<T, CPL>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, L, r, (const T*)v, s, cur);
     POLAR_LAUNCH_CHECK();
+    if (prof) POLAR_HIP(hipEventRecord(ev[slot + 2], st));
     hipLaunchKernelGGL((k_update<T, CPL>), dim3(nb_d + nb_s), dim3(kBlock), 0, st, n, nA, p, (int)R, L,
                        (const T*)r, d, active, P, Xo, sj, si, nb_d, s, cur);
     POLAR_LAUNCH_CHECK();
+    if (prof) POLAR_HIP(hipEventRecord(ev[slot + 3], st));
     if (it % 8 == 7) {
       int flag = 1;
       POLAR_HIP(hipMemcpyAsync(&flag, s.flag + ((it + 1) & 1), sizeof(int), hipMemcpyDeviceToHost, st));
       POLAR_HIP(hipStreamSynchronize(st));
+      if (prof) {
+        for (int k = 0; k < 8; ++k) {
+          for (int c = 0; c < 3; ++c) {
+            float ms = 0.f;
+            POLAR_HIP(hipEventElapsedTime(&ms, ev[k * 4 + c], ev[k * 4 + c + 1]));
+            kms.push_back(ms);
+          }
+        }
+      }
       if (!flag) {
         converged = true;
         break;
       }
     }
+  }
+  if (prof) {
+    // count only the iterations that did work (launches after convergence return at once)
+    std::vector<int64_t> ih(R);
+    unsigned long long st2[2];
+    POLAR_HIP(hipMemcpyAsync(ih.data(), s.iters, sizeof(int64_t) * R, hipMemcpyDeviceToHost, st));
+    POLAR_HIP(hipMemcpyAsync(st2, s.stats, sizeof(st2), hipMemcpyDeviceToHost, st));
+    POLAR_HIP(hipStreamSynchronize(st));
+    int64_t real = 0;
+    for (int64_t c = 0; c < R; ++c) real = std::max(real, ih[c]);
+    real = std::min<int64_t>(real, (int64_t)kms.size() / 3);
+    const double esz = sizeof(T);
+    const double nnz = (double)((int64_t)0);
+    (void)nnz;
+    double opbytes = 0.0;
+    {
+      int32_t nnz_h = 0;
+      POLAR_HIP(hipMemcpy(&nnz_h, op->indptr + op->n, sizeof(int32_t), hipMemcpyDeviceToHost));
+      opbytes = 4.0 * (n + 1) + 4.0 * nnz_h + (op->weighted ? esz * nnz_h : 0.0) + esz * n + 2.0 * esz * n * R;
+    }
+    for (int64_t k = 0; k < real; ++k) {
+      g_prof[0].ms += kms[3 * k];
+      g_prof[1].ms += kms[3 * k + 1];
+      g_prof[2].ms += kms[3 * k + 2];
+    }
+    g_prof[0].launches += real;
+    g_prof[1].launches += real;
+    g_prof[2].launches += real;
+    g_prof[0].bytes += opbytes * real;
+    g_prof[1].bytes += 3.0 * esz * n * (double)st2[0];
+    g_prof[2].bytes += esz * (3.0 * n + nA) * (double)st2[0] + 4.0 * esz * nA * (double)st2[1];
+    g_prof[3].launches += 1;
+    g_prof[3].bytes += (double)st2[1];
+    for (auto e : ev) (void)hipEventDestroy(e);
   }
   if (h_out) POLAR_HIP(hipMemcpyAsync(h_out, s.h, sizeof(double) * p * R, hipMemcpyDeviceToDevice, st));
   if (te_out) POLAR_HIP(hipMemcpyAsync(te_out, s.te, sizeof(double) * p * R, hipMemcpyDeviceToDevice, st));
@@ -811,6 +918,20 @@ using namespace polar;
 extern "C" {
 
 int polar_abi_version(void) { return POLAR_ABI_VERSION; }
+
+int polar_prof_set(int32_t on) {
+  g_prof_on = on;
+  for (auto& c : g_prof) c = ProfClass();
+  return 0;
+}
+
+int polar_prof_get(int32_t cls, double* ms, int64_t* launches, double* bytes) {
+  if (cls < 0 || cls > 3) return set_error(POLAR_E_ARG, "bad profile class");
+  if (ms) *ms = g_prof[cls].ms;
+  if (launches) *launches = g_prof[cls].launches;
+  if (bytes) *bytes = g_prof[cls].bytes;
+  return 0;
+}
 
 const char* polar_last_error(void) { return g_err; }
 

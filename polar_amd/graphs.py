@@ -65,30 +65,27 @@ def chung_lu_weights(n, gamma, mean_deg):
     return w * (mean_deg * n / w.sum())
 
 
-def chung_lu(n, gamma, mean_deg, seed, m=None, batch=1 << 24):
+def chung_lu(n, gamma, mean_deg, seed, m=None):
     """Sample endpoint pairs i.i.d. proportional to w until m unique undirected non-loop edges
-    (default m = round(n*mean_deg/2))."""
+    (default m = round(n*mean_deg/2)); a surplus of distinct pairs is thinned uniformly."""
     rng = np.random.default_rng(seed)
     w = chung_lu_weights(n, gamma, mean_deg)
     cdf = np.cumsum(w)
     cdf /= cdf[-1]
     m = int(round(n * mean_deg / 2)) if m is None else int(m)
     keys = np.empty(0, dtype=np.int64)
+    factor = 1.15
     while len(keys) < m:
-        need = m - len(keys)
-        b = min(batch, int(need * 1.3) + 1024)
-        u = np.searchsorted(cdf, rng.random(b), side='right')
-        v = np.searchsorted(cdf, rng.random(b), side='right')
+        b = int((m - len(keys)) * factor) + 1024
+        u = np.searchsorted(cdf, rng.random(b), side='right').astype(np.int64)
+        v = np.searchsorted(cdf, rng.random(b), side='right').astype(np.int64)
         np.minimum(u, n - 1, out=u)
         np.minimum(v, n - 1, out=v)
-        lo = np.minimum(u, v).astype(np.int64)
-        hi = np.maximum(u, v).astype(np.int64)
+        lo = np.minimum(u, v)
+        hi = np.maximum(u, v)
         ok = lo != hi
-        new = lo[ok] * n + hi[ok]
-        keys = np.concatenate([keys, new])
-        keys = np.unique(keys) if len(keys) > m else keys
-        if len(keys) >= m:
-            keys = np.unique(keys)
+        keys = np.unique(np.concatenate([keys, lo[ok] * n + hi[ok]]))
+        factor *= 1.5
     if len(keys) > m:
         keys = np.sort(rng.choice(keys, m, replace=False))
     return np.stack([keys // n, keys % n], axis=1)

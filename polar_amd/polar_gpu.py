@@ -1,0 +1,628 @@
+"""MI355X drop-in for realworld/polar_gpu.py (POLAR's pole-shared Krylov core).
+
+Same entry points and argument meaning as the reference (citations per function); the numerical
+work runs in the hand-written gfx950 kernels of libpolar_amd.so, called through ctypes on raw
+device pointers of PyTorch-ROCm tensors.  There is no CPU fallback: without the library or a GPU
+every compute entry point raises.
+
+Additive extensions (the reference signatures stay valid):
+  * `precision='fp64'|'fp32'`: dtype of the operator, vectors, shifted blocks and X/Y caches;
+    scalar recurrences, inner products, h/t caches and scores are FP64 in both.
+  * `poles=(tau, om)`: pole override (configs prescribe 16/24/32 poles; see quadrature.fixed_rule).
+  * `width=`: columns per bank block solve (numerically neutral: columns are independent).
+  * `freeze=False`: no pole freezing (the single-RHS variant of synthetic/scalability/benchmark.py).
+  * results carry `marginals` (per-round G_q of the winner) and `trace_estimate`.
+"""
+import math
+import time
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from . import _lib as L
+from .quadrature import composite, fixed_rule, poles  # noqa: F401  (re-exported API)
+
+__all__ = ['composite', 'fixed_rule', 'poles', 'Graph', 'Op', 'Counter', 'msolve', 'sketches',
+           'chunk_width', 'polar', 'degree_greedy', 'random_set', 'true_marginals', 'spmm',
+           'RademacherRows']
+
+
+def _device():
+    L.require_gpu()
+    return torch.device('cuda', torch.cuda.current_device())
+
+
+def _dt(precision):
+    if precision in ('fp64', torch.float64, 'float64'):
+        return torch.float64
+    if precision in ('fp32', torch.float32, 'float32', 'mixed'):
+        return torch.float32
+    raise ValueError(f'precision {precision!r}')
+
+
+# ------------------------------------------------------------------------------------------------
+class LaplacianCSR:
+    """M0 = diag(deg + sigma) - A held as diagonal + off-diagonal adjacency (polar_gpu.py:52-55).
+    `.get()` returns the full host CSR (with the diagonal), as the reference's cupyx matrix does."""
+
+    def __init__(self, G):
+        self._G = G
+        self.shape = (G.n, G.n)
+
+    def get(self):
+        G = self._G
+        A = sp.csr_matrix((G._w, G._indices, G._indptr), shape=(G.n, G.n))
+        M = (sp.diags(G.deg + G.sigma) - A).tocsr()
+        M.sort_indices()
+        return M
+
+
+class Graph:
+    """Graph(n, edges, sigma, weights=None) of polar_gpu.py:47-60, stored on the GPU."""
+
+    def __init__(self, n, edges, sigma, weights=None):
+        self.n = int(n)
+        self.sigma = float(sigma)
+        edges = np.asarray(edges, dtype=np.int64)
+        u, v = edges[:, 0], edges[:, 1]
+        w = np.ones(len(u)) if weights is None else np.asarray(weights, float)
+        deg = np.bincount(u, w, self.n) + np.bincount(v, w, self.n)
+        A = sp.coo_matrix((np.r_[w, w], (np.r_[u, v], np.r_[v, u])), shape=(self.n, self.n)).tocsr()
+        A.sum_duplicates()
+        A.sort_indices()
+        if A.nnz >= 2 ** 31:
+            raise ValueError('graphs with >= 2^31 adjacency entries need int64 indices')
+        self._indptr = A.indptr.astype(np.int32)
+        self._indices = A.indices.astype(np.int32)
+        self._w = A.data.astype(np.float64)
+        self.unit = bool(np.all(self._w == 1.0))
+        self.deg = deg
+        self.dmax = float(deg.max()) if self.n else 0.0
+        self.m = len(u)
+        self.edges = edges
+        self.weights = w
+        self.M0 = LaplacianCSR(self)
+        self._dev = {}
+        self._sched = None
+
+    def Lam(self, k):
+        return self.sigma + 2 * self.dmax + 4 * k
+
+    @property
+    def nnz(self):
+        return int(self._indptr[-1])
+
+    def schedule(self):
+        """Long-row split (rows with more than POLAR_CHUNK neighbours), host arrays."""
+        if self._sched is None:
+            d = np.diff(self._indptr).astype(np.int64)
+            long_rows = np.flatnonzero(d > L.CHUNK).astype(np.int32)
+            nch = (d[long_rows] + L.CHUNK - 1) // L.CHUNK
+            base = np.zeros(len(long_rows), dtype=np.int64)
+            if len(long_rows):
+                base[1:] = np.cumsum(nch)[:-1]
+            li = np.repeat(np.arange(len(long_rows)), np.maximum(nch - 1, 0))
+            ci = np.concatenate([np.arange(1, c) for c in nch]) if len(nch) else np.zeros(0, np.int64)
+            extra = np.stack([li, ci], axis=1).astype(np.int32) if len(li) else np.zeros((0, 2), np.int32)
+            self._sched = dict(long_rows=long_rows, long_base=base.astype(np.int32),
+                               extra=np.ascontiguousarray(extra), n_partial=int(nch.sum()))
+        return self._sched
+
+    def device(self, dtype):
+        """Device arrays for operator dtype (indices shared between dtypes)."""
+        dev = _device()
+        if 'idx' not in self._dev:
+            sc = self.schedule()
+            self._dev['idx'] = dict(
+                indptr=torch.from_numpy(self._indptr).to(dev),
+                indices=torch.from_numpy(self._indices).to(dev) if self.nnz else torch.zeros(1, dtype=torch.int32, device=dev),
+                long_rows=torch.from_numpy(sc['long_rows']).to(dev),
+                long_base=torch.from_numpy(sc['long_base']).to(dev),
+                extra=torch.from_numpy(sc['extra'].reshape(-1)).to(dev),
+            )
+        if dtype not in self._dev:
+            self._dev[dtype] = dict(
+                diag=torch.from_numpy(self.deg + self.sigma).to(dev, dtype),
+                weights=None if self.unit else torch.from_numpy(self._w).to(dev, dtype),
+            )
+        return self._dev['idx'], self._dev[dtype]
+
+
+class Op:
+    """y = (M0 + sum_S b b^T + shift I) x with unit-weight added edges (polar_gpu.py:63-80).
+    The added edges form a row patch applied inside the SpMM kernel (deterministic; no atomics)."""
+
+    def __init__(self, G, precision='fp64'):
+        self.G = G
+        self.M0 = G.M0
+        self.A = []
+        self.B = []
+        self.dtype = _dt(precision)
+        self._patch = None
+        self._struct = None
+
+    def add(self, a, b):
+        self.A.append(int(a))
+        self.B.append(int(b))
+        self._patch = None
+        self._struct = None
+
+    def _patch_arrays(self):
+        if self._patch is None:
+            dev = _device()
+            if self.A:
+                nbr = {}
+                for a, b in zip(self.A, self.B):
+                    nbr.setdefault(a, []).append(b)
+                    nbr.setdefault(b, []).append(a)
+                rows = np.array(sorted(nbr), dtype=np.int32)
+                ptr = np.zeros(len(rows) + 1, dtype=np.int32)
+                ptr[1:] = np.cumsum([len(nbr[r]) for r in rows])
+                cols = np.array([c for r in rows for c in nbr[r]], dtype=np.int32)
+                self._patch = (torch.from_numpy(rows).to(dev), torch.from_numpy(ptr).to(dev),
+                               torch.from_numpy(cols).to(dev))
+            else:
+                self._patch = ()
+        return self._patch
+
+    def c_op(self, dtype=None):
+        """ctypes polar_op for this operator (kept alive by self)."""
+        dtype = self.dtype if dtype is None else dtype
+        if self._struct is not None and self._struct[0] == dtype:
+            return self._struct[1]
+        idx, dd = self.G.device(dtype)
+        sc = self.G.schedule()
+        patch = self._patch_arrays()
+        s = L.PolarOp()
+        s.n = self.G.n
+        s.dtype = L.dtype_code(dtype)
+        s.weighted = 0 if dd['weights'] is None else 1
+        s.indptr = L.ptr(idx['indptr'])
+        s.indices = L.ptr(idx['indices'])
+        s.weights = L.ptr(dd['weights'])
+        s.diag = L.ptr(dd['diag'])
+        s.n_long = len(sc['long_rows'])
+        s.n_extra = len(sc['extra'])
+        s.n_partial = sc['n_partial']
+        s.long_rows = L.ptr(idx['long_rows']) if s.n_long else None
+        s.long_base = L.ptr(idx['long_base']) if s.n_long else None
+        s.extra = L.ptr(idx['extra']) if s.n_extra else None
+        if patch:
+            s.n_patch = len(patch[0])
+            s.patch_rows, s.patch_ptr, s.patch_cols = (L.ptr(t) for t in patch)
+        self._struct = (dtype, s)
+        return s
+
+    def __call__(self, x, shift):
+        return _apply(self, x, shift)
+
+
+def _apply(op, x, shift):
+    lib = L.require_gpu()
+    x = torch.as_tensor(x, device=_device())
+    one = x.ndim == 1
+    X = x.reshape(op.G.n, -1).to(op.dtype).contiguous()
+    R = X.shape[1]
+    Y = torch.empty_like(X)
+    s = op.c_op(X.dtype)
+    ws = torch.empty(max(lib.polar_spmm_ws_bytes(s, min(R, 64)), 256), dtype=torch.uint8, device=X.device)
+    for lo in range(0, R, 64):
+        r = min(64, R - lo)
+        L.check(lib.polar_spmm(s, float(shift or 0.0), L.ptr(X[:, lo:]), R, L.ptr(Y[:, lo:]), R, r,
+                               L.ptr(ws), ws.numel(), L.stream_ptr()), 'polar_spmm')
+    return Y.reshape(-1) if one else Y
+
+
+def spmm(A, X):
+    """Y = M0 X (realworld/spmm.py:18-25); A is a Graph, its M0, or an Op."""
+    if isinstance(A, LaplacianCSR):
+        A = A._G
+    op = A if isinstance(A, Op) else Op(A, precision=torch.as_tensor(X).dtype)
+    return _apply(op, X, 0.0)
+
+
+class Counter:
+    """polar_gpu.py:83-86 (plus `solves`: column-iterations of each msolve call)."""
+
+    def __init__(self):
+        self.spmv = 0
+        self.rhs = 0
+        self.solves = []
+
+
+def _max_cols(dtype):
+    return 128 if dtype == torch.float64 else 256
+
+
+def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, freeze=True,
+           trace=False, out=None):
+    """Block multi-shift CG for (M + tau_j I) U = W on all poles (polar_gpu.py:88-133).
+
+    Returns (X[p, nA, R], h[p, R] or None, iters[R] host int64) and, with trace=True, a 4th value
+    te[p, R] = w^T (M + tau_j)^{-1} w.  `out` may be a (p, nA, R) view with unit column stride
+    into which X is written (the bank)."""
+    lib = L.require_gpu()
+    dev = _device()
+    dtype = op.dtype
+    W = torch.as_tensor(W, device=dev)
+    if W.ndim == 1:
+        W = W[:, None]
+    W = W.to(dtype)
+    if W.stride(1) != 1:
+        W = W.contiguous()
+    n, R = W.shape
+    tau_h = np.asarray(tau, dtype=np.float64)
+    p = len(tau_h)
+    tau_d = torch.from_numpy(tau_h).to(dev)
+    if torch.is_tensor(active) and active.is_cuda and active.dtype == torch.int32:
+        act_d = active.contiguous()
+    else:
+        act_d = torch.as_tensor(np.asarray(active.cpu() if torch.is_tensor(active) else active),
+                                dtype=torch.int32).to(dev)
+    nA = act_d.numel()
+    X = out if out is not None else torch.empty((p, nA, R), dtype=dtype, device=dev)
+    assert X.shape == (p, nA, R) and X.stride(2) == 1
+    h = torch.empty((p, R), dtype=torch.float64, device=dev) if track else None
+    te = torch.empty((p, R), dtype=torch.float64, device=dev) if trace else None
+    iters = torch.empty(R, dtype=torch.int64, device=dev)
+    flags = (L.POLAR_TRACK if track else 0) | (L.POLAR_TRACE if trace else 0) | (0 if freeze else L.POLAR_NOFREEZE)
+    s = op.c_op(dtype)
+    step = min(R, _max_cols(dtype))
+    ws_bytes = lib.polar_msolve_ws_bytes(s, step, p, nA, flags)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    for lo in range(0, R, step):
+        r = min(step, R - lo)
+        # h/te blocks are written as contiguous p x r arrays
+        hb = (h if r == R else torch.empty((p, r), dtype=torch.float64, device=dev)) if track else None
+        tb = (te if r == R else torch.empty((p, r), dtype=torch.float64, device=dev)) if trace else None
+        rc = lib.polar_msolve(s, L.ptr(W[:, lo:]), W.stride(0), r, L.ptr(tau_d), p, L.ptr(act_d), nA,
+                              float(tol), int(maxit), flags, L.ptr(X[:, :, lo:]), X.stride(0),
+                              X.stride(1), L.ptr(hb), L.ptr(tb), L.ptr(iters[lo:]), L.ptr(ws),
+                              ws_bytes, L.stream_ptr())
+        L.check(rc, 'polar_msolve')
+        if track and r != R:
+            h[:, lo:lo + r] = hb
+        if trace and r != R:
+            te[:, lo:lo + r] = tb
+    it_h = iters.cpu().numpy()
+    if counter is not None:
+        counter.spmv += int(it_h.sum())
+        counter.rhs += R
+        counter.solves.append(int(it_h.sum()))
+    res = (X, h, it_h)
+    if trace:
+        res = res + (te,)
+    return res
+
+
+# ------------------------------------------------------------------------------------------------
+def _block_rows(rows, q, rng, center=False):
+    """Stacked orthogonal q x q row blocks (polar_gpu.py:137-145), host NumPy."""
+    nb = -(-rows // q)
+    Wb = rng.normal(size=(nb, q, q))
+    Q, Rm = np.linalg.qr(Wb)
+    Q = Q * np.sign(np.diagonal(Rm, axis1=1, axis2=2))[:, None, :]
+    Wb = np.transpose(Q, (0, 2, 1)).reshape(nb * q, q)[:rows]
+    if center:
+        Wb = Wb - Wb.mean(0)
+    return Wb
+
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _splitmix64(x):
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & _M64
+    x = ((x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & _M64
+    x = ((x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & _M64
+    return x ^ (x >> np.uint64(31))
+
+
+def _rad_rows(seed, stream, keys, q):
+    with np.errstate(over='ignore'):
+        base = _splitmix64(np.uint64(seed * 4 + stream))
+        k = np.asarray(keys, dtype=np.uint64)[:, None] * np.uint64(q) + np.arange(q, dtype=np.uint64)[None]
+        hsh = _splitmix64((base + k) & _M64)
+    return np.where((hsh >> np.uint64(63)) == 1, -1.0, 1.0)
+
+
+class RademacherRows:
+    """Lazily generated candidate sketch rows Gc[e] (counter hash, stream 2)."""
+
+    def __init__(self, seed, q, N):
+        self.seed, self.q, self.N = int(seed), int(q), int(N)
+
+    def __len__(self):
+        return self.N
+
+    def __getitem__(self, e):
+        return _rad_rows(self.seed, 2, np.atleast_1d(e), self.q)[0] / math.sqrt(self.q)
+
+    def full(self):
+        return _rad_rows(self.seed, 2, np.arange(self.N), self.q) / math.sqrt(self.q)
+
+
+def sketches(G, cand, q, seed, kind='block', chunk=4_000_000, on_device=False, precision='fp64'):
+    """(Z n x q, F = C0^T U n x q, Gc N x q) of polar_gpu.py:148-165.  kind 'block' / 'gauss' follow
+    the reference on the host; kind 'rademacher' uses the counter hash (+-1/sqrt(q)), which the
+    GPU can generate in place (on_device=True: Z and F are device tensors, Gc rows are generated
+    per use)."""
+    n = G.n
+    if kind == 'rademacher':
+        scale = 1.0 / math.sqrt(q)
+        if on_device:
+            lib = L.require_gpu()
+            dev = _device()
+            dtype = _dt(precision)
+            Z = torch.empty((n, q), dtype=dtype, device=dev)
+            F = torch.empty((n, q), dtype=dtype, device=dev)
+            L.check(lib.polar_rademacher(L.dtype_code(dtype), int(seed), 0, None, n, q, scale, L.ptr(Z), q,
+                                         L.stream_ptr()), 'polar_rademacher')
+            op = Op(G, precision=dtype)
+            L.check(lib.polar_edge_sketch(op.c_op(dtype), int(seed), q, scale, L.ptr(F), q, L.stream_ptr()),
+                    'polar_edge_sketch')
+            return Z, F, RademacherRows(seed, q, len(cand))
+        Z = _rad_rows(seed, 0, np.arange(n), q) * scale
+        F = np.zeros((n, q))
+        u = np.minimum(G.edges[:, 0], G.edges[:, 1])
+        v = np.maximum(G.edges[:, 0], G.edges[:, 1])
+        sw = np.sqrt(G.weights)
+        for lo in range(0, G.m, chunk):
+            hi = min(G.m, lo + chunk)
+            U = _rad_rows(seed, 1, u[lo:hi] * n + v[lo:hi], q) * (scale * sw[lo:hi, None])
+            np.add.at(F, u[lo:hi], U)
+            np.add.at(F, v[lo:hi], -U)
+        return Z, F, RademacherRows(seed, q, len(cand)).full()
+    rng = np.random.default_rng(seed)
+    if kind == 'gauss':
+        Z = rng.normal(size=(n, q)) / np.sqrt(q)
+    else:
+        Z = _block_rows(n, q, rng, center=True)
+    F = np.zeros((n, q))
+    u, v = G.edges[:, 0], G.edges[:, 1]
+    sw = np.sqrt(G.weights)
+    for lo in range(0, G.m, chunk):
+        hi = min(G.m, lo + chunk)
+        U = rng.normal(size=(hi - lo, q)) / np.sqrt(q) if kind == 'gauss' else _block_rows(hi - lo, q, rng)
+        U *= sw[lo:hi, None]
+        C = sp.coo_matrix((np.r_[np.ones(hi - lo), -np.ones(hi - lo)],
+                           (np.r_[u[lo:hi], v[lo:hi]], np.r_[np.arange(hi - lo), np.arange(hi - lo)])),
+                          shape=(n, hi - lo)).tocsr()
+        F += C @ U
+    Gc = rng.normal(size=(len(cand), q)) / np.sqrt(q) if kind == 'gauss' else _block_rows(len(cand), q, rng)
+    return Z, F, Gc
+
+
+def chunk_width(n, p, nA, budget=10e9):
+    """polar_gpu.py:169-173 (the reference's column budget; kept for API parity)."""
+    per_col = 8.0 * (10 * n + 4 * p * nA)
+    cap = 16 if n > 3e6 else 64
+    return int(max(8, min(cap, budget // per_col)))
+
+
+def bank_width(n, p, nA, q, dtype, budget=None):
+    """Columns per bank block solve on one MI355X: as wide as one wave covers (128 FP64 / 256
+    FP32 columns) within a memory budget for the shifted P block and the CG vectors."""
+    esz = 8 if dtype == torch.float64 else 4
+    if budget is None:
+        free, _ = torch.cuda.mem_get_info()
+        budget = 0.5 * free
+    per_col = esz * (4 * n + p * nA) + 64
+    w = int(min(2 * q, _max_cols(dtype), max(1, budget // per_col)))
+    if w >= 8:
+        w -= w % 8
+    return max(1, w)
+
+
+def _sync():
+    torch.cuda.synchronize()
+
+
+# ------------------------------------------------------------------------------------------------
+def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=None, naive=False,
+          seed=2026, kind='block', trace=None, time_limit=None, poles=None, width=None,
+          freeze=True, precision='fp64'):
+    """POLAR greedy, cache backend (polar_gpu.py:180-254), on the GPU.
+
+    Returns dict(sequence, seconds, spmv, rhs, p, rounds, truncated, marginals, trace_estimate,
+    solves, precision)."""
+    lib = L.require_gpu()
+    dev = _device()
+    dtype = _dt(precision)
+    dcode = L.dtype_code(dtype)
+    stream = L.stream_ptr()
+    n, N = G.n, len(cand)
+    sigma = G.sigma
+    if poles is None:
+        tau, om, _ = globals()['poles'](obj, s, sigma, G.Lam(k), eps)
+    else:
+        tau, om = np.asarray(poles[0], float), np.asarray(poles[1], float)
+    p = len(tau)
+    if sketch is None:
+        sketch = sketches(G, cand, q, seed, kind)
+    Z, F, Gc = sketch
+    q = Z.shape[1]
+    cand = np.asarray(cand)
+    active_h = np.unique(cand)                                                    # :191
+    es_h = np.searchsorted(active_h, cand)
+    nA = len(active_h)
+    active = torch.from_numpy(active_h.astype(np.int32)).to(dev)
+    ea = torch.from_numpy(es_h[:, 0].astype(np.int32)).to(dev)
+    eb = torch.from_numpy(es_h[:, 1].astype(np.int32)).to(dev)
+    tau_d = torch.from_numpy(tau).to(dev)
+    om_d = torch.from_numpy(om).to(dev)
+    RHS = torch.empty((n, 2 * q), dtype=dtype, device=dev)                         # [Z | F]
+    RHS[:, :q] = torch.as_tensor(Z, device=dev).to(dtype)
+    RHS[:, q:] = torch.as_tensor(F, device=dev).to(dtype)
+    Gc_dev = None
+    if isinstance(Gc, torch.Tensor):
+        Gc_dev = Gc.to(dev, torch.float64)
+    elif not isinstance(Gc, RademacherRows):
+        Gc = np.asarray(Gc, dtype=np.float64)
+    op = Op(G, precision=dtype)
+    cnt = Counter()
+    live = np.ones(N, bool)
+    dead = torch.zeros(N, dtype=torch.uint8, device=dev)
+    seq, marg = [], []
+    traces = []
+    want_trace = obj != 'logdet'
+    objc = 2 if obj == 'logdet' else 0
+    score = torch.empty(N, dtype=torch.float64, device=dev)
+    sel_ws = torch.empty(lib.polar_select_ws_bytes(N), dtype=torch.uint8, device=dev)
+    sel_out = torch.empty(64, dtype=torch.int32, device=dev)
+    coef = torch.empty((p, 2 * q + 3), dtype=torch.float64, device=dev)
+    wX = torch.empty((p, nA), dtype=torch.float64, device=dev)
+    wY = torch.empty((p, nA), dtype=torch.float64, device=dev)
+    gvec = torch.empty(max(B, 1) + 2, dtype=torch.float64, device=dev)
+    gc_row = torch.empty(q, dtype=torch.float64, device=dev)
+    tr_hist = []
+    _sync()
+    t0 = time.perf_counter()                                                      # :196
+
+    if width is None:
+        width = bank_width(n, p, nA, q, dtype)
+    XY = torch.empty((p, nA, 2 * q), dtype=dtype, device=dev)
+
+    def bank():                                                                    # :200-206
+        terms = []
+        for lo in range(0, 2 * q, width):
+            hi = min(2 * q, lo + width)
+            res = msolve(op, RHS[:, lo:hi], tau, active, tol, counter=cnt, freeze=freeze,
+                         trace=want_trace and lo < q, out=XY[:, :, lo:hi])
+            if want_trace and lo < q:
+                ncol = min(q, hi) - lo
+                terms.append((om[:, None] * res[3][:, :ncol].cpu().numpy()).ravel())
+        tr = math.fsum(np.concatenate(terms)) if terms else 0.0   # correctly rounded
+        X, Y = XY[:, :, :q], XY[:, :, q:]
+        h = torch.empty((p, N), dtype=torch.float64, device=dev)
+        t = torch.empty((p, N), dtype=torch.float64, device=dev)
+        L.check(lib.polar_pair_sqnorm(dcode, L.ptr(X), X.stride(0), X.stride(1), p, q, L.ptr(ea), L.ptr(eb), N,
+                                      L.ptr(h), stream), 'pair_sqnorm')
+        L.check(lib.polar_pair_sqnorm(dcode, L.ptr(Y), Y.stride(0), Y.stride(1), p, q, L.ptr(ea), L.ptr(eb), N,
+                                      L.ptr(t), stream), 'pair_sqnorm')
+        return X, Y, h, t, tr
+
+    X, Y, h, t, tr = bank()
+    traces.append(tr if want_trace else None)
+    for rnd in range(k):
+        if rnd and time_limit is not None and time.perf_counter() - t0 > time_limit:   # :215
+            break
+        if naive and rnd:
+            X, Y, h, t, tr = bank()
+            traces.append(tr if want_trace else None)
+        L.check(lib.polar_proxy(objc, L.ptr(h), L.ptr(t), L.ptr(tau_d), L.ptr(om_d), p, N, sigma, L.ptr(dead),
+                                L.ptr(score), stream), 'proxy')
+        Br = 1 if naive else min(B, int(live.sum()))
+        short = np.zeros(Br, dtype=np.int32)
+        L.check(lib.polar_select_topb(L.ptr(score), N, Br, L.ptr(sel_out), short.ctypes.data_as(L.vp),
+                                      L.ptr(sel_ws), sel_ws.numel(), stream), 'select_topb')   # :221
+        if naive:
+            e = int(short[0])
+            marg.append(float(score[e].item()))
+        else:
+            a_idx = torch.from_numpy(cand[short, 0].astype(np.int32)).to(dev)
+            b_idx = torch.from_numpy(cand[short, 1].astype(np.int32)).to(dev)
+            Wb = torch.empty((n, Br), dtype=dtype, device=dev)
+            L.check(lib.polar_pair_rhs(dcode, n, Br, L.ptr(a_idx), L.ptr(b_idx), L.ptr(Wb), stream), 'pair_rhs')
+            need_track = obj != 'logdet'
+            U, hh, _ = msolve(op, Wb, tau, active, tol, track=need_track, counter=cnt, freeze=freeze)[:3]
+            sa = torch.from_numpy(es_h[short, 0].astype(np.int32)).to(dev)
+            sb = torch.from_numpy(es_h[short, 1].astype(np.int32)).to(dev)
+            win = np.zeros(1, dtype=np.int32)
+            val = np.zeros(1, dtype=np.float64)
+            L.check(lib.polar_rerank_pick(objc, dcode, L.ptr(U), U.stride(0), U.stride(1), p, Br, L.ptr(sa), L.ptr(sb),
+                                          L.ptr(hh), L.ptr(om_d), L.ptr(gvec), win.ctypes.data_as(L.vp),
+                                          val.ctypes.data_as(L.vp), stream), 'rerank_pick')
+            j = 0 if Br == 1 else int(win[0])
+            e = int(short[j])
+            u = U[:, :, j]
+            marg.append(float(val[0]))
+        if trace is not None:
+            trace.append(dict(shortlist=[int(x) for x in short], chosen=e))
+        seq.append(e)
+        live[e] = False
+        dead[e] = 1
+        aa, bb = (int(x) for x in es_h[e])
+        a, b = (int(x) for x in cand[e])
+        if Gc_dev is not None:
+            gc_row.copy_(Gc_dev[e])
+        else:
+            gc_row.copy_(torch.from_numpy(np.ascontiguousarray(Gc[e], dtype=np.float64)))
+        if not naive and rnd + 1 < k:                                              # :242-249
+            L.check(lib.polar_maint_prep(dcode, L.ptr(X), L.ptr(Y), X.stride(0), X.stride(1), p, q, L.ptr(u),
+                                         u.stride(0), u.stride(1), aa, bb, L.ptr(gc_row), L.ptr(coef), stream),
+                    'maint_prep')
+            L.check(lib.polar_maint_rank1(dcode, L.ptr(X), L.ptr(Y), X.stride(0), X.stride(1), p, nA, q, L.ptr(u),
+                                          u.stride(0), u.stride(1), L.ptr(coef), L.ptr(wX), L.ptr(wY), stream),
+                    'maint_rank1')
+            L.check(lib.polar_maint_cache(dcode, L.ptr(u), u.stride(0), u.stride(1), p, nA, N, L.ptr(ea), L.ptr(eb),
+                                          L.ptr(coef), q, L.ptr(wX), L.ptr(wY), L.ptr(h), L.ptr(t), stream),
+                    'maint_cache')
+            if want_trace:
+                tr_hist.append(coef[:, 2 * q:2 * q + 2].clone())
+        Fv = RHS[:, q:]
+        L.check(lib.polar_rows_axpy(dcode, L.ptr(Fv), Fv.stride(0), q, a, b, L.ptr(gc_row), stream), 'rows_axpy')  # :250
+        op.add(a, b)                                                               # :251
+    _sync()
+    seconds = time.perf_counter() - t0
+    if want_trace and not naive:
+        for dn in tr_hist:
+            dn_h = dn.cpu().numpy()
+            traces.append(traces[-1] - math.fsum(om * dn_h[:, 1] / dn_h[:, 0]))
+    return dict(sequence=seq, seconds=seconds, spmv=cnt.spmv, rhs=cnt.rhs, p=p, rounds=len(seq),
+                truncated=len(seq) < k, marginals=marg, trace_estimate=traces[:max(len(seq), 1)],
+                solves=list(cnt.solves), precision='fp64' if dtype == torch.float64 else 'fp32')
+
+
+# ------------------------------------------------------------------------------------------------
+def degree_greedy(G, cand, k):
+    """polar_gpu.py:258-265 (host)."""
+    deg = np.bincount(G.edges[:, 0], G.weights, G.n) + np.bincount(G.edges[:, 1], G.weights, G.n)
+    live = np.ones(len(cand), bool)
+    seq = []
+    for _ in range(k):
+        val = np.where(live, deg[cand[:, 0]] + deg[cand[:, 1]], np.inf)
+        e = int(np.argmin(val))
+        seq.append(e)
+        live[e] = False
+        deg[cand[e, 0]] += 1
+        deg[cand[e, 1]] += 1
+    return seq
+
+
+def random_set(cand, k, seed):
+    """polar_gpu.py:268-269 (host)."""
+    return np.random.default_rng(seed).choice(len(cand), k, replace=False).tolist()
+
+
+def true_marginals(G, cand, seq, s, obj='frac', eps=1e-9, tol=1e-10, kmax=None, poles=None,
+                   precision='fp64'):
+    """Certified marginals along a sequence (polar_gpu.py:273-291): single-RHS multi-shift CG on
+    the GPU with active=[a, b] and g = sum_j om_j ||u_j||^2 / (1 + b^T u_j)."""
+    L.require_gpu()
+    dev = _device()
+    kmax = kmax or len(seq)
+    if poles is None:
+        tau, om, qerr = globals()['poles'](obj, s, G.sigma, G.Lam(kmax), eps)
+    else:
+        tau, om = np.asarray(poles[0], float), np.asarray(poles[1], float)
+        qerr = float('nan')
+    op = Op(G, precision=precision)
+    cnt = Counter()
+    out = []
+    for e in seq:
+        a, b = int(cand[e, 0]), int(cand[e, 1])
+        w = torch.zeros((G.n, 1), dtype=op.dtype, device=dev)
+        w[a, 0] = 1.0
+        w[b, 0] = -1.0
+        U, hh, _ = msolve(op, w, tau, np.asarray([a, b]), tol, track=obj != 'logdet', counter=cnt)[:3]
+        bu = (U[:, 0, 0].double() - U[:, 1, 0].double()).cpu().numpy()
+        if obj == 'logdet':
+            g = float(np.log1p(bu[0]))
+        else:
+            g = float(np.sum(om * hh[:, 0].cpu().numpy() / (1 + bu)))
+        out.append(g)
+        op.add(a, b)
+    rho = tol * float(np.max((G.Lam(kmax) + tau) / (G.sigma + tau)))
+    return np.array(out), dict(quadrature_bound=qerr, residual_rho=rho, p=len(tau), spmv=cnt.spmv)

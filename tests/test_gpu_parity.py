@@ -1,0 +1,228 @@
+"""GPU parity of the HIP path against the oracle (oracle/polar_oracle.py) and SciPy.
+
+The oracle runs with arith='cr' (the pinned arithmetic the kernels implement: double-double
+reductions, no FMA, fixed operator order), so the FP64 HIP path is compared to it at:
+  * integer/index results (sequences, shortlists, iteration counts, spmv/rhs counters): exact;
+  * FP64 msolve outputs, marginals, trace estimates: 1e-13 relative (bitwise in practice);
+  * FP64 operator products vs SciPy (a different summation order): 1e-12 of the row scale;
+  * FP32-mixed marginals vs the FP64 oracle: 1e-4 relative (stated in the test).
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+torch = pytest.importorskip('torch')
+
+from oracle import polar_oracle as O  # noqa: E402
+from polar_amd import configs, graphs  # noqa: E402
+from polar_amd import polar_gpu as pg  # noqa: E402
+import scalability_inputs as SI  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / 'golden'
+
+
+def hub_graph(n=3000, seed=5, weighted=False):
+    """Power-law graph plus two hubs of degree > 2*256 (exercises the long-row chunk path)."""
+    e = graphs.chung_lu(n, 2.3, 8.0, seed)
+    hubs = np.array([[0, j] for j in range(1, 900)] + [[7, j] for j in range(300, 1000)])
+    e = graphs.simple_edges(np.concatenate([e, hubs]), n)
+    w = np.random.default_rng(seed).uniform(0.5, 1.5, len(e)) if weighted else None
+    return n, e, w
+
+
+def dense_patch(n, pairs):
+    P = np.zeros((n, n))
+    for a, b in pairs:
+        P[a, a] += 1
+        P[b, b] += 1
+        P[a, b] -= 1
+        P[b, a] -= 1
+    return P
+
+
+@pytest.mark.parametrize('weighted', [False, True])
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+@pytest.mark.parametrize('R', [1, 3, 8, 16, 64, 128])
+def test_spmm_matches_scipy(weighted, precision, R):
+    n, e, w = hub_graph(weighted=weighted)
+    G = pg.Graph(n, e, 0.3, weights=w)
+    assert len(G.schedule()['long_rows']) >= 2
+    op = pg.Op(G, precision=precision)
+    pairs = [(3, 2500), (0, 2999), (7, 8)]
+    for a, b in pairs:
+        op.add(a, b)
+    rng = np.random.default_rng(R)
+    X = rng.standard_normal((n, R))
+    M = G.M0.get().toarray() + dense_patch(n, pairs)
+    shift = 0.0125
+    ref = M @ X + shift * X
+    dt = torch.float64 if precision == 'fp64' else torch.float32
+    Y = op(torch.from_numpy(X).to('cuda', dt), shift).double().cpu().numpy()
+    scale = np.abs(M).sum(1, keepdims=True) * np.abs(X).max()
+    tol = 1e-12 if precision == 'fp64' else 2e-6
+    assert np.max(np.abs(Y - ref) / scale) < tol
+
+
+def test_spmm_dropin_no_patch():
+    n, e, _ = hub_graph()
+    G = pg.Graph(n, e, 1.0)
+    X = np.random.default_rng(0).standard_normal((n, 16))
+    Y = pg.spmm(G.M0, torch.from_numpy(X).cuda()).cpu().numpy()
+    assert np.allclose(Y, G.M0.get() @ X, rtol=0, atol=1e-10)
+
+
+def _small_problem(seed=0):
+    n = 1500
+    e = graphs.barabasi_albert(n, 4, seed)
+    G0 = O.Graph(n, e, 0.1)
+    G = pg.Graph(n, e, 0.1)
+    tau, om, _ = pg.fixed_rule(0.5, 12, 0.1, G.Lam(5))
+    return n, e, G0, G, tau, om
+
+
+@pytest.mark.parametrize('freeze', [True, False])
+@pytest.mark.parametrize('R', [1, 6, 24])
+def test_msolve_matches_oracle(freeze, R):
+    n, e, G0, G, tau, om = _small_problem()
+    rng = np.random.default_rng(R)
+    W = rng.standard_normal((n, R))
+    W[:, 0] = 0.0 if R > 1 else W[:, 0]        # an all-zero column stays inactive (polar_gpu.py:94)
+    active = np.sort(rng.choice(n, 300, replace=False))
+    op0 = O.Op(G0, arith='cr')
+    op0.add(5, 900)
+    op = pg.Op(G)
+    op.add(5, 900)
+    Xo, ho, ito, teo = O.msolve(op0, W, tau, active, 1e-9, track=True, freeze=freeze, trace=True)
+    Xg, hg, itg, teg = pg.msolve(op, torch.from_numpy(W).cuda(), tau, active, 1e-9, track=True,
+                                 freeze=freeze, trace=True)
+    assert np.array_equal(ito, itg), (ito, itg)
+    Xg = Xg.cpu().numpy()
+    assert np.max(np.abs(Xg - Xo)) <= 1e-13 * np.max(np.abs(Xo))
+    assert np.max(np.abs(hg.cpu().numpy() - ho)) <= 1e-13 * np.max(np.abs(ho))
+    assert np.max(np.abs(teg.cpu().numpy() - teo)) <= 1e-13 * np.max(np.abs(teo))
+
+
+def test_msolve_iteration_cap():
+    n, e, G0, G, tau, om = _small_problem()
+    W = np.random.default_rng(0).standard_normal((n, 4))
+    with pytest.raises(RuntimeError, match='multi-shift CG iteration cap'):
+        pg.msolve(pg.Op(G), torch.from_numpy(W).cuda(), tau, np.arange(10), 1e-12, maxit=16)
+
+
+def test_select_topb_ties_and_dead():
+    from polar_amd import _lib as L
+    lib = L.require_gpu()
+    rng = np.random.default_rng(1)
+    s = np.round(rng.standard_normal(100_003), 1)
+    s[rng.choice(len(s), 500, replace=False)] = -np.inf
+    sd = torch.from_numpy(s).cuda()
+    for Br in (1, 5, 16, 64):
+        out = torch.empty(64, dtype=torch.int32, device='cuda')
+        host = np.zeros(Br, dtype=np.int32)
+        ws = torch.empty(lib.polar_select_ws_bytes(len(s)), dtype=torch.uint8, device='cuda')
+        L.check(lib.polar_select_topb(L.ptr(sd), len(s), Br, L.ptr(out), host.ctypes.data_as(L.vp), L.ptr(ws),
+                                      ws.numel(), L.stream_ptr()))
+        assert host.tolist() == np.argsort(-s, kind='stable')[:Br].tolist()
+
+
+def test_rademacher_device_matches_host():
+    n, e, w = hub_graph(n=2000, weighted=True)
+    G = pg.Graph(n, e, 0.1, weights=w)
+    cand = graphs.candidates(n, e, 300, 3)
+    Zh, Fh, Gh = pg.sketches(G, cand, 24, 11, kind='rademacher')
+    Zd, Fd, Gd = pg.sketches(G, cand, 24, 11, kind='rademacher', on_device=True)
+    assert np.array_equal(Zd.cpu().numpy(), Zh)
+    assert np.max(np.abs(Fd.cpu().numpy() - Fh)) < 1e-12
+    assert np.array_equal(np.stack([Gd[i] for i in range(len(cand))]), Gh)
+    Zo, Fo, Go = O.sketches(O.Graph(n, e, 0.1, weights=w), cand, 24, 11, kind='rademacher')
+    assert np.array_equal(Zo, Zh) and np.array_equal(Go, Gh) and np.max(np.abs(Fo - Fh)) < 1e-12
+
+
+def _c1_inputs(scale=1.0):
+    c = configs.make('c1', scale)
+    Go = O.Graph(c['n'], c['edges'], c['sigma'])
+    sk = O.sketches(Go, c['cand'], c['q'], c['seed'], kind='rademacher')
+    return c, Go, sk
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def test_polar_c1_fp64_parity():
+    """c1 (BA n=2000, q=32, 16 poles, N=5000, k=10, B=1): identical sequence, identical spmv / rhs
+    counters and per-solve column-iterations; marginals and trace estimates within 1e-13."""
+    c, Go, sk = _c1_inputs()
+    kw = dict(q=c['q'], B=c['B'], tol=c['tol'], sketch=sk, poles=(c['tau'], c['om']))
+    ro = O.polar(Go, c['cand'], c['k'], c['s'], width=64, arith='cr', **kw)
+    G = pg.Graph(c['n'], c['edges'], c['sigma'])
+    rg = pg.polar(G, c['cand'], c['k'], c['s'], **kw)
+    assert rg['sequence'] == ro['sequence']
+    assert rg['rhs'] == ro['rhs'] == 2 * c['q'] + c['k'] * c['B']
+    assert rg['spmv'] == ro['spmv']
+    assert _rel(rg['marginals'], ro['marginals']) < 1e-13
+    assert _rel(rg['trace_estimate'], ro['trace_estimate']) < 1e-13
+
+
+def test_polar_b4_and_naive_parity():
+    c = configs.make('c2', 0.02)
+    Go = O.Graph(c['n'], c['edges'], c['sigma'])
+    sk = O.sketches(Go, c['cand'], 16, c['seed'], kind='rademacher')
+    G = pg.Graph(c['n'], c['edges'], c['sigma'])
+    kw = dict(q=16, tol=c['tol'], sketch=sk, poles=(c['tau'], c['om']))
+    for B, naive in ((4, False), (1, False), (1, True)):
+        ro = O.polar(Go, c['cand'], 6, c['s'], B=B, naive=naive, width=32, arith='cr', **kw)
+        rg = pg.polar(G, c['cand'], 6, c['s'], B=B, naive=naive, **kw)
+        assert rg['sequence'] == ro['sequence'], (B, naive)
+        assert rg['spmv'] == ro['spmv'], (B, naive)
+        assert _rel(rg['marginals'], ro['marginals']) < 1e-13
+        assert _rel(rg['trace_estimate'], ro['trace_estimate']) < 1e-13
+
+
+@pytest.mark.parametrize('idx', [0, 1, 5])
+def test_reference_fixture_sequences(idx):
+    """synthetic/scalability/results.json sequences (Naive-ShiftedCG and Inc-Full) through the HIP
+    path: single-RHS variant = no pole freezing, quad rule p=20, Gaussian q=24 sketches."""
+    fx = json.loads((GOLD / 'scalability_fixture.json').read_text())['configurations'][idx]
+    d = SI.make(fx['n'], fx['topology'], fx['support'])
+    tau, om = SI.quad(0.5, 20)
+    G = pg.Graph(d['n'], d['edges'], 1.0)
+    sk = (d['Z'], d['F'], d['G'])
+    for naive, name in ((True, 'Naive-ShiftedCG'), (False, 'Inc-Full')):
+        out = pg.polar(G, d['cand'], 15, 0.5, B=1, tol=1e-7, sketch=sk, naive=naive, poles=(tau, om),
+                       width=48, freeze=False)
+        assert out['sequence'] == fx['methods'][name]['sequence'], name
+        sp_ = out['spmv'] if naive else sum(out['solves'][:-1])
+        assert abs(sp_ - fx['methods'][name]['spmvs']) <= 0.01 * fx['methods'][name]['spmvs']
+
+
+def test_fp32_mixed_vs_fp64():
+    """FP32 vectors / FP64 recurrences: same c1 sequence; marginals within 1e-4 relative and
+    trace estimate within 1e-4 relative of the FP64 oracle."""
+    c, Go, sk = _c1_inputs()
+    kw = dict(q=c['q'], B=c['B'], tol=c['tol'], sketch=sk, poles=(c['tau'], c['om']))
+    ro = O.polar(Go, c['cand'], c['k'], c['s'], width=64, arith='cr', **kw)
+    G = pg.Graph(c['n'], c['edges'], c['sigma'])
+    rg = pg.polar(G, c['cand'], c['k'], c['s'], precision='fp32', **kw)
+    assert rg['sequence'] == ro['sequence']
+    assert _rel(rg['marginals'], ro['marginals']) < 1e-4
+    assert _rel(rg['trace_estimate'], ro['trace_estimate']) < 1e-4
+
+
+def test_true_marginals_vs_eigenvalues():
+    n = 600
+    e = graphs.barabasi_albert(n, 3, 4)
+    G = pg.Graph(n, e, 0.5)
+    cand = graphs.candidates(n, e, 200, 9)
+    seq = pg.random_set(cand, 5, 3)
+    g, cert = pg.true_marginals(G, cand, seq, 0.5)
+    exact = np.diff(np.r_[0.0, O.eval_prefix(G.M0.get().toarray(), cand, seq, 0.5)])
+    assert np.max(np.abs(g - exact) / exact) < 1e-7
+    g_o, cert_o = O.true_marginals(O.Graph(n, e, 0.5), cand, seq, 0.5, arith='cr')
+    assert _rel(g, g_o) < 1e-13
+    assert cert['p'] == cert_o['p'] and abs(cert['residual_rho'] - cert_o['residual_rho']) < 1e-20
