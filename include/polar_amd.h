@@ -51,8 +51,10 @@ enum polar_status {
 /* Operator y = (diag + shift) x - A_w x + patch(x): the graph Laplacian M0 = diag(deg+sigma) - A
  * of realworld/polar_gpu.py:47-55 stored as diagonal + off-diagonal adjacency (value-free when
  * every weight is 1), plus the unit-weight added edges of polar_gpu.py:68-79 as a row patch.
- * Long rows (degree > POLAR_CHUNK) are split into chunks; their schedule is built by the host. */
-#define POLAR_CHUNK 256
+ * Row sums run in adjacency order; rows longer than `chunk` entries are split into chunks of
+ * `chunk` entries whose partial sums are added in chunk order (the schedule is built by the host).
+ * POLAR_CHUNK is the default chunk length. */
+#define POLAR_CHUNK 64
 typedef struct polar_op {
   int64_t n;
   int32_t dtype;              /* polar_dtype of weights, diag and all vectors */
@@ -61,10 +63,10 @@ typedef struct polar_op {
   const int32_t* indices;     /* [nnz]  off-diagonal columns */
   const void* weights;        /* [nnz]  off-diagonal weights (positive), or NULL */
   const void* diag;           /* [n]    weighted degree + sigma */
-  int32_t n_long;             /* rows with degree > POLAR_CHUNK */
+  int32_t n_long;             /* rows with degree > chunk */
   int32_t n_extra;            /* chunks of long rows beyond their first */
   int32_t n_partial;          /* total chunks of long rows */
-  int32_t pad0;
+  int32_t chunk;              /* long-row chunk length (>= 8) */
   const int32_t* long_rows;   /* [n_long] ascending row ids */
   const int32_t* long_base;   /* [n_long] first partial slot of each long row */
   const int32_t* extra;       /* [n_extra][2]: (long index, chunk index >= 1) */

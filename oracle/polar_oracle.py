@@ -13,8 +13,8 @@ It restates, in NumPy/SciPy FP64, the semantics of the reference's ``realworld/p
 * ``polar(..., poles=(tau, om))`` pole override and per-round marginal / trace-estimate outputs.
 * ``arith='cr'``: the same algorithm with a pinned arithmetic: every reduction (inner products,
   squared norms, row dots) is summed in double-double and rounded once (order-independent), the
-  operator is applied as ((diag + shift) * x) - (sequential neighbour sum, rows longer than 256
-  split in 256-entry chunks added in order) + added-edge terms in addition order, no fused
+  operator is applied as ((diag + shift) * x) - (sequential neighbour sum, rows longer than 64
+  split in 64-entry chunks added in order) + added-edge terms in addition order, no fused
   multiply-add anywhere, and scalar sums over poles run in pole order.  The HIP path implements
   exactly this arithmetic, so its FP64 results equal the oracle's bit for bit (up to ties of the
   double-double rounding).  ``arith='numpy'`` (default) keeps NumPy's native reduction order,
@@ -124,7 +124,7 @@ def ddsum(P, axis=0):
     return hi[0] + lo[0]
 
 
-CHUNK = 256
+CHUNK = 64
 
 
 class _Adjacency:

@@ -14,7 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / 'libpolar_amd.so'
 POLAR_F64, POLAR_F32 = 0, 1
 POLAR_TRACK, POLAR_TRACE, POLAR_NOFREEZE = 1, 2, 4
 POLAR_E_ITERCAP, POLAR_E_ARG = -2, -3
-CHUNK = 256  # POLAR_CHUNK
+CHUNK = 64  # POLAR_CHUNK: default long-row chunk length
 
 EXPORTS = [
     'polar_abi_version', 'polar_last_error', 'polar_spmm_ws_bytes', 'polar_spmm',
@@ -35,7 +35,7 @@ class PolarOp(ctypes.Structure):
     _fields_ = [
         ('n', i64), ('dtype', i32), ('weighted', i32),
         ('indptr', vp), ('indices', vp), ('weights', vp), ('diag', vp),
-        ('n_long', i32), ('n_extra', i32), ('n_partial', i32), ('pad0', i32),
+        ('n_long', i32), ('n_extra', i32), ('n_partial', i32), ('chunk', i32),
         ('long_rows', vp), ('long_base', vp), ('extra', vp),
         ('n_patch', i32), ('pad1', i32),
         ('patch_rows', vp), ('patch_ptr', vp), ('patch_cols', vp),

@@ -85,40 +85,15 @@ __global__ __launch_bounds__(kBlock) void k_proxy(int obj, const double* __restr
 // ------------------------------------------------------------------------------------------------
 // Top-B in the order (score desc, index asc): pass b selects the maximum among elements that come
 // strictly after the pass b-1 pick.  sel_s/sel_i: device slots of previous picks.
-struct Best {
-  double s;
-  int64_t i;
-};
-
 __device__ __forceinline__ bool better(double s1, int64_t i1, double s2, int64_t i2) {
   // true if (s1, i1) precedes (s2, i2) in (score desc, index asc)
   return (s1 > s2) || (s1 == s2 && i1 < i2);
 }
 
-__global__ __launch_bounds__(kBlock) void k_select_pass(const double* __restrict__ score, int64_t N, int pass,
-                                                        double* __restrict__ sel_s, int32_t* __restrict__ out,
-                                                        double* __restrict__ bs, int64_t* __restrict__ bi,
-                                                        unsigned* __restrict__ counter) {
-  const bool has_prev = pass > 0;
-  const double ps = has_prev ? sel_s[pass - 1] : 0.0;
-  const int64_t pi = has_prev ? (int64_t)out[pass - 1] : -1;
-  double best_s = -INFINITY;
-  int64_t best_i = -1;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (int64_t)gridDim.x * blockDim.x) {
-    const double s = score[e];
-    const bool after = !has_prev || better(ps, pi, s, e);
-    if (after && (best_i < 0 || better(s, e, best_s, best_i))) {
-      best_s = s;
-      best_i = e;
-    }
-  }
-  __shared__ double ss[kBlock];
-  __shared__ int64_t si[kBlock];
-  ss[threadIdx.x] = best_s;
-  si[threadIdx.x] = best_i;
+__device__ __forceinline__ void best_tree(double* ss, int64_t* si) {
   __syncthreads();
   for (int o = kBlock / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) {
+    if ((int)threadIdx.x < o) {
       const int64_t i2 = si[threadIdx.x + o];
       if (i2 >= 0 && (si[threadIdx.x] < 0 || better(ss[threadIdx.x + o], i2, ss[threadIdx.x], si[threadIdx.x]))) {
         ss[threadIdx.x] = ss[threadIdx.x + o];
@@ -127,30 +102,49 @@ __global__ __launch_bounds__(kBlock) void k_select_pass(const double* __restrict
     }
     __syncthreads();
   }
+}
+
+// Pass `pass`, stage 1: best element (after the previous pick) of each workgroup's slice.
+__global__ __launch_bounds__(kBlock) void k_select_blocks(const double* __restrict__ score, int64_t N, int pass,
+                                                          const double* __restrict__ sel_s,
+                                                          const int32_t* __restrict__ out, double* __restrict__ bs,
+                                                          int64_t* __restrict__ bi) {
+  const bool has_prev = pass > 0;
+  const double ps = has_prev ? sel_s[pass - 1] : 0.0;
+  const int64_t pi = has_prev ? (int64_t)out[pass - 1] : -1;
+  double best_s = -INFINITY;
+  int64_t best_i = -1;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (int64_t)gridDim.x * blockDim.x) {
+    const double sc = score[e];
+    const bool after = !has_prev || better(ps, pi, sc, e);
+    if (after && (best_i < 0 || better(sc, e, best_s, best_i))) {
+      best_s = sc;
+      best_i = e;
+    }
+  }
+  __shared__ double ss[kBlock];
+  __shared__ int64_t si[kBlock];
+  ss[threadIdx.x] = best_s;
+  si[threadIdx.x] = best_i;
+  best_tree(ss, si);
   if (threadIdx.x == 0) {
     bs[blockIdx.x] = ss[0];
     bi[blockIdx.x] = si[0];
   }
-  __threadfence();
-  __syncthreads();
-  __shared__ int last;
-  if (threadIdx.x == 0) last = (atomicAdd(counter, 1u) == gridDim.x - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+}
+
+// Stage 2 (one workgroup): best of the workgroup results -> out[pass], sel_s[pass].
+__global__ __launch_bounds__(kBlock) void k_select_final(int pass, int nb, const double* __restrict__ bs,
+                                                         const int64_t* __restrict__ bi, double* __restrict__ sel_s,
+                                                         int32_t* __restrict__ out) {
+  __shared__ double ss[kBlock];
+  __shared__ int64_t si[kBlock];
+  ss[threadIdx.x] = (int)threadIdx.x < nb ? bs[threadIdx.x] : -INFINITY;
+  si[threadIdx.x] = (int)threadIdx.x < nb ? bi[threadIdx.x] : -1;
+  best_tree(ss, si);
   if (threadIdx.x == 0) {
-    double b_s = -INFINITY;
-    int64_t b_i = -1;
-    for (unsigned b = 0; b < gridDim.x; ++b) {
-      const int64_t i2 = bi[b];
-      if (i2 >= 0 && (b_i < 0 || better(bs[b], i2, b_s, b_i))) {
-        b_s = bs[b];
-        b_i = i2;
-      }
-    }
-    sel_s[pass] = b_s;
-    out[pass] = (int32_t)b_i;
-    *counter = 0;
+    sel_s[pass] = ss[0];
+    out[pass] = (int32_t)si[0];
   }
 }
 
@@ -455,11 +449,13 @@ int polar_select_topb(const double* score, int64_t N, int32_t Br, int32_t* out, 
   double* bs = (double*)w;
   int64_t* bi = (int64_t*)(w + kMaxGrid * 8);
   double* sel_s = (double*)(w + kMaxGrid * 16);
-  unsigned* counter = (unsigned*)(w + kMaxGrid * 16 + 64 * 8);
-  POLAR_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned), st));
-  const int grid = grid_for(N, kBlock, kMaxGrid);
+  const int grid = grid_for(N, kBlock, kBlock);   // <= 256 workgroup results per pass
   for (int b = 0; b < Br; ++b) {
-    hipLaunchKernelGGL(k_select_pass, dim3(grid), dim3(kBlock), 0, st, score, N, b, sel_s, out, bs, bi, counter);
+    hipLaunchKernelGGL(k_select_blocks, dim3(grid), dim3(kBlock), 0, st, score, N, b, (const double*)sel_s,
+                       (const int32_t*)out, bs, bi);
+    POLAR_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_select_final, dim3(1), dim3(kBlock), 0, st, b, grid, (const double*)bs, (const int64_t*)bi,
+                       sel_s, out);
     POLAR_LAUNCH_CHECK();
   }
   if (host_out) {

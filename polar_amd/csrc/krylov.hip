@@ -5,7 +5,7 @@
 //  * Operator stored as diagonal + off-diagonal adjacency (value-free for unit weights).  A row is
 //    processed by a group of L lanes, each lane owning CPL consecutive columns (16-byte loads);
 //    the group loads L column indices in one coalesced load and broadcasts them with ds_bpermute.
-//  * Rows longer than POLAR_CHUNK nnz are split into chunks processed by independent groups;
+//  * Rows longer than the chunk length are split into chunks processed by independent groups;
 //    chunk partials are combined in chunk order by a finishing kernel (deterministic).
 //  * Every reduction (d.v, |r|^2) is a fixed-order two-level sum: per-workgroup partials, then the
 //    last workgroup to finish sums them in block order and runs the scalar recurrences of the
@@ -55,112 +55,108 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t* __restrict__ a, in
 template <int CPL>
 __device__ __forceinline__ void block_colsum(const Geo& g, const DD (&acc)[CPL], int R,
                                              double* __restrict__ out) {
-  __shared__ double red_hi[kBlock * 4];
-  __shared__ double red_lo[kBlock * 4];
+  // lanes L, 2L, 4L, ... apart own the same columns: butterfly across the wave's groups, then
+  // the kWaves wave results in LDS (fixed tree: the result does not depend on timing)
+  __shared__ double red_hi[kWaves * 64 * 4];
+  __shared__ double red_lo[kWaves * 64 * 4];
+  DD v[CPL];
 #pragma unroll
   for (int k = 0; k < CPL; ++k) {
-    red_hi[threadIdx.x * CPL + k] = acc[k].hi;
-    red_lo[threadIdx.x * CPL + k] = acc[k].lo;
+    v[k] = acc[k];
+    for (int o = g.L; o < 64; o <<= 1) {
+      DD w;
+      w.hi = __shfl_xor(v[k].hi, o, 64);
+      w.lo = __shfl_xor(v[k].lo, o, 64);
+      v[k] = dd_merge(v[k], w);
+    }
+  }
+  if (g.gi == 0) {
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      red_hi[(g.wave * 64 + g.li) * CPL + k] = v[k].hi;
+      red_lo[(g.wave * 64 + g.li) * CPL + k] = v[k].lo;
+    }
   }
   __syncthreads();
   for (int c = threadIdx.x; c < R; c += kBlock) {
-    int li = c / CPL, k = c % CPL;
+    const int li = c / CPL, k = c % CPL;
     DD s{0.0, 0.0};
-    for (int w = 0; w < kWaves; ++w)
-      for (int gi = 0; gi < g.G; ++gi) {
-        const int ix = ((w * 64) + gi * g.L + li) * CPL + k;
-        s = dd_merge(s, DD{red_hi[ix], red_lo[ix]});
-      }
+    for (int w = 0; w < kWaves; ++w) {
+      const int ix = (w * 64 + li) * CPL + k;
+      s = dd_merge(s, DD{red_hi[ix], red_lo[ix]});
+    }
     out[c] = s.hi;
     out[R + c] = s.lo;
   }
   __syncthreads();
 }
 
-// Last-workgroup election after every workgroup stored its partial row.  Returns true in the
-// last workgroup (all threads), after an acquire fence.
-__device__ __forceinline__ bool last_block(unsigned* counter) {
-  __shared__ int s_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(counter, 1u);
-    s_last = (prev == gridDim.x - 1) ? 1 : 0;
+// Column total of nparts double-double partial rows (parts[b*2R + c] hi, parts[b*2R + R + c] lo)
+// by one workgroup: strided double-double accumulation, then a fixed-shape tree; every thread gets
+// the rounded total.
+__device__ double col_total(const double* __restrict__ parts, int nparts, int R, int c) {
+  __shared__ double sh_hi[kBlock];
+  __shared__ double sh_lo[kBlock];
+  DD acc{0.0, 0.0};
+  for (int b = threadIdx.x; b < nparts; b += kBlock) {
+    const double* pb = parts + (int64_t)b * 2 * R;
+    acc = dd_merge(acc, DD{pb[c], pb[R + c]});
   }
+  sh_hi[threadIdx.x] = acc.hi;
+  sh_lo[threadIdx.x] = acc.lo;
   __syncthreads();
-  if (s_last) __threadfence();
-  return s_last != 0;
-}
-
-// Sum the double-double partial rows parts[b*2R + (0|R) + c], b < nparts, into sh_out[c].
-__device__ __forceinline__ void sum_parts(const double* __restrict__ parts, int nparts, int R,
-                                          double* __restrict__ sh_out /* shared [R] */) {
-  __shared__ double seg_hi[kBlock];
-  __shared__ double seg_lo[kBlock];
-  for (int c0 = 0; c0 < R; c0 += kBlock) {
-    const int t = threadIdx.x;
-    const int cw = min(R - c0, kBlock);
-    const int ns = max(1, kBlock / cw);
-    const int col = t % cw, seg = t / cw;
-    if (seg < ns) {
-      DD s{0.0, 0.0};
-      for (int b = seg; b < nparts; b += ns) {
-        const double* pb = parts + (int64_t)b * 2 * R;
-        s = dd_merge(s, DD{pb[c0 + col], pb[R + c0 + col]});
-      }
-      seg_hi[seg * cw + col] = s.hi;
-      seg_lo[seg * cw + col] = s.lo;
-    }
-    __syncthreads();
-    if (t < cw) {
-      DD tot{0.0, 0.0};
-      for (int sgi = 0; sgi < ns; ++sgi) tot = dd_merge(tot, DD{seg_hi[sgi * cw + t], seg_lo[sgi * cw + t]});
-      sh_out[c0 + t] = tot.hi + tot.lo;
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      const DD m = dd_merge(DD{sh_hi[threadIdx.x], sh_lo[threadIdx.x]}, DD{sh_hi[threadIdx.x + o], sh_lo[threadIdx.x + o]});
+      sh_hi[threadIdx.x] = m.hi;
+      sh_lo[threadIdx.x] = m.lo;
     }
     __syncthreads();
   }
+  const double tot = sh_hi[0] + sh_lo[0];
+  __syncthreads();
+  return tot;
 }
 
 // ------------------------------------------------------------------------------------------------
-// Gather-accumulate acc[:] += sum_k w_k X[idx_k, c0:c0+CPL] over k in [beg, end).
+// Gather-accumulate acc[:] += sum_k w_k X[idx_k, c0:c0+CPL] over k in [beg, end), in k order.
+// The group's L lanes load 8 consecutive column indices each (8L per index round); batches of 8
+// gathers are issued before their ordered adds, so 8 row loads per lane are in flight.
 template <typename T, int CPL, bool WEIGHTED>
 __device__ __forceinline__ void gather_acc(const int32_t* __restrict__ idx, const T* __restrict__ wt,
                                            const T* __restrict__ X, int64_t ldx, int beg, int end,
                                            const Geo& g, bool colok, int c0, T (&acc)[CPL]) {
   const int gbase = g.gi * g.L;
-  for (int k0 = beg; k0 < end; k0 += g.L) {
-    const int cnt = min(g.L, end - k0);
-    const int myj = (g.li < cnt) ? idx[k0 + g.li] : 0;
-    T myw = T(1);
-    if (WEIGHTED) myw = (g.li < cnt) ? wt[k0 + g.li] : T(0);
-    int t = 0;
-    for (; t + 4 <= cnt; t += 4) {
-      int j[4];
-      T w[4];
+  for (int k0 = beg; k0 < end; k0 += 8 * g.L) {
+    const int cnt = min(8 * g.L, end - k0);
+    int myj[8];
+    T myw[8];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        j[u] = __shfl(myj, gbase + t + u, 64);
-        if (WEIGHTED) w[u] = __shfl(myw, gbase + t + u, 64);
-      }
-      if (colok) {
-        T x[4][CPL];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) vload<T, CPL>(X + (int64_t)j[u] * ldx + c0, x[u]);
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int e = 0; e < CPL; ++e) acc[e] += (WEIGHTED ? w[u] * x[u][e] : x[u][e]);
-      }
+    for (int u = 0; u < 8; ++u) {
+      const int t = g.li * 8 + u;
+      myj[u] = (t < cnt) ? idx[k0 + t] : 0;
+      if (WEIGHTED) myw[u] = (t < cnt) ? wt[k0 + t] : T(0);
     }
-    for (; t < cnt; ++t) {
-      int jj = __shfl(myj, gbase + t, 64);
-      T ww = T(1);
-      if (WEIGHTED) ww = __shfl(myw, gbase + t, 64);
-      if (colok) {
-        T x[CPL];
-        vload<T, CPL>(X + (int64_t)jj * ldx + c0, x);
+    for (int b = 0; 8 * b < cnt; ++b) {
+      int j[8];
+      T w[8];
 #pragma unroll
-        for (int e = 0; e < CPL; ++e) acc[e] += (WEIGHTED ? ww * x[e] : x[e]);
+      for (int u = 0; u < 8; ++u) {
+        j[u] = __shfl(myj[u], gbase + b, 64);
+        if (WEIGHTED) w[u] = __shfl(myw[u], gbase + b, 64);
+      }
+      if (colok) {
+        const int nb = min(8, cnt - 8 * b);
+        T x[8][CPL];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (u < nb) vload<T, CPL>(X + (int64_t)j[u] * ldx + c0, x[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (u < nb) {
+#pragma unroll
+            for (int e = 0; e < CPL; ++e) acc[e] += (WEIGHTED ? w[u] * x[u][e] : x[u][e]);
+          }
       }
     }
   }
@@ -197,48 +193,44 @@ struct CGState {
   double *z, *zp, *zn, *aj, *bj, *h, *ip, *pn, *te, *tg, *shift;
   int *live, *m;
   int* flag;           // [2] any-active flags, indexed by iteration parity
-  unsigned* counter;   // [4] last-block counters
   unsigned long long* stats;  // [2] sum over iterations of active columns / live (pole, column) pairs
-  double* parts;       // [2*kMaxGrid][2][R] double-double block partials
-  int R, p;
+  double* parts;       // [2*kMaxGrid][2][R] double-double workgroup partials
+  int* colmap;         // [R] slot -> caller column (active-column compaction)
+  int R, p;            // R: current (compacted) width
+  int ld;              // row stride of vectors, P and per-pole scalars (the caller's width)
+  int ident;           // colmap is the identity (no compaction yet)
   double tol;
   int freeze, track, trace;
 };
 
-// Phase 1 (after d.v): a = act ? rr/dv : 0.
-__device__ void phase_alpha(const CGState& s, int nparts) {
-  __shared__ double dv[512];
-  sum_parts(s.parts, nparts, s.R, dv);
-  for (int c = threadIdx.x; c < s.R; c += kBlock) s.a[c] = s.act[c] ? s.rr[c] / dv[c] : 0.0;
+// Finisher after d.v (one workgroup per column): a = act ? rr/dv : 0.
+__global__ __launch_bounds__(kBlock) void k_alpha(CGState s, int nparts, int cur) {
+  if (s.flag[cur] == 0) return;
+  const int c = blockIdx.x;
+  const double dv = col_total(s.parts, nparts, s.R, c);
+  if (threadIdx.x == 0) s.a[c] = s.act[c] ? s.rr[c] / dv : 0.0;
 }
 
-// Phase 2 (after |rn|^2): all scalar recurrences of polar_gpu.py:106-125.
-__device__ void phase_scalars(const CGState& s, int nparts, int cur) {
-  __shared__ double rrn[512];
-  __shared__ double bec[512];
-  __shared__ double ccs[512];
-  __shared__ int any_s;
-  sum_parts(s.parts, nparts, s.R, rrn);
-  const int R = s.R;
-  __shared__ int nlive_s, nact_s;
-  if (threadIdx.x == 0) {
-    any_s = 0;
-    nlive_s = 0;
-    nact_s = 0;
-  }
-  for (int c = threadIdx.x; c < R; c += kBlock) {
-    const bool act = s.act[c] != 0;
-    bec[c] = act ? rrn[c] / s.rr[c] : 0.0;
-    ccs[c] = act ? s.a[c] * s.bp[c] / s.ap[c] : 0.0;
-  }
+// Finisher after |rn|^2 (one workgroup per column c, threads over poles): every scalar recurrence
+// of polar_gpu.py:106-125 for column c, freezing, convergence and the any-active flag.
+__global__ __launch_bounds__(kBlock) void k_scalars(CGState s, int nparts, int cur) {
+  if (s.flag[cur] == 0) return;
+  const int c = blockIdx.x;
+  const double rrn = col_total(s.parts, nparts, s.R, c);
+  __shared__ int nlive_s;
+  if (threadIdx.x == 0) nlive_s = 0;
+  const bool act = s.act[c] != 0;
+  const double a = s.a[c];
+  const double be = act ? rrn / s.rr[c] : 0.0;
+  const double cc = act ? a * s.bp[c] / s.ap[c] : 0.0;
+  const double nr = sqrt(rrn);
+  const double stop = s.stop[c];
   __syncthreads();
   int nlive = 0;
-  for (int idx = threadIdx.x; idx < s.p * R; idx += kBlock) {
-    const int c = idx % R;
-    const bool act = s.act[c] != 0;
-    const double a = s.a[c], be = bec[c], cc = ccs[c];
+  for (int j = threadIdx.x; j < s.p; j += kBlock) {
+    const int idx = j * s.ld + c;
     const double z = s.z[idx], zp = s.zp[idx];
-    const double zn = z / (1.0 + a * s.shift[idx / R] + cc * (1.0 - z / zp));
+    const double zn = z / (1.0 + a * s.shift[j] + cc * (1.0 - z / zp));
     const double fac = zn / z;
     const bool m = (s.live[idx] != 0) && act;
     const double aj = m ? a * fac : 0.0;
@@ -253,7 +245,7 @@ __device__ void phase_scalars(const CGState& s, int nparts, int cur) {
       s.h[idx] += 2.0 * aj * ip + aj * aj * pn;
       if (m) {
         s.ip[idx] = bj * (ip + aj * pn);
-        s.pn[idx] = zn * zn * rrn[c] + bj * bj * pn;
+        s.pn[idx] = zn * zn * rrn + bj * bj * pn;
       }
     }
     if (s.trace) {
@@ -265,34 +257,61 @@ __device__ void phase_scalars(const CGState& s, int nparts, int cur) {
       s.zp[idx] = z;
       s.z[idx] = zn;
     }
-    const double nr = sqrt(rrn[c]);
-    if (s.freeze && m && fabs(zn) * nr <= s.stop[c]) s.live[idx] = 0;
+    if (s.freeze && m && fabs(zn) * nr <= stop) s.live[idx] = 0;
   }
   if (nlive) atomicAdd(&nlive_s, nlive);
   __syncthreads();
-  int any = 0, nact = 0;
-  for (int c = threadIdx.x; c < R; c += kBlock) {
-    const bool act = s.act[c] != 0;
-    nact += act ? 1 : 0;
-    s.dmask[c] = act ? 1 : 0;
-    s.be[c] = bec[c];
-    if (act) {
-      const double nr = sqrt(rrn[c]);
-      s.iters[c] += 1;
-      const bool done = nr <= s.stop[c];
-      s.rr[c] = rrn[c];
-      s.ap[c] = s.a[c];
-      s.bp[c] = bec[c];
-      if (done) s.act[c] = 0; else any = 1;
-    }
-  }
-  if (any) atomicOr(&any_s, 1);
-  if (nact) atomicAdd(&nact_s, nact);
-  __syncthreads();
   if (threadIdx.x == 0) {
-    s.flag[cur ^ 1] = any_s;
-    s.stats[0] += (unsigned long long)nact_s;
-    s.stats[1] += (unsigned long long)nlive_s;
+    s.dmask[c] = act ? 1 : 0;
+    s.be[c] = be;
+    bool still = false;
+    if (act) {
+      s.iters[c] += 1;
+      const bool done = nr <= stop;
+      s.rr[c] = rrn;
+      s.ap[c] = a;
+      s.bp[c] = be;
+      if (done) s.act[c] = 0; else still = true;
+      atomicAdd(&s.stats[0], 1ull);
+    }
+    if (nlive_s) atomicAdd(&s.stats[1], (unsigned long long)nlive_s);
+    if (still) atomicOr(&s.flag[cur ^ 1], 1);
+  }
+}
+
+// Finisher of the init pass: rr, stop, act and every per-pole scalar of column c.
+__global__ __launch_bounds__(kBlock) void k_init_fin(CGState s, int nparts) {
+  const int c = blockIdx.x;
+  const double rr = col_total(s.parts, nparts, s.R, c);
+  for (int j = threadIdx.x; j < s.p; j += kBlock) {
+    const int idx = j * s.ld + c;
+    s.z[idx] = 1.0;
+    s.zp[idx] = 1.0;
+    s.zn[idx] = 1.0;
+    s.aj[idx] = 0.0;
+    s.bj[idx] = 0.0;
+    s.live[idx] = rr > 0.0 ? 1 : 0;
+    s.m[idx] = 0;
+    s.h[idx] = 0.0;
+    s.ip[idx] = 0.0;
+    s.pn[idx] = rr;
+    s.te[idx] = 0.0;
+    s.tg[idx] = rr;
+  }
+  if (threadIdx.x == 0) {
+    s.rr[c] = rr;
+    s.stop[c] = s.tol * sqrt(rr);
+    s.act[c] = rr > 0.0 ? 1 : 0;
+    s.dmask[c] = 0;
+    s.a[c] = 0.0;
+    s.be[c] = 0.0;
+    s.ap[c] = 1.0;
+    s.bp[c] = 0.0;
+    s.iters[c] = 0;
+    if (rr > 0.0) {
+      atomicOr(&s.flag[0], 1);
+      atomicOr(&s.flag[1], 1);
+    }
   }
 }
 
@@ -301,8 +320,7 @@ __device__ void phase_scalars(const CGState& s, int nparts, int cur) {
 template <typename T, int CPL, bool WEIGHTED, bool DOT>
 __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, const T* __restrict__ X,
                                                   int64_t ldx, T* __restrict__ Y, int64_t ldy, int R,
-                                                  int L, T* __restrict__ lpart, CGState s, int cur,
-                                                  int finish) {
+                                                  int L, T* __restrict__ lpart, CGState s, int cur) {
   if (DOT && s.flag[cur] == 0) return;
   Geo g(L);
   const int c0 = g.li * CPL;
@@ -312,28 +330,32 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
   const int gpb = kWaves * g.G;
   int64_t u = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi;
   const int64_t stride = (int64_t)gridDim.x * gpb;
+  const int64_t nx = op.n_extra;
+  const int CH = op.chunk;
   DD dacc[CPL];
 #pragma unroll
   for (int e = 0; e < CPL; ++e) dacc[e] = DD{0.0, 0.0};
-  int pc = (op.n_patch > 0 && u < n) ? lower_bound_i32(op.patch_rows, op.n_patch, (int)u) : 0;
+  int pc = -1;
+  // units [0, n_extra): chunks >= 1 of long rows (scheduled first: their latency chains overlap
+  // the rest); units [n_extra, n_extra + n): rows (the first chunk of a long row)
   for (; u < nunits; u += stride) {
     int row, beg, end, lidx = -1, chunk = 0;
-    if (u < n) {
-      row = (int)u;
+    if (u >= nx) {
+      row = (int)(u - nx);
       beg = op.indptr[row];
       end = op.indptr[row + 1];
-      if (end - beg > POLAR_CHUNK) {
+      if (end - beg > CH) {
         lidx = lower_bound_i32(op.long_rows, op.n_long, row);
-        end = beg + POLAR_CHUNK;
+        end = beg + CH;
       }
     } else {
-      const int2 ex = reinterpret_cast<const int2*>(op.extra)[u - n];
+      const int2 ex = reinterpret_cast<const int2*>(op.extra)[u];
       lidx = ex.x;
       chunk = ex.y;
       row = op.long_rows[lidx];
       const int rb = op.indptr[row], re = op.indptr[row + 1];
-      beg = rb + chunk * POLAR_CHUNK;
-      end = min(re, beg + POLAR_CHUNK);
+      beg = rb + chunk * CH;
+      end = min(re, beg + CH);
     }
     T acc[CPL];
 #pragma unroll
@@ -342,6 +364,7 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
     if (lidx < 0) {
       int pidx = -1;
       if (op.n_patch > 0) {
+        if (pc < 0) pc = lower_bound_i32(op.patch_rows, op.n_patch, row);
         while (pc < op.n_patch && op.patch_rows[pc] < row) ++pc;
         if (pc < op.n_patch && op.patch_rows[pc] == row) pidx = pc;
       }
@@ -360,11 +383,6 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
   }
   if (DOT) {
     block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)blockIdx.x * 2 * R);
-    if (finish && last_block(&s.counter[0])) {
-      phase_alpha(s, gridDim.x);
-      __syncthreads();
-      if (threadIdx.x == 0) s.counter[0] = 0;
-    }
   }
 }
 
@@ -386,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
        l += (int64_t)gridDim.x * gpb) {
     const int row = op.long_rows[l];
     const int deg = op.indptr[row + 1] - op.indptr[row];
-    const int nch = (deg + POLAR_CHUNK - 1) / POLAR_CHUNK;
+    const int nch = (deg + op.chunk - 1) / op.chunk;
     int pidx = -1;
     if (op.n_patch > 0) {
       int pc = lower_bound_i32(op.patch_rows, op.n_patch, row);
@@ -414,23 +432,17 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
   }
   if (DOT) {
     block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)(parts_off + blockIdx.x) * 2 * R);
-    if (last_block(&s.counter[1])) {
-      phase_alpha(s, parts_off + gridDim.x);
-      __syncthreads();
-      if (threadIdx.x == 0) s.counter[1] = 0;
-    }
   }
 }
 
-// Kernel B: rn = r - a v (active columns), |rn|^2 partials; last block runs phase_scalars.
+// Kernel B: rn = r - a v (active columns) and |rn|^2 partials (finished by k_scalars).
 template <typename T, int CPL>
 __global__ __launch_bounds__(kBlock) void k_resid(int64_t n, int R, int L, T* __restrict__ r,
                                                   const T* __restrict__ v, CGState s, int cur) {
-  if (s.flag[cur] == 0) {
-    // converged: keep both parities at 0 so every later launch is a no-op
-    if (blockIdx.x == 0 && threadIdx.x == 0) s.flag[cur ^ 1] = 0;
-    return;
-  }
+  // the finisher of this iteration raises flag[cur^1] again if a column is still active; when
+  // converged, both parities stay 0 so every later launch is a no-op
+  if (blockIdx.x == 0 && threadIdx.x == 0) s.flag[cur ^ 1] = 0;
+  if (s.flag[cur] == 0) return;
   Geo g(L);
   const int c0 = g.li * CPL;
   const bool colok = c0 < R;
@@ -450,29 +462,24 @@ __global__ __launch_bounds__(kBlock) void k_resid(int64_t n, int R, int L, T* __
     const int gpb = kWaves * g.G;
     for (int64_t i = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi; i < n; i += (int64_t)gridDim.x * gpb) {
       T rv[CPL], vv[CPL];
-      vload<T, CPL>(r + i * R + c0, rv);
+      vload<T, CPL>(r + i * s.ld + c0, rv);
       if (any) {
-        vload<T, CPL>(v + i * R + c0, vv);
+        vload<T, CPL>(v + i * s.ld + c0, vv);
 #pragma unroll
         for (int e = 0; e < CPL; ++e)
           if (act[e]) rv[e] = rv[e] - (a[e] * vv[e]);
-        vstore<T, CPL>(r + i * R + c0, rv);
+        vstore<T, CPL>(r + i * s.ld + c0, rv);
       }
 #pragma unroll
       for (int e = 0; e < CPL; ++e) dd_add(dacc[e], (double)rv[e] * (double)rv[e]);
     }
   }
   block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)blockIdx.x * 2 * R);
-  if (last_block(&s.counter[2])) {
-    phase_scalars(s, gridDim.x, cur);
-    __syncthreads();
-    if (threadIdx.x == 0) s.counter[2] = 0;
-  }
 }
 
 // Kernel C: d = rn + be d (columns active at the start of the iteration) on all rows, and the
 // shifted updates X_j += aj P_j; P_j = zn rn[active] + bj P_j (live poles) on the active rows.
-template <typename T, int CPL>
+template <typename T, int CPL, bool STAGED>
 __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p, int R, int L,
                                                    const T* __restrict__ r, T* __restrict__ d,
                                                    const int32_t* __restrict__ active,
@@ -481,9 +488,9 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
   if (s.flag[cur] == 0) return;
   Geo g(L);
   const int c0 = g.li * CPL;
-  if (c0 >= R) return;
   const int gpb = kWaves * g.G;
   if ((int)blockIdx.x < nb_d) {
+    if (c0 >= R) return;     // no barrier in this part
     bool msk[CPL];
     T be[CPL];
     bool any = false;
@@ -496,42 +503,82 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
     if (!any) return;
     for (int64_t i = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi; i < n; i += (int64_t)nb_d * gpb) {
       T rv[CPL], dv[CPL];
-      vload<T, CPL>(r + i * R + c0, rv);
-      vload<T, CPL>(d + i * R + c0, dv);
+      vload<T, CPL>(r + i * s.ld + c0, rv);
+      vload<T, CPL>(d + i * s.ld + c0, dv);
 #pragma unroll
       for (int e = 0; e < CPL; ++e)
         if (msk[e]) dv[e] = rv[e] + (be[e] * dv[e]);
-      vstore<T, CPL>(d + i * R + c0, dv);
+      vstore<T, CPL>(d + i * s.ld + c0, dv);
     }
     return;
   }
   const int nb_s = gridDim.x - nb_d;
+  // Stage this block's view of the per-pole scalars in LDS and list the poles that are live in
+  // some column (frozen poles are skipped entirely); falls back to global loads when p*R is large.
+  constexpr int kStage = STAGED ? 1024 : 1;
+  __shared__ double s_aj[kStage], s_bj[kStage], s_zn[kStage];
+  __shared__ int s_m[kStage];
+  __shared__ int s_jl[kStage];
+  __shared__ int s_nj;
+  const bool staged = STAGED;
+  if (STAGED) {
+    if (threadIdx.x == 0) s_nj = 0;
+    __syncthreads();
+    for (int q = threadIdx.x; q < p * R; q += kBlock) {
+      const int j = q / R, c = q - j * R;
+      const int src = j * s.ld + c;
+      s_aj[q] = s.aj[src];
+      s_bj[q] = s.bj[src];
+      s_zn[q] = s.zn[src];
+      s_m[q] = s.m[src];
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < p; j += kBlock) {
+      bool any = false;
+      for (int c = 0; c < R; ++c) any |= s_m[j * R + c] != 0;
+      if (any) s_jl[atomicAdd(&s_nj, 1)] = j;   // pole order is irrelevant: poles are independent
+    }
+    __syncthreads();
+  }
+  const int nj = staged ? s_nj : p;
   for (int64_t i = (int64_t)(blockIdx.x - nb_d) * gpb + g.wave * g.G + g.gi; i < nA;
        i += (int64_t)nb_s * gpb) {
+    if (c0 >= R || nj == 0) break;
     T rn[CPL];
-    vload<T, CPL>(r + (int64_t)active[i] * R + c0, rn);
-    for (int j = 0; j < p; ++j) {
-      const int base = j * R + c0;
+    vload<T, CPL>(r + (int64_t)active[i] * s.ld + c0, rn);
+    for (int jj = 0; jj < nj; ++jj) {
+      const int j = staged ? s_jl[jj] : jj;
       bool mj[CPL];
       bool anym = false;
-#pragma unroll
-      for (int e = 0; e < CPL; ++e) {
-        mj[e] = s.m[base + e] != 0;
-        anym |= mj[e];
-      }
-      if (!anym) continue;
       T aj[CPL], bj[CPL], zn[CPL];
 #pragma unroll
       for (int e = 0; e < CPL; ++e) {
-        aj[e] = (T)s.aj[base + e];
-        bj[e] = (T)s.bj[base + e];
-        zn[e] = (T)s.zn[base + e];
+        if (staged) {
+          const int q = j * R + c0 + e;
+          mj[e] = s_m[q] != 0;
+          aj[e] = (T)s_aj[q];
+          bj[e] = (T)s_bj[q];
+          zn[e] = (T)s_zn[q];
+        } else {
+          const int q = j * s.ld + c0 + e;
+          mj[e] = s.m[q] != 0;
+          aj[e] = (T)s.aj[q];
+          bj[e] = (T)s.bj[q];
+          zn[e] = (T)s.zn[q];
+        }
+        anym |= mj[e];
       }
-      T* pp = P + ((int64_t)j * nA + i) * R + c0;
-      T* xp = Xo + (int64_t)j * sj + i * si + c0;
+      if (!anym) continue;
+      T* pp = P + ((int64_t)j * nA + i) * s.ld + c0;
+      T* xrow = Xo + (int64_t)j * sj + i * si;
       T pv[CPL], xv[CPL];
       vload<T, CPL>(pp, pv);
-      vload<T, CPL>(xp, xv);
+      if (s.ident) {
+        vload<T, CPL>(xrow + c0, xv);
+      } else {
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) xv[e] = mj[e] ? xrow[s.colmap[c0 + e]] : T(0);
+      }
 #pragma unroll
       for (int e = 0; e < CPL; ++e) {
         if (mj[e]) {
@@ -539,7 +586,13 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
           pv[e] = (zn[e] * rn[e]) + (bj[e] * pv[e]);
         }
       }
-      vstore<T, CPL>(xp, xv);
+      if (s.ident) {
+        vstore<T, CPL>(xrow + c0, xv);
+      } else {
+#pragma unroll
+        for (int e = 0; e < CPL; ++e)
+          if (mj[e]) xrow[s.colmap[c0 + e]] = xv[e];
+      }
       vstore<T, CPL>(pp, pv);
     }
   }
@@ -561,55 +614,13 @@ __global__ __launch_bounds__(kBlock) void k_init(int64_t n, int R, int L, const 
     for (int64_t i = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi; i < n; i += (int64_t)gridDim.x * gpb) {
       T wv[CPL];
       vload<T, CPL>(W + i * ldw + c0, wv);
-      vstore<T, CPL>(r + i * R + c0, wv);
-      vstore<T, CPL>(d + i * R + c0, wv);
+      vstore<T, CPL>(r + i * s.ld + c0, wv);
+      vstore<T, CPL>(d + i * s.ld + c0, wv);
 #pragma unroll
       for (int e = 0; e < CPL; ++e) dd_add(dacc[e], (double)wv[e] * (double)wv[e]);
     }
   }
   block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)blockIdx.x * 2 * R);
-  if (last_block(&s.counter[3])) {
-    __shared__ double rr[512];
-    __shared__ int any_s;
-    sum_parts(s.parts, gridDim.x, R, rr);
-    if (threadIdx.x == 0) any_s = 0;
-    __syncthreads();
-    int any = 0;
-    for (int c = threadIdx.x; c < R; c += kBlock) {
-      s.rr[c] = rr[c];
-      s.stop[c] = s.tol * sqrt(rr[c]);
-      s.act[c] = rr[c] > 0.0 ? 1 : 0;
-      s.dmask[c] = 0;
-      s.a[c] = 0.0;
-      s.be[c] = 0.0;
-      s.ap[c] = 1.0;
-      s.bp[c] = 0.0;
-      s.iters[c] = 0;
-      any |= (rr[c] > 0.0);
-    }
-    for (int idx = threadIdx.x; idx < s.p * R; idx += kBlock) {
-      const int c = idx % R;
-      s.z[idx] = 1.0;
-      s.zp[idx] = 1.0;
-      s.zn[idx] = 1.0;
-      s.aj[idx] = 0.0;
-      s.bj[idx] = 0.0;
-      s.live[idx] = rr[c] > 0.0 ? 1 : 0;
-      s.m[idx] = 0;
-      s.h[idx] = 0.0;
-      s.ip[idx] = 0.0;
-      s.pn[idx] = rr[c];
-      s.te[idx] = 0.0;
-      s.tg[idx] = rr[c];
-    }
-    if (any) atomicOr(&any_s, 1);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      s.flag[0] = any_s;
-      s.flag[1] = any_s;
-      s.counter[3] = 0;
-    }
-  }
 }
 
 // P_j = W[active, :R] for every pole; X_j = 0.
@@ -697,6 +708,7 @@ static int check_op(const polar_op* op) {
   if (op->n_patch > 0 && (!op->patch_rows || !op->patch_ptr || !op->patch_cols))
     return set_error(POLAR_E_ARG, "patch arrays missing");
   if (op->n > INT32_MAX) return set_error(POLAR_E_ARG, "n exceeds int32 row ids");
+  if (op->chunk < 8) return set_error(POLAR_E_ARG, "long-row chunk length must be >= 8");
   return 0;
 }
 
@@ -725,7 +737,6 @@ static void carve_state(Carver& cv, CGState& s, int64_t R, int p) {
   s.tg = cv.take<double>(pr); s.shift = cv.take<double>(p);
   s.live = cv.take<int>(pr); s.m = cv.take<int>(pr);
   s.flag = cv.take<int>(2);
-  s.counter = cv.take<unsigned>(4);
   s.stats = cv.take<unsigned long long>(2);
   s.parts = cv.take<double>((int64_t)4 * kMaxGrid * R);
 }
@@ -733,17 +744,16 @@ static void carve_state(Carver& cv, CGState& s, int64_t R, int p) {
 template <typename T, int CPL>
 static int launch_opmul(const polar_op* op, const Plan& pl, double shift, const T* X, int64_t ldx, T* Y,
                         int64_t ldy, int R, T* lpart, const CGState& s, int cur, bool dot, hipStream_t st) {
-  const int finishA = (op->n_long == 0) ? 1 : 0;
   if (dot) {
     if (op->weighted)
-      hipLaunchKernelGGL((k_opmul<T, CPL, true, true>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X, ldx, Y, ldy, R, pl.L, lpart, s, cur, finishA);
+      hipLaunchKernelGGL((k_opmul<T, CPL, true, true>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X, ldx, Y, ldy, R, pl.L, lpart, s, cur);
     else
-      hipLaunchKernelGGL((k_opmul<T, CPL, false, true>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X, ldx, Y, ldy, R, pl.L, lpart, s, cur, finishA);
+      hipLaunchKernelGGL((k_opmul<T, CPL, false, true>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X, ldx, Y, ldy, R, pl.L, lpart, s, cur);
   } else {
     if (op->weighted)
-      hipLaunchKernelGGL((k_opmul<T, CPL, true, false>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X, ldx, Y, ldy, R, pl.L, lpart, s, cur, 0);
+      hipLaunchKernelGGL((k_opmul<T, CPL, true, false>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X, ldx, Y, ldy, R, pl.L, lpart, s, cur);
     else
-      hipLaunchKernelGGL((k_opmul<T, CPL, false, false>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X, ldx, Y, ldy, R, pl.L, lpart, s, cur, 0);
+      hipLaunchKernelGGL((k_opmul<T, CPL, false, false>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X, ldx, Y, ldy, R, pl.L, lpart, s, cur);
   }
   POLAR_LAUNCH_CHECK();
   if (op->n_long > 0) {
@@ -769,131 +779,284 @@ static int spmm_impl(const polar_op* op, double shift, const T* X, int64_t ldx, 
 }
 
 // ---------------------------------------------------------------- msolve
+// In-place column compaction of `rows` rows of stride ld: A[row][k] = A[row][perm[k]], k < Rn.
+// One wave per row; perm is increasing with perm[k] >= k, every lane's load of block b completes
+// before its store of block b, and block b only writes columns below 64(b+1) while later loads
+// read columns >= 64(b+1): the in-place move is safe.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_compact_rows(int64_t rows, int64_t ld, int Rn,
+                                                         const int* __restrict__ perm, T* A) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; row < rows; row += nw) {
+    T* a = A + row * ld;
+    T v[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (lane + 64 * b < Rn) v[b] = a[perm[lane + 64 * b]];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (lane + 64 * b < Rn) a[lane + 64 * b] = v[b];
+  }
+}
+
+// dst[i][k] = src[i][perm[k]] (out of place), i < n, k < Rn.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_compact_vec(int64_t n, int64_t ld, int Rn, const int* __restrict__ perm,
+                                                        const T* __restrict__ src, T* __restrict__ dst) {
+  const int64_t total = n * Rn;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx / Rn;
+    const int k = (int)(idx - i * Rn);
+    dst[i * ld + k] = src[i * ld + perm[k]];
+  }
+}
+
+// Write the per-column outputs of the current slots back to the caller's column order.
+__global__ void k_writeback(CGState s, int Rorig, double* __restrict__ h_out, double* __restrict__ te_out,
+                            int64_t* __restrict__ iters_out) {
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < s.p * s.R; idx += gridDim.x * blockDim.x) {
+    const int c = idx % s.R, j = idx / s.R;
+    const int col = s.colmap[c];
+    if (h_out) h_out[(int64_t)j * Rorig + col] = s.h[j * s.ld + c];
+    if (te_out) te_out[(int64_t)j * Rorig + col] = s.te[j * s.ld + c];
+    if (j == 0 && iters_out) iters_out[col] = s.iters[c];
+  }
+}
+
+__global__ void k_iota(int* __restrict__ a, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = i;
+}
+
 template <typename T, int CPL>
-static int msolve_impl(const polar_op* op, const T* W, int64_t ldw, int64_t R, const double* tau, int p,
+static int run_iter(const polar_op* op, const Plan& pl, double tau0, T* r, T* d, T* v, T* lpart, T* P, T* Xo,
+                    int64_t sj, int64_t si, const int32_t* active, int64_t nA, int p, const CGState& s, int cur,
+                    hipEvent_t* ev, hipStream_t st) {
+  const int64_t n = op->n;
+  const int R = s.R;
+  const int64_t gpb = (int64_t)kWaves * pl.G;
+  const int nb_d = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (n + gpb - 1) / gpb));
+  const int nb_s = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (nA + gpb - 1) / gpb));
+  if (ev) POLAR_HIP(hipEventRecord(ev[0], st));
+  int rc = launch_opmul<T, CPL>(op, pl, tau0, d, s.ld, v, s.ld, R, lpart, s, cur, true, st);
+  if (rc) return rc;
+  const int nparts_a = pl.grid_a + (op->n_long > 0 ? pl.grid_l : 0);
+  hipLaunchKernelGGL(k_alpha, dim3(R), dim3(kBlock), 0, st, s, nparts_a, cur);
+  POLAR_LAUNCH_CHECK();
+  if (ev) POLAR_HIP(hipEventRecord(ev[1], st));
+  hipLaunchKernelGGL((k_resid<T, CPL>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, R, pl.L, r, (const T*)v, s, cur);
+  POLAR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scalars, dim3(R), dim3(kBlock), 0, st, s, pl.grid_rows, cur);
+  POLAR_LAUNCH_CHECK();
+  if (ev) POLAR_HIP(hipEventRecord(ev[2], st));
+  if (p * R <= 1024)
+    hipLaunchKernelGGL((k_update<T, CPL, true>), dim3(nb_d + nb_s), dim3(kBlock), 0, st, n, nA, p, R, pl.L,
+                       (const T*)r, d, active, P, Xo, sj, si, nb_d, s, cur);
+  else
+    hipLaunchKernelGGL((k_update<T, CPL, false>), dim3(nb_d + nb_s), dim3(kBlock), 0, st, n, nA, p, R, pl.L,
+                       (const T*)r, d, active, P, Xo, sj, si, nb_d, s, cur);
+  POLAR_LAUNCH_CHECK();
+  if (ev) POLAR_HIP(hipEventRecord(ev[3], st));
+  return 0;
+}
+
+template <typename T>
+static int run_iter_any(int cpl, const polar_op* op, const Plan& pl, double tau0, T* r, T* d, T* v, T* lpart,
+                        T* P, T* Xo, int64_t sj, int64_t si, const int32_t* active, int64_t nA, int p,
+                        const CGState& s, int cur, hipEvent_t* ev, hipStream_t st) {
+  if (cpl == 2) return run_iter<T, 2>(op, pl, tau0, r, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, ev, st);
+  if constexpr (sizeof(T) == 4) {
+    if (cpl == 4) return run_iter<T, 4>(op, pl, tau0, r, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, ev, st);
+  }
+  return run_iter<T, 1>(op, pl, tau0, r, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, ev, st);
+}
+
+template <typename T>
+static int compact_rows(int64_t rows, int64_t ld, int Rn, const int* perm, T* A, hipStream_t st) {
+  const int grid = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (rows * 64 + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_compact_rows<T>, dim3(grid), dim3(kBlock), 0, st, rows, ld, Rn, perm, A);
+  POLAR_LAUNCH_CHECK();
+  return 0;
+}
+
+// Columns per lane for width Rc at row stride ld (both multiples of cpl; 16-byte loads at most).
+template <typename T>
+static int cpl_for(int64_t Rc, int64_t ld, int cpl_max) {
+  for (int c = cpl_max; c > 1; c /= 2)
+    if (Rc % c == 0 && ld % c == 0 && (Rc + c - 1) / c <= 64) return c;
+  return 1;
+}
+
+template <typename T>
+static int msolve_impl(int cpl0, const polar_op* op, const T* W, int64_t ldw, int64_t R, const double* tau, int p,
                        const int32_t* active, int64_t nA, double tol, int64_t maxit, int flags, T* Xo,
                        int64_t sj, int64_t si, double* h_out, double* te_out, int64_t* iters_out,
                        void* ws, int64_t ws_bytes, hipStream_t st) {
-  Plan pl = make_plan(op, R, CPL);
   const int64_t n = op->n;
   Carver cv(ws);
   CGState s{};
   carve_state<T>(cv, s, R, p);
+  s.colmap = cv.take<int>(R);
+  int* perm = cv.take<int>(R);
   T* r = cv.take<T>(n * R);
   T* d = cv.take<T>(n * R);
   T* v = cv.take<T>(n * R);
+  T* t2 = cv.take<T>(n * R);
   T* P = cv.take<T>((int64_t)p * nA * R);
   T* lpart = cv.take<T>((int64_t)op->n_partial * R);
   if (cv.off > ws_bytes) return set_error(POLAR_E_ARG, "msolve workspace too small (%lld < %lld)", (long long)ws_bytes, (long long)cv.off);
   s.R = (int)R;
+  s.ld = (int)R;
+  s.ident = 1;
   s.p = p;
   s.tol = tol;
   s.freeze = (flags & POLAR_NOFREEZE) ? 0 : 1;
   s.track = (flags & POLAR_TRACK) ? 1 : 0;
   s.trace = (flags & POLAR_TRACE) ? 1 : 0;
-  const int L = pl.L;
-  const int64_t gpb = (int64_t)kWaves * pl.G;
+  int cpl = cpl0;
+  Plan pl = make_plan(op, R, cpl);
+  const int64_t gpb0 = (int64_t)kWaves * pl.G;
 
-  POLAR_HIP(hipMemsetAsync(s.counter, 0, 4 * sizeof(unsigned), st));
+  POLAR_HIP(hipMemsetAsync(s.flag, 0, 2 * sizeof(int), st));
   POLAR_HIP(hipMemsetAsync(s.stats, 0, 2 * sizeof(unsigned long long), st));
-  const bool prof = g_prof_on != 0;
-  std::vector<hipEvent_t> ev;
-  std::vector<float> kms;  // per iteration: opmul, resid, update milliseconds
-  if (prof) ev.reserve(4 * 8);
+  hipLaunchKernelGGL(k_iota, dim3(1), dim3(256), 0, st, s.colmap, (int)R);
+  POLAR_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_shift, dim3((p + 255) / 256), dim3(256), 0, st, tau, p, s.shift);
   POLAR_LAUNCH_CHECK();
-  hipLaunchKernelGGL((k_init<T, CPL>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, L, W, ldw, r, d, s);
+  if (cpl == 2) {
+    hipLaunchKernelGGL((k_init<T, 2>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, pl.L, W, ldw, r, d, s);
+  } else if (cpl == 4) {
+    if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_init<T, 4>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, pl.L, W, ldw, r, d, s);
+  } else {
+    hipLaunchKernelGGL((k_init<T, 1>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, pl.L, W, ldw, r, d, s);
+  }
   POLAR_LAUNCH_CHECK();
-  const int grid_sh = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (nA + gpb - 1) / gpb));
-  hipLaunchKernelGGL((k_init_shift<T, CPL>), dim3(grid_sh), dim3(kBlock), 0, st, nA, p, (int)R, L, W, ldw, active, P, Xo, sj, si);
+  hipLaunchKernelGGL(k_init_fin, dim3(R), dim3(kBlock), 0, st, s, pl.grid_rows);
+  POLAR_LAUNCH_CHECK();
+  const int grid_sh = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (nA + gpb0 - 1) / gpb0));
+  if (cpl == 2) {
+    hipLaunchKernelGGL((k_init_shift<T, 2>), dim3(grid_sh), dim3(kBlock), 0, st, nA, p, (int)R, pl.L, W, ldw, active, P, Xo, sj, si);
+  } else if (cpl == 4) {
+    if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_init_shift<T, 4>), dim3(grid_sh), dim3(kBlock), 0, st, nA, p, (int)R, pl.L, W, ldw, active, P, Xo, sj, si);
+  } else {
+    hipLaunchKernelGGL((k_init_shift<T, 1>), dim3(grid_sh), dim3(kBlock), 0, st, nA, p, (int)R, pl.L, W, ldw, active, P, Xo, sj, si);
+  }
   POLAR_LAUNCH_CHECK();
 
-  const double shift0 = 0.0;  // tau[0] is applied through diag + shift below
   double tau0 = 0.0;
   POLAR_HIP(hipMemcpyAsync(&tau0, tau, sizeof(double), hipMemcpyDeviceToHost, st));
   POLAR_HIP(hipStreamSynchronize(st));
-  (void)shift0;
-  const int nb_d = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (n + gpb - 1) / gpb));
-  const int nb_s = grid_sh;
+
+  const bool prof = g_prof_on != 0;
+  std::vector<hipEvent_t> ev;
+  std::vector<float> kms;        // per iteration: opmul, resid, update milliseconds
+  std::vector<int> widths;       // per iteration: processed width
+  const int esz = (int)sizeof(T);
+  const int align = cpl0;        // compacted widths stay multiples of the initial vector width
+  std::vector<int> acth(R), permh(R);
   bool converged = false;
-  int64_t it = 0;
-  for (; it < maxit; ++it) {
+  for (int64_t it = 0; it < maxit; ++it) {
     const int cur = (int)(it & 1);
-    const int slot = (int)(it % 8) * 4;
+    hipEvent_t* evp = nullptr;
     if (prof) {
+      const int slot = (int)(it % 8) * 4;
       while ((int)ev.size() < slot + 4) {
         hipEvent_t e;
         POLAR_HIP(hipEventCreate(&e));
         ev.push_back(e);
       }
-      POLAR_HIP(hipEventRecord(ev[slot], st));
+      evp = ev.data() + slot;
+      widths.push_back(s.R);
     }
-    int rc = launch_opmul<T, CPL>(op, pl, tau0, d, R, v, R, (int)R, lpart, s, cur, true, st);
+    int rc = run_iter_any<T>(cpl, op, pl, tau0, r, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, evp, st);
     if (rc) return rc;
-    if (prof) POLAR_HIP(hipEventRecord(ev[slot + 1], st));
-    hipLaunchKernelGGL((k_resid<T, CPL>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, L, r, (const T*)v, s, cur);
-    POLAR_LAUNCH_CHECK();
-    if (prof) POLAR_HIP(hipEventRecord(ev[slot + 2], st));
-    hipLaunchKernelGGL((k_update<T, CPL>), dim3(nb_d + nb_s), dim3(kBlock), 0, st, n, nA, p, (int)R, L,
-                       (const T*)r, d, active, P, Xo, sj, si, nb_d, s, cur);
-    POLAR_LAUNCH_CHECK();
-    if (prof) POLAR_HIP(hipEventRecord(ev[slot + 3], st));
-    if (it % 8 == 7) {
-      int flag = 1;
-      POLAR_HIP(hipMemcpyAsync(&flag, s.flag + ((it + 1) & 1), sizeof(int), hipMemcpyDeviceToHost, st));
-      POLAR_HIP(hipStreamSynchronize(st));
-      if (prof) {
-        for (int k = 0; k < 8; ++k) {
-          for (int c = 0; c < 3; ++c) {
-            float ms = 0.f;
-            POLAR_HIP(hipEventElapsedTime(&ms, ev[k * 4 + c], ev[k * 4 + c + 1]));
-            kms.push_back(ms);
-          }
+    if (it % 8 != 7) continue;
+    int flag = 1;
+    POLAR_HIP(hipMemcpyAsync(&flag, s.flag + ((it + 1) & 1), sizeof(int), hipMemcpyDeviceToHost, st));
+    POLAR_HIP(hipMemcpyAsync(acth.data(), s.act, sizeof(int) * s.R, hipMemcpyDeviceToHost, st));
+    POLAR_HIP(hipStreamSynchronize(st));
+    if (prof) {
+      for (int k = 0; k < 8; ++k)
+        for (int c = 0; c < 3; ++c) {
+          float ms = 0.f;
+          POLAR_HIP(hipEventElapsedTime(&ms, ev[k * 4 + c], ev[k * 4 + c + 1]));
+          kms.push_back(ms);
         }
-      }
-      if (!flag) {
-        converged = true;
-        break;
-      }
+    }
+    if (!flag) {
+      converged = true;
+      break;
+    }
+    // Active-column compaction: when at most half of the current slots are still active, move
+    // them (padded to the vector width with finished slots) to the front.  Columns are
+    // independent, so every column's arithmetic is unchanged.
+    int count = 0;
+    for (int c = 0; c < s.R; ++c) count += acth[c] != 0;
+    const int Rn = std::min(s.R, (int)align_up(std::max(count, 1), align));
+    if (Rn <= s.R / 2) {
+      int k = 0;
+      for (int c = 0; c < s.R; ++c)
+        if (acth[c]) permh[k++] = c;
+      for (int c = 0; c < s.R && k < Rn; ++c)
+        if (!acth[c]) permh[k++] = c;
+      POLAR_HIP(hipMemcpyAsync(perm, permh.data(), sizeof(int) * Rn, hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_writeback, dim3((p * s.R + 255) / 256), dim3(256), 0, st, s, (int)R, h_out, te_out, iters_out);
+      POLAR_LAUNCH_CHECK();
+      const int gv = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (n * Rn + kBlock - 1) / kBlock));
+      hipLaunchKernelGGL(k_compact_vec<T>, dim3(gv), dim3(kBlock), 0, st, n, (int64_t)R, Rn, (const int*)perm, (const T*)r, v);
+      POLAR_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_compact_vec<T>, dim3(gv), dim3(kBlock), 0, st, n, (int64_t)R, Rn, (const int*)perm, (const T*)d, t2);
+      POLAR_LAUNCH_CHECK();
+      std::swap(r, v);
+      std::swap(d, t2);
+      if (int e = compact_rows<T>((int64_t)p * nA, R, Rn, perm, P, st)) return e;
+      for (double* a : {s.rr, s.stop, s.a, s.be, s.ap, s.bp})
+        if (int e = compact_rows<double>(1, R, Rn, perm, a, st)) return e;
+      for (int* a : {s.act, s.dmask})
+        if (int e = compact_rows<int>(1, R, Rn, perm, a, st)) return e;
+      if (int e = compact_rows<int64_t>(1, R, Rn, perm, s.iters, st)) return e;
+      for (double* a : {s.z, s.zp, s.zn, s.aj, s.bj, s.h, s.ip, s.pn, s.te, s.tg})
+        if (int e = compact_rows<double>(p, R, Rn, perm, a, st)) return e;
+      for (int* a : {s.live, s.m})
+        if (int e = compact_rows<int>(p, R, Rn, perm, a, st)) return e;
+      if (int e = compact_rows<int>(1, R, Rn, perm, s.colmap, st)) return e;
+      s.R = Rn;
+      s.ident = 0;
+      cpl = cpl_for<T>(Rn, R, cpl0);
+      pl = make_plan(op, Rn, cpl);
     }
   }
   if (prof) {
     // count only the iterations that did work (launches after convergence return at once)
-    std::vector<int64_t> ih(R);
+    std::vector<int64_t> ih(s.R);
     unsigned long long st2[2];
-    POLAR_HIP(hipMemcpyAsync(ih.data(), s.iters, sizeof(int64_t) * R, hipMemcpyDeviceToHost, st));
+    POLAR_HIP(hipMemcpyAsync(ih.data(), s.iters, sizeof(int64_t) * s.R, hipMemcpyDeviceToHost, st));
     POLAR_HIP(hipMemcpyAsync(st2, s.stats, sizeof(st2), hipMemcpyDeviceToHost, st));
     POLAR_HIP(hipStreamSynchronize(st));
     int64_t real = 0;
-    for (int64_t c = 0; c < R; ++c) real = std::max(real, ih[c]);
+    for (int c = 0; c < s.R; ++c) real = std::max(real, ih[c]);
     real = std::min<int64_t>(real, (int64_t)kms.size() / 3);
-    const double esz = sizeof(T);
-    const double nnz = (double)((int64_t)0);
-    (void)nnz;
-    double opbytes = 0.0;
-    {
-      int32_t nnz_h = 0;
-      POLAR_HIP(hipMemcpy(&nnz_h, op->indptr + op->n, sizeof(int32_t), hipMemcpyDeviceToHost));
-      opbytes = 4.0 * (n + 1) + 4.0 * nnz_h + (op->weighted ? esz * nnz_h : 0.0) + esz * n + 2.0 * esz * n * R;
-    }
+    int32_t nnz_h = 0;
+    POLAR_HIP(hipMemcpy(&nnz_h, op->indptr + op->n, sizeof(int32_t), hipMemcpyDeviceToHost));
     for (int64_t k = 0; k < real; ++k) {
       g_prof[0].ms += kms[3 * k];
       g_prof[1].ms += kms[3 * k + 1];
       g_prof[2].ms += kms[3 * k + 2];
+      g_prof[0].bytes += 4.0 * (n + 1) + 4.0 * nnz_h + (op->weighted ? (double)esz * nnz_h : 0.0) +
+                         (double)esz * n + 2.0 * esz * (double)n * widths[k];
     }
     g_prof[0].launches += real;
     g_prof[1].launches += real;
     g_prof[2].launches += real;
-    g_prof[0].bytes += opbytes * real;
     g_prof[1].bytes += 3.0 * esz * n * (double)st2[0];
     g_prof[2].bytes += esz * (3.0 * n + nA) * (double)st2[0] + 4.0 * esz * nA * (double)st2[1];
     g_prof[3].launches += 1;
     g_prof[3].bytes += (double)st2[1];
     for (auto e : ev) (void)hipEventDestroy(e);
   }
-  if (h_out) POLAR_HIP(hipMemcpyAsync(h_out, s.h, sizeof(double) * p * R, hipMemcpyDeviceToDevice, st));
-  if (te_out) POLAR_HIP(hipMemcpyAsync(te_out, s.te, sizeof(double) * p * R, hipMemcpyDeviceToDevice, st));
-  if (iters_out) POLAR_HIP(hipMemcpyAsync(iters_out, s.iters, sizeof(int64_t) * R, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_writeback, dim3((p * s.R + 255) / 256), dim3(256), 0, st, s, (int)R, h_out, te_out, iters_out);
+  POLAR_LAUNCH_CHECK();
   if (!converged) return set_error(POLAR_E_ITERCAP, "multi-shift CG iteration cap");
   return 0;
 }
@@ -903,6 +1066,9 @@ static int64_t msolve_ws(const polar_op* op, int64_t R, int p, int64_t nA) {
   Carver cv(nullptr);
   CGState s{};
   carve_state<T>(cv, s, R, p);
+  cv.take<int>(R);
+  cv.take<int>(R);
+  cv.take<T>(op->n * R);
   cv.take<T>(op->n * R);
   cv.take<T>(op->n * R);
   cv.take<T>(op->n * R);
@@ -974,12 +1140,9 @@ int polar_msolve(const polar_op* op, const void* W, int64_t ldw, int64_t R, cons
   const int cpl = pick_cpl(op->dtype, R, {{W, ldw}, {Xout, x_si}, {(const char*)Xout + 0, x_sj}});
   if (lanes_for(R, cpl) > 64) return set_error(POLAR_E_ARG, "R=%lld too wide for one wave at this alignment", (long long)R);
   if (op->dtype == POLAR_F64) {
-    if (cpl == 2) return msolve_impl<double, 2>(op, (const double*)W, ldw, R, tau, p, active, nA, tol, maxit, flags, (double*)Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
-    return msolve_impl<double, 1>(op, (const double*)W, ldw, R, tau, p, active, nA, tol, maxit, flags, (double*)Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
+    return msolve_impl<double>(cpl, op, (const double*)W, ldw, R, tau, p, active, nA, tol, maxit, flags, (double*)Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
   }
-  if (cpl == 4) return msolve_impl<float, 4>(op, (const float*)W, ldw, R, tau, p, active, nA, tol, maxit, flags, (float*)Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
-  if (cpl == 2) return msolve_impl<float, 2>(op, (const float*)W, ldw, R, tau, p, active, nA, tol, maxit, flags, (float*)Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
-  return msolve_impl<float, 1>(op, (const float*)W, ldw, R, tau, p, active, nA, tol, maxit, flags, (float*)Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
+  return msolve_impl<float>(cpl, op, (const float*)W, ldw, R, tau, p, active, nA, tol, maxit, flags, (float*)Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
 }
 
 }  // extern "C"

@@ -185,6 +185,7 @@ class Op:
         s.n_long = len(sc['long_rows'])
         s.n_extra = len(sc['extra'])
         s.n_partial = sc['n_partial']
+        s.chunk = L.CHUNK
         s.long_rows = L.ptr(idx['long_rows']) if s.n_long else None
         s.long_base = L.ptr(idx['long_base']) if s.n_long else None
         s.extra = L.ptr(idx['extra']) if s.n_extra else None
@@ -419,11 +420,31 @@ def _sync():
     torch.cuda.synchronize()
 
 
+class _Phases:
+    """Optional per-phase wall-clock breakdown of polar() (device-synchronised marks)."""
+
+    def __init__(self, on):
+        self.on = on
+        self.acc = {}
+        self.t = time.perf_counter()
+
+    def mark(self, name):
+        if not self.on:
+            return
+        torch.cuda.synchronize()
+        now = time.perf_counter()
+        self.acc[name] = self.acc.get(name, 0.0) + (now - self.t)
+        self.t = now
+
+
 # ------------------------------------------------------------------------------------------------
 def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=None, naive=False,
           seed=2026, kind='block', trace=None, time_limit=None, poles=None, width=None,
-          freeze=True, precision='fp64'):
+          freeze=True, precision='fp64', profile=False):
     """POLAR greedy, cache backend (polar_gpu.py:180-254), on the GPU.
+
+    profile=True adds `phases`: seconds per phase (bank, select, rerank_solve, rerank_pick,
+    maintenance, update), measured with a device synchronisation at each phase boundary.
 
     Returns dict(sequence, seconds, spmv, rhs, p, rounds, truncated, marginals, trace_estimate,
     solves, precision)."""
@@ -479,6 +500,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     tr_hist = []
     _sync()
     t0 = time.perf_counter()                                                      # :196
+    ph = _Phases(profile)
 
     if width is None:
         width = bank_width(n, p, nA, q, dtype)
@@ -504,6 +526,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
         return X, Y, h, t, tr
 
     X, Y, h, t, tr = bank()
+    ph.mark('bank')
     traces.append(tr if want_trace else None)
     for rnd in range(k):
         if rnd and time_limit is not None and time.perf_counter() - t0 > time_limit:   # :215
@@ -511,6 +534,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
         if naive and rnd:
             X, Y, h, t, tr = bank()
             traces.append(tr if want_trace else None)
+            ph.mark('bank')
         L.check(lib.polar_proxy(objc, L.ptr(h), L.ptr(t), L.ptr(tau_d), L.ptr(om_d), p, N, sigma, L.ptr(dead),
                                 L.ptr(score), stream), 'proxy')
         Br = 1 if naive else min(B, int(live.sum()))
@@ -526,7 +550,9 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             Wb = torch.empty((n, Br), dtype=dtype, device=dev)
             L.check(lib.polar_pair_rhs(dcode, n, Br, L.ptr(a_idx), L.ptr(b_idx), L.ptr(Wb), stream), 'pair_rhs')
             need_track = obj != 'logdet'
+            ph.mark('select')
             U, hh, _ = msolve(op, Wb, tau, active, tol, track=need_track, counter=cnt, freeze=freeze)[:3]
+            ph.mark('rerank_solve')
             sa = torch.from_numpy(es_h[short, 0].astype(np.int32)).to(dev)
             sb = torch.from_numpy(es_h[short, 1].astype(np.int32)).to(dev)
             win = np.zeros(1, dtype=np.int32)
@@ -549,6 +575,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             gc_row.copy_(Gc_dev[e])
         else:
             gc_row.copy_(torch.from_numpy(np.ascontiguousarray(Gc[e], dtype=np.float64)))
+        ph.mark('rerank_pick')
         if not naive and rnd + 1 < k:                                              # :242-249
             L.check(lib.polar_maint_prep(dcode, L.ptr(X), L.ptr(Y), X.stride(0), X.stride(1), p, q, L.ptr(u),
                                          u.stride(0), u.stride(1), aa, bb, L.ptr(gc_row), L.ptr(coef), stream),
@@ -563,7 +590,9 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
                 tr_hist.append(coef[:, 2 * q:2 * q + 2].clone())
         Fv = RHS[:, q:]
         L.check(lib.polar_rows_axpy(dcode, L.ptr(Fv), Fv.stride(0), q, a, b, L.ptr(gc_row), stream), 'rows_axpy')  # :250
+        ph.mark('maintenance')
         op.add(a, b)                                                               # :251
+        ph.mark('update')
     _sync()
     seconds = time.perf_counter() - t0
     if want_trace and not naive:
@@ -572,7 +601,8 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             traces.append(traces[-1] - math.fsum(om * dn_h[:, 1] / dn_h[:, 0]))
     return dict(sequence=seq, seconds=seconds, spmv=cnt.spmv, rhs=cnt.rhs, p=p, rounds=len(seq),
                 truncated=len(seq) < k, marginals=marg, trace_estimate=traces[:max(len(seq), 1)],
-                solves=list(cnt.solves), precision='fp64' if dtype == torch.float64 else 'fp32')
+                solves=list(cnt.solves), precision='fp64' if dtype == torch.float64 else 'fp32',
+                **({'phases': ph.acc} if profile else {}))
 
 
 # ------------------------------------------------------------------------------------------------

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 def test_polar_op_struct_layout():
     from polar_amd._lib import PolarOp
     off = {name: getattr(PolarOp, name).offset for name, _ in PolarOp._fields_}
-    assert off['indptr'] == 16 and off['diag'] == 40 and off['n_long'] == 48
+    assert off['indptr'] == 16 and off['diag'] == 40 and off['n_long'] == 48 and off['chunk'] == 60
     assert off['long_rows'] == 64 and off['n_patch'] == 88 and off['patch_cols'] == 112
     assert ctypes.sizeof(PolarOp) == 120
 
@@ -67,6 +67,7 @@ def test_workspace_queries_through_ctypes():
     s.indptr = ip.ctypes.data_as(L.vp)
     s.diag = ip.ctypes.data_as(L.vp)
     s.n_partial = 3
+    s.chunk = L.CHUNK
     s.n_long = 0
     assert lib.polar_spmm_ws_bytes(s, 16) >= 3 * 16 * 8
     ws = lib.polar_msolve_ws_bytes(s, 32, 24, 500, 0)

@@ -226,3 +226,36 @@ def test_true_marginals_vs_eigenvalues():
     g_o, cert_o = O.true_marginals(O.Graph(n, e, 0.5), cand, seq, 0.5, arith='cr')
     assert _rel(g, g_o) < 1e-13
     assert cert['p'] == cert_o['p'] and abs(cert['residual_rho'] - cert_o['residual_rho']) < 1e-20
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_msolve_column_compaction(precision):
+    """Half the columns are zero (inactive from the start) and the rest mix dense random columns
+    with e_a - e_b pairs (fast convergence): the solver compacts its active columns several times.
+    FP64 must still equal the oracle bit for bit (columns are independent); FP32 within 1e-4."""
+    n, e, G0, G, tau, om = _small_problem(seed=2)
+    R = 32
+    rng = np.random.default_rng(7)
+    W = np.zeros((n, R))
+    for c in range(1, R, 2):
+        if c % 4 == 1:
+            W[:, c] = rng.standard_normal(n)
+        else:
+            a, b = rng.choice(n, 2, replace=False)
+            W[a, c], W[b, c] = 1.0, -1.0
+    active = np.sort(rng.choice(n, 200, replace=False))
+    op0 = O.Op(G0, arith='cr')
+    Xo, ho, ito, teo = O.msolve(op0, W, tau, active, 1e-10, track=True, trace=True)
+    dt = torch.float64 if precision == 'fp64' else torch.float32
+    Xg, hg, itg, teg = pg.msolve(pg.Op(G, precision=precision), torch.from_numpy(W).to('cuda', dt), tau, active,
+                                 1e-10, track=True, trace=True)
+    Xg = Xg.double().cpu().numpy()
+    if precision == 'fp64':
+        assert np.array_equal(ito, itg), (ito, itg)
+        assert np.max(np.abs(Xg - Xo)) <= 1e-13 * np.max(np.abs(Xo))
+        assert np.max(np.abs(hg.cpu().numpy() - ho)) <= 1e-13 * np.max(np.abs(ho))
+        assert np.max(np.abs(teg.cpu().numpy() - teo)) <= 1e-13 * np.max(np.abs(teo))
+    else:
+        assert np.all(itg[0::2] == 0)
+        assert np.max(np.abs(Xg - Xo)) <= 1e-4 * np.max(np.abs(Xo))
+        assert np.max(np.abs(hg.cpu().numpy() - ho) / np.maximum(ho, 1e-300)) <= 1e-4
