@@ -81,10 +81,12 @@ int polar_abi_version(void);
 const char* polar_last_error(void);
 
 /* Live profiling of the CG iteration kernels with HIP events (no counterpart in the reference;
- * used by the benchmark).  polar_prof_set(1) resets and enables it.  Class 0: operator product,
- * 1: residual update, 2: direction + shifted x/p update; per class the summed event time (ms),
- * launches that did work, and algorithmic HBM bytes.  Class 3: solves, and (bytes) the number of
- * live (pole, column) pairs summed over iterations. */
+ * used by the benchmark).  polar_prof_set(1) resets and enables it.  Classes 0..5: k_opmul
+ * (operator product + d.v partials), k_longfin (long-row chunk combine), k_alpha, k_resid
+ * (residual update + |r|^2 partials), k_scalars (per-pole recurrences), k_update (direction and
+ * shifted x/p updates); per class the summed event time (ms), launches that did work, and
+ * algorithmic HBM bytes.  Class 6: solves, and (bytes) the number of live (pole, column) pairs
+ * summed over iterations. */
 int polar_prof_set(int32_t on);
 int polar_prof_get(int32_t cls, double* ms, int64_t* launches, double* bytes);
 

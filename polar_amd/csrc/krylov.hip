@@ -655,14 +655,15 @@ __global__ void k_shift(const double* __restrict__ tau, int p, double* __restric
 // ------------------------------------------------------------------------------------------------
 // Host side
 // Live kernel profiling (HIP events around each launch of the CG iteration), off by default.
-// Class 0: opmul (+ long-row finish), 1: resid, 2: update; class 3 counts solves and, in `bytes`,
-// the sum over iterations of live (pole, column) pairs.
+// Classes 0..5: k_opmul, k_longfin, k_alpha, k_resid, k_scalars, k_update; class 6 counts solves
+// and, in `bytes`, the sum over iterations of live (pole, column) pairs.
 struct ProfClass {
   double ms = 0.0;
   int64_t launches = 0;
   double bytes = 0.0;
 };
-static ProfClass g_prof[4];
+constexpr int kEv = 7;        // events per iteration (6 kernels)
+static ProfClass g_prof[kEv];
 static int g_prof_on = 0;
 
 struct Plan {
@@ -839,17 +840,27 @@ static int run_iter(const polar_op* op, const Plan& pl, double tau0, T* r, T* d,
   const int nb_d = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (n + gpb - 1) / gpb));
   const int nb_s = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (nA + gpb - 1) / gpb));
   if (ev) POLAR_HIP(hipEventRecord(ev[0], st));
-  int rc = launch_opmul<T, CPL>(op, pl, tau0, d, s.ld, v, s.ld, R, lpart, s, cur, true, st);
-  if (rc) return rc;
+  if (op->weighted)
+    hipLaunchKernelGGL((k_opmul<T, CPL, true, true>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, tau0, (const T*)d, (int64_t)s.ld, v, (int64_t)s.ld, R, pl.L, lpart, s, cur);
+  else
+    hipLaunchKernelGGL((k_opmul<T, CPL, false, true>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, tau0, (const T*)d, (int64_t)s.ld, v, (int64_t)s.ld, R, pl.L, lpart, s, cur);
+  POLAR_LAUNCH_CHECK();
+  if (ev) POLAR_HIP(hipEventRecord(ev[1], st));
+  if (op->n_long > 0) {
+    hipLaunchKernelGGL((k_longfin<T, CPL, true>), dim3(pl.grid_l), dim3(kBlock), 0, st, *op, tau0, (const T*)d, (int64_t)s.ld, v, (int64_t)s.ld, R, pl.L, (const T*)lpart, s, cur, pl.grid_a);
+    POLAR_LAUNCH_CHECK();
+  }
+  if (ev) POLAR_HIP(hipEventRecord(ev[2], st));
   const int nparts_a = pl.grid_a + (op->n_long > 0 ? pl.grid_l : 0);
   hipLaunchKernelGGL(k_alpha, dim3(R), dim3(kBlock), 0, st, s, nparts_a, cur);
   POLAR_LAUNCH_CHECK();
-  if (ev) POLAR_HIP(hipEventRecord(ev[1], st));
+  if (ev) POLAR_HIP(hipEventRecord(ev[3], st));
   hipLaunchKernelGGL((k_resid<T, CPL>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, R, pl.L, r, (const T*)v, s, cur);
   POLAR_LAUNCH_CHECK();
+  if (ev) POLAR_HIP(hipEventRecord(ev[4], st));
   hipLaunchKernelGGL(k_scalars, dim3(R), dim3(kBlock), 0, st, s, pl.grid_rows, cur);
   POLAR_LAUNCH_CHECK();
-  if (ev) POLAR_HIP(hipEventRecord(ev[2], st));
+  if (ev) POLAR_HIP(hipEventRecord(ev[5], st));
   if (p * R <= 1024)
     hipLaunchKernelGGL((k_update<T, CPL, true>), dim3(nb_d + nb_s), dim3(kBlock), 0, st, n, nA, p, R, pl.L,
                        (const T*)r, d, active, P, Xo, sj, si, nb_d, s, cur);
@@ -857,7 +868,7 @@ static int run_iter(const polar_op* op, const Plan& pl, double tau0, T* r, T* d,
     hipLaunchKernelGGL((k_update<T, CPL, false>), dim3(nb_d + nb_s), dim3(kBlock), 0, st, n, nA, p, R, pl.L,
                        (const T*)r, d, active, P, Xo, sj, si, nb_d, s, cur);
   POLAR_LAUNCH_CHECK();
-  if (ev) POLAR_HIP(hipEventRecord(ev[3], st));
+  if (ev) POLAR_HIP(hipEventRecord(ev[6], st));
   return 0;
 }
 
@@ -960,8 +971,8 @@ static int msolve_impl(int cpl0, const polar_op* op, const T* W, int64_t ldw, in
     const int cur = (int)(it & 1);
     hipEvent_t* evp = nullptr;
     if (prof) {
-      const int slot = (int)(it % 8) * 4;
-      while ((int)ev.size() < slot + 4) {
+      const int slot = (int)(it % 8) * kEv;
+      while ((int)ev.size() < slot + kEv) {
         hipEvent_t e;
         POLAR_HIP(hipEventCreate(&e));
         ev.push_back(e);
@@ -978,9 +989,9 @@ static int msolve_impl(int cpl0, const polar_op* op, const T* W, int64_t ldw, in
     POLAR_HIP(hipStreamSynchronize(st));
     if (prof) {
       for (int k = 0; k < 8; ++k)
-        for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < kEv - 1; ++c) {
           float ms = 0.f;
-          POLAR_HIP(hipEventElapsedTime(&ms, ev[k * 4 + c], ev[k * 4 + c + 1]));
+          POLAR_HIP(hipEventElapsedTime(&ms, ev[k * kEv + c], ev[k * kEv + c + 1]));
           kms.push_back(ms);
         }
     }
@@ -1036,23 +1047,23 @@ static int msolve_impl(int cpl0, const polar_op* op, const T* W, int64_t ldw, in
     POLAR_HIP(hipStreamSynchronize(st));
     int64_t real = 0;
     for (int c = 0; c < s.R; ++c) real = std::max(real, ih[c]);
-    real = std::min<int64_t>(real, (int64_t)kms.size() / 3);
+    real = std::min<int64_t>(real, (int64_t)kms.size() / (kEv - 1));
     int32_t nnz_h = 0;
     POLAR_HIP(hipMemcpy(&nnz_h, op->indptr + op->n, sizeof(int32_t), hipMemcpyDeviceToHost));
     for (int64_t k = 0; k < real; ++k) {
-      g_prof[0].ms += kms[3 * k];
-      g_prof[1].ms += kms[3 * k + 1];
-      g_prof[2].ms += kms[3 * k + 2];
+      for (int c = 0; c < kEv - 1; ++c) g_prof[c].ms += kms[(kEv - 1) * k + c];
+      const double w = widths[k];
       g_prof[0].bytes += 4.0 * (n + 1) + 4.0 * nnz_h + (op->weighted ? (double)esz * nnz_h : 0.0) +
-                         (double)esz * n + 2.0 * esz * (double)n * widths[k];
+                         (double)esz * n + 2.0 * esz * (double)n * w;
+      g_prof[1].bytes += (double)esz * w * ((double)op->n_partial + 2.0 * op->n_long);
+      g_prof[2].bytes += 16.0 * w * (pl.grid_a + pl.grid_l);
+      g_prof[4].bytes += 16.0 * w * pl.grid_rows + 120.0 * w * p;
     }
-    g_prof[0].launches += real;
-    g_prof[1].launches += real;
-    g_prof[2].launches += real;
-    g_prof[1].bytes += 3.0 * esz * n * (double)st2[0];
-    g_prof[2].bytes += esz * (3.0 * n + nA) * (double)st2[0] + 4.0 * esz * nA * (double)st2[1];
-    g_prof[3].launches += 1;
-    g_prof[3].bytes += (double)st2[1];
+    for (int c = 0; c < kEv - 1; ++c) g_prof[c].launches += real;
+    g_prof[3].bytes += 3.0 * esz * n * (double)st2[0];
+    g_prof[5].bytes += esz * (3.0 * n + nA) * (double)st2[0] + 4.0 * esz * nA * (double)st2[1];
+    g_prof[6].launches += 1;
+    g_prof[6].bytes += (double)st2[1];
     for (auto e : ev) (void)hipEventDestroy(e);
   }
   hipLaunchKernelGGL(k_writeback, dim3((p * s.R + 255) / 256), dim3(256), 0, st, s, (int)R, h_out, te_out, iters_out);
@@ -1092,7 +1103,7 @@ int polar_prof_set(int32_t on) {
 }
 
 int polar_prof_get(int32_t cls, double* ms, int64_t* launches, double* bytes) {
-  if (cls < 0 || cls > 3) return set_error(POLAR_E_ARG, "bad profile class");
+  if (cls < 0 || cls >= kEv) return set_error(POLAR_E_ARG, "bad profile class");
   if (ms) *ms = g_prof[cls].ms;
   if (launches) *launches = g_prof[cls].launches;
   if (bytes) *bytes = g_prof[cls].bytes;
