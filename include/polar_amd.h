@@ -64,12 +64,12 @@ typedef struct polar_op {
   const void* weights;        /* [nnz]  off-diagonal weights (positive), or NULL */
   const void* diag;           /* [n]    weighted degree + sigma */
   int32_t n_long;             /* rows with degree > chunk */
-  int32_t n_extra;            /* chunks of long rows beyond their first */
+  int32_t n_extra;            /* entries of `extra` (= n_partial: every chunk of every long row) */
   int32_t n_partial;          /* total chunks of long rows */
   int32_t chunk;              /* long-row chunk length (>= 8) */
   const int32_t* long_rows;   /* [n_long] ascending row ids */
   const int32_t* long_base;   /* [n_long] first partial slot of each long row */
-  const int32_t* extra;       /* [n_extra][2]: (long index, chunk index >= 1) */
+  const int32_t* extra;       /* [n_extra][2]: (long index, chunk index) */
   int32_t n_patch;            /* rows touched by added edges */
   int32_t pad1;
   const int32_t* patch_rows;  /* [n_patch] ascending */

@@ -336,18 +336,15 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
 #pragma unroll
   for (int e = 0; e < CPL; ++e) dacc[e] = DD{0.0, 0.0};
   int pc = -1;
-  // units [0, n_extra): chunks >= 1 of long rows (scheduled first: their latency chains overlap
-  // the rest); units [n_extra, n_extra + n): rows (the first chunk of a long row)
+  // units [0, n_extra): every chunk of every long row (scheduled first: their latency chains
+  // overlap the rest); units [n_extra, n_extra + n): the remaining rows
   for (; u < nunits; u += stride) {
     int row, beg, end, lidx = -1, chunk = 0;
     if (u >= nx) {
       row = (int)(u - nx);
       beg = op.indptr[row];
       end = op.indptr[row + 1];
-      if (end - beg > CH) {
-        lidx = lower_bound_i32(op.long_rows, op.n_long, row);
-        end = beg + CH;
-      }
+      if (end - beg > CH) continue;    // long row: all its chunks are units of the chunk list
     } else {
       const int2 ex = reinterpret_cast<const int2*>(op.extra)[u];
       lidx = ex.x;
@@ -415,11 +412,18 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
 #pragma unroll
       for (int e = 0; e < CPL; ++e) acc[e] = T(0);
       const T* base = lpart + (int64_t)op.long_base[l] * R + c0;
-      for (int ch = 0; ch < nch; ++ch) {
-        T pv[CPL];
-        vload<T, CPL>(base + (int64_t)ch * R, pv);
+      for (int ch0 = 0; ch0 < nch; ch0 += 8) {
+        const int nb = min(8, nch - ch0);
+        T pv[8][CPL];
 #pragma unroll
-        for (int e = 0; e < CPL; ++e) acc[e] += pv[e];
+        for (int u = 0; u < 8; ++u)
+          if (u < nb) vload<T, CPL>(base + (int64_t)(ch0 + u) * R, pv[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (u < nb) {
+#pragma unroll
+            for (int e = 0; e < CPL; ++e) acc[e] += pv[u][e];
+          }
       }
       T xv[CPL], yv[CPL];
       finish_row<T, CPL>(op, shift, X, ldx, row, c0, pidx, acc, xv, yv);

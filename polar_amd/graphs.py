@@ -160,3 +160,114 @@ def candidates_fast(n, edges, N, seed, batch=1 << 23):
     got = np.sort(got[:N])
     c = np.stack([got // n, got % n], axis=1)
     return np.ascontiguousarray(c[rng.permutation(N)])
+
+
+# ------------------------------------------------------------------------------------------------
+# Dataset loading (realworld/graphs.py:9-58): SNAP / KONECT edge lists under $POLAR_DATA/raw,
+# relabelled, undirected, deduplicated, self-loops removed, cached as npz.  The SNAP/KONECT
+# files are not shipped; synthetic workloads use the generators above (see configs.make).
+import gzip as _gzip
+import hashlib as _hashlib
+import io as _io
+import json as _json
+import os as _os
+import tarfile as _tarfile
+from pathlib import Path as _Path
+
+SNAP = {
+    'email-Eu-core': 'email-Eu-core.txt.gz', 'ca-GrQc': 'ca-GrQc.txt.gz', 'ca-HepPh': 'ca-HepPh.txt.gz',
+    'ca-CondMat': 'ca-CondMat.txt.gz', 'email-Enron': 'email-Enron.txt.gz',
+    'com-DBLP': 'com-dblp.ungraph.txt.gz', 'com-Amazon': 'com-amazon.ungraph.txt.gz',
+    'web-Google': 'web-Google.txt.gz', 'roadNet-PA': 'roadNet-PA.txt.gz',
+    'com-Youtube': 'com-youtube.ungraph.txt.gz', 'as-Skitter': 'as-skitter.txt.gz',
+    'roadNet-CA': 'roadNet-CA.txt.gz', 'com-LiveJournal': 'com-lj.ungraph.txt.gz',
+}
+KONECT = {
+    'Zachary': 'download.tsv.ucidata-zachary.tar.bz2', 'Dolphins': 'download.tsv.dolphins.tar.bz2',
+    'LesMis': 'download.tsv.moreno_lesmis.tar.bz2', 'Polbooks': 'download.tsv.dimacs10-polbooks.tar.bz2',
+    'Hamster': 'download.tsv.petster-hamster-household.tar.bz2',
+}
+
+
+def data_dir():
+    return _Path(_os.environ.get('POLAR_DATA', _Path.cwd() / 'data'))
+
+
+def _parse(text_bytes, comment):
+    rows = []
+    for line in _io.BytesIO(text_bytes).read().decode(errors='replace').splitlines():
+        line = line.strip()
+        if not line or line.startswith(comment):
+            continue
+        parts = line.split()
+        if len(parts) >= 2:
+            rows.append((int(parts[0]), int(parts[1])))
+    return np.asarray(rows, dtype=np.int64).reshape(-1, 2)
+
+
+def preprocess(raw):
+    """Relabel to 0..n-1 (all endpoint vertices kept), undirect, dedup, drop self-loops."""
+    ids, inv = np.unique(np.asarray(raw).ravel(), return_inverse=True)
+    e = inv.reshape(-1, 2)
+    e = np.sort(e, axis=1)
+    e = e[e[:, 0] != e[:, 1]]
+    e = np.unique(e, axis=0)
+    return len(ids), e.astype(np.int64)
+
+
+def load(name):
+    """(n, edges, meta) from $POLAR_DATA/raw with an npz cache in $POLAR_DATA/npz."""
+    root = data_dir()
+    npz = root / 'npz'
+    cache = npz / f'{name}.npz'
+    if cache.exists():
+        z = np.load(cache, allow_pickle=False)
+        return int(z['n']), z['edges'], _json.loads(str(z['meta']))
+    raw_dir = root / 'raw'
+    if name in SNAP:
+        path = raw_dir / SNAP[name]
+        raw = _parse(_gzip.open(path).read(), '#')
+    elif name in KONECT:
+        path = raw_dir / KONECT[name]
+        with _tarfile.open(path, 'r:bz2') as tf:
+            mem = next(m for m in tf.getmembers() if _Path(m.name).name.startswith('out.'))
+            raw = _parse(tf.extractfile(mem).read(), '%')
+    else:
+        raise FileNotFoundError(f'unknown graph {name!r} (not a SNAP/KONECT name or synthetic config)')
+    n, e = preprocess(raw)
+    meta = dict(name=name, source=path.name, sha256=_hashlib.sha256(path.read_bytes()).hexdigest(),
+                n=int(n), m=int(len(e)),
+                preprocessing='retain all endpoint vertices; undirect; deduplicate; remove self-loops')
+    npz.mkdir(parents=True, exist_ok=True)
+    np.savez(cache, n=n, edges=e, meta=_json.dumps(meta))
+    return n, e, meta
+
+
+def all_nonedges(n, edges):
+    """Every non-edge pair u<v (realworld/graphs.py:72-74), for tiny graphs."""
+    present = set(map(tuple, np.asarray(edges).tolist()))
+    return np.array([(u, v) for u in range(n) for v in range(u + 1, n) if (u, v) not in present],
+                    dtype=np.int64).reshape(-1, 2)
+
+
+def candidates_2hop(n, edges, N, seed):
+    """Friend-of-friend pool by random 2-step walks (realworld/graphs.py:77-93)."""
+    import scipy.sparse as sp
+    A = sp.coo_matrix((np.ones(2 * len(edges)), (np.r_[edges[:, 0], edges[:, 1]], np.r_[edges[:, 1], edges[:, 0]])),
+                      shape=(n, n)).tocsr()
+    ptr, nbr = A.indptr, A.indices
+    deg = np.diff(ptr)
+    keys = np.sort(edges[:, 0] * n + edges[:, 1])
+    rng = np.random.default_rng(seed)
+    C = set()
+    starts = np.flatnonzero(deg > 0)
+    while len(C) < N:
+        u = int(starts[rng.integers(len(starts))])
+        v = int(nbr[ptr[u] + rng.integers(deg[u])])
+        w = int(nbr[ptr[v] + rng.integers(deg[v])])
+        a, b = min(u, w), max(u, w)
+        i = np.searchsorted(keys, a * n + b)
+        if a != b and not (i < len(keys) and keys[i] == a * n + b):
+            C.add((a, b))
+    c = np.asarray(sorted(C), dtype=np.int64)
+    return np.ascontiguousarray(c[rng.permutation(N)])

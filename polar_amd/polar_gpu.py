@@ -102,8 +102,8 @@ class Graph:
             base = np.zeros(len(long_rows), dtype=np.int64)
             if len(long_rows):
                 base[1:] = np.cumsum(nch)[:-1]
-            li = np.repeat(np.arange(len(long_rows)), np.maximum(nch - 1, 0))
-            ci = np.concatenate([np.arange(1, c) for c in nch]) if len(nch) else np.zeros(0, np.int64)
+            li = np.repeat(np.arange(len(long_rows)), nch)
+            ci = np.concatenate([np.arange(c) for c in nch]) if len(nch) else np.zeros(0, np.int64)
             extra = np.stack([li, ci], axis=1).astype(np.int32) if len(li) else np.zeros((0, 2), np.int32)
             self._sched = dict(long_rows=long_rows, long_base=base.astype(np.int32),
                                extra=np.ascontiguousarray(extra), n_partial=int(nch.sum()))
