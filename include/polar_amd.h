@@ -112,6 +112,15 @@ int polar_pair_sqnorm(int32_t dtype, const void* X, int64_t x_sj, int64_t x_si, 
                       int64_t q, const int32_t* ea, const int32_t* eb, int64_t N, double* out,
                       void* stream);
 
+/* out[e] (+)= sum_j om_j sum_c (X[j, ea[e], c] - X[j, eb[e], c])^2 over E index pairs (rows of X).
+ * With X_j = (M + tau_j)^{-1} Z on all rows this is minus the derivative of the sketched trace
+ * sum_j om_j tr(Z^T (M + tau_j)^{-1} Z) with respect to the weight of pair e (the adjoint form;
+ * no counterpart kernel in the reference, whose h cache at polar_gpu.py:205 holds the per-pole
+ * terms for candidate pairs).  accumulate != 0 adds to out (column-chunked streaming). */
+int polar_pair_wsum(int32_t dtype, const void* X, int64_t x_sj, int64_t x_si, int32_t p, int64_t q,
+                    const int32_t* ea, const int32_t* eb, int64_t E, const double* om, double* out,
+                    int32_t accumulate, void* stream);
+
 /* score[e] = sum_j om_j h/(1 + t + (sigma + tau_j) h)  (obj 0 = frac/er), or
  * log1p(t[0,e] + sigma h[0,e]) (obj 2 = logdet); dead[e] != 0 -> -inf (polar_gpu.py:208-219). */
 int polar_proxy(int32_t obj, const double* h, const double* t, const double* tau, const double* om,

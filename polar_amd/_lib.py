@@ -21,7 +21,7 @@ EXPORTS = [
     'polar_msolve_ws_bytes', 'polar_msolve', 'polar_pair_sqnorm', 'polar_proxy',
     'polar_select_ws_bytes', 'polar_select_topb', 'polar_pair_rhs', 'polar_rerank_pick',
     'polar_maint_prep', 'polar_maint_rank1', 'polar_maint_cache', 'polar_rows_axpy',
-    'polar_rademacher', 'polar_edge_sketch', 'polar_prof_set', 'polar_prof_get',
+    'polar_rademacher', 'polar_edge_sketch', 'polar_prof_set', 'polar_prof_get', 'polar_pair_wsum',
 ]
 
 vp = ctypes.c_void_p
@@ -65,6 +65,7 @@ _SIGS = {
     'polar_rademacher': (i32, [i32, u64, i32, vp, i64, i64, f64, vp, i64, vp]),
     'polar_edge_sketch': (i32, [OPP, u64, i64, f64, vp, i64, vp]),
     'polar_prof_set': (i32, [i32]),
+    'polar_pair_wsum': (i32, [i32, vp, i64, i64, i32, i64, vp, vp, i64, vp, vp, i32, vp]),
     'polar_prof_get': (i32, [i32, vp, vp, vp]),
 }
 

@@ -61,6 +61,39 @@ __global__ __launch_bounds__(kBlock) void k_pair_sqnorm(const T* __restrict__ X,
   }
 }
 
+// out[e] (+)= sum_j om_j |X[j, ea[e], :] - X[j, eb[e], :]|^2 (pole-weighted pair norms; the
+// adjoint of the sketched trace with respect to the weight of pair e is minus this sum).
+template <typename T, int CPL>
+__global__ __launch_bounds__(kBlock) void k_pair_wsum(const T* __restrict__ X, int64_t sj, int64_t si, int p,
+                                                      int64_t q, const int32_t* __restrict__ ea,
+                                                      const int32_t* __restrict__ eb, int64_t E,
+                                                      const double* __restrict__ om, double* __restrict__ out,
+                                                      int accumulate, int L) {
+  Geo g(L);
+  const int c0 = g.li * CPL;
+  const int64_t gpb = (int64_t)kWaves * g.G;
+  for (int64_t e = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi; e < E; e += (int64_t)gridDim.x * gpb) {
+    const int64_t a = ea[e], b = eb[e];
+    DD tot{0.0, 0.0};
+    for (int j = 0; j < p; ++j) {
+      DD s{0.0, 0.0};
+      if (c0 < q) {
+        double va[CPL], vb[CPL];
+        load_cols<T, CPL>(X + (int64_t)j * sj + a * si, c0, q, va);
+        load_cols<T, CPL>(X + (int64_t)j * sj + b * si, c0, q, vb);
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) {
+          const double df = va[k] - vb[k];
+          dd_add(s, df * df);
+        }
+      }
+      s = dd_group_sum(s, L);
+      dd_add(tot, om[j] * (s.hi + s.lo));
+    }
+    if (g.li == 0) out[e] = (accumulate ? out[e] : 0.0) + (tot.hi + tot.lo);
+  }
+}
+
 // score[e] = sum_j om_j h / (1 + t + (sigma + tau_j) h)  | log1p(t0 + sigma h0); dead -> -inf
 __global__ __launch_bounds__(kBlock) void k_proxy(int obj, const double* __restrict__ h,
                                                   const double* __restrict__ t, const double* __restrict__ tau,
@@ -420,6 +453,28 @@ int polar_pair_sqnorm(int32_t dtype, const void* X, int64_t x_sj, int64_t x_si, 
     if (cpl == 4) hipLaunchKernelGGL((k_pair_sqnorm<float, 4>), dim3(grid), dim3(kBlock), 0, st, (const float*)X, x_sj, x_si, p, q, ea, eb, N, out, L);
     else if (cpl == 2) hipLaunchKernelGGL((k_pair_sqnorm<float, 2>), dim3(grid), dim3(kBlock), 0, st, (const float*)X, x_sj, x_si, p, q, ea, eb, N, out, L);
     else hipLaunchKernelGGL((k_pair_sqnorm<float, 1>), dim3(grid), dim3(kBlock), 0, st, (const float*)X, x_sj, x_si, p, q, ea, eb, N, out, L);
+  }
+  POLAR_LAUNCH_CHECK();
+  return 0;
+}
+
+int polar_pair_wsum(int32_t dtype, const void* X, int64_t x_sj, int64_t x_si, int32_t p, int64_t q,
+                    const int32_t* ea, const int32_t* eb, int64_t E, const double* om, double* out,
+                    int32_t accumulate, void* stream) {
+  if (!X || !ea || !eb || !om || !out || p <= 0 || q <= 0 || E < 0) return set_error(POLAR_E_ARG, "bad pair_wsum args");
+  if (E == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  const int cpl = pick_cpl_rows(dtype, q, X, x_sj, x_si);
+  const int L = lanes_for(q, cpl);
+  if (L > 64) return set_error(POLAR_E_ARG, "q=%lld too wide for pair_wsum at this alignment", (long long)q);
+  const int grid = grid_for(E, kWaves * (64 / L));
+  if (dtype == POLAR_F64) {
+    if (cpl == 2) hipLaunchKernelGGL((k_pair_wsum<double, 2>), dim3(grid), dim3(kBlock), 0, st, (const double*)X, x_sj, x_si, p, q, ea, eb, E, om, out, accumulate, L);
+    else hipLaunchKernelGGL((k_pair_wsum<double, 1>), dim3(grid), dim3(kBlock), 0, st, (const double*)X, x_sj, x_si, p, q, ea, eb, E, om, out, accumulate, L);
+  } else {
+    if (cpl == 4) hipLaunchKernelGGL((k_pair_wsum<float, 4>), dim3(grid), dim3(kBlock), 0, st, (const float*)X, x_sj, x_si, p, q, ea, eb, E, om, out, accumulate, L);
+    else if (cpl == 2) hipLaunchKernelGGL((k_pair_wsum<float, 2>), dim3(grid), dim3(kBlock), 0, st, (const float*)X, x_sj, x_si, p, q, ea, eb, E, om, out, accumulate, L);
+    else hipLaunchKernelGGL((k_pair_wsum<float, 1>), dim3(grid), dim3(kBlock), 0, st, (const float*)X, x_sj, x_si, p, q, ea, eb, E, om, out, accumulate, L);
   }
   POLAR_LAUNCH_CHECK();
   return 0;
