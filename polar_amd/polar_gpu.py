@@ -237,12 +237,13 @@ def _max_cols(dtype):
 
 
 def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, freeze=True,
-           trace=False, out=None):
+           trace=False, out=None, width=None):
     """Block multi-shift CG for (M + tau_j I) U = W on all poles (polar_gpu.py:88-133).
 
     Returns (X[p, nA, R], h[p, R] or None, iters[R] host int64) and, with trace=True, a 4th value
     te[p, R] = w^T (M + tau_j)^{-1} w.  `out` may be a (p, nA, R) view with unit column stride
-    into which X is written (the bank)."""
+    into which X is written (the bank).  `width` caps the columns solved together (the workspace
+    scales with it); columns are independent, so the results do not depend on it."""
     lib = L.require_gpu()
     dev = _device()
     dtype = op.dtype
@@ -269,7 +270,7 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
     iters = torch.empty(R, dtype=torch.int64, device=dev)
     flags = (L.POLAR_TRACK if track else 0) | (L.POLAR_TRACE if trace else 0) | (0 if freeze else L.POLAR_NOFREEZE)
     s = op.c_op(dtype)
-    step = min(R, _max_cols(dtype))
+    step = min(R, _max_cols(dtype), width or R)
     ws_bytes = lib.polar_msolve_ws_bytes(s, step, p, nA, flags)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     for lo in range(0, R, step):
@@ -405,12 +406,18 @@ def chunk_width(n, p, nA, budget=10e9):
 def bank_width(n, p, nA, q, dtype, budget=None):
     """Columns per bank block solve on one MI355X: as wide as one wave covers (128 FP64 / 256
     FP32 columns) within a memory budget for the shifted P block and the CG vectors."""
+    return fit_width(n, p, nA, 2 * q, dtype, budget)
+
+
+def fit_width(n, p, nA, R, dtype, budget=None):
+    """Columns (at most R) whose msolve workspace fits in `budget` bytes (default: half of the
+    free device memory): 4 n-row CG vectors and the shifted P block per column."""
     esz = 8 if dtype == torch.float64 else 4
     if budget is None:
         free, _ = torch.cuda.mem_get_info()
         budget = 0.5 * free
     per_col = esz * (4 * n + p * nA) + 64
-    w = int(min(2 * q, _max_cols(dtype), max(1, budget // per_col)))
+    w = int(min(R, _max_cols(dtype), max(1, budget // per_col)))
     if w >= 8:
         w -= w % 8
     return max(1, w)
@@ -440,11 +447,13 @@ class _Phases:
 # ------------------------------------------------------------------------------------------------
 def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=None, naive=False,
           seed=2026, kind='block', trace=None, time_limit=None, poles=None, width=None,
-          freeze=True, precision='fp64', profile=False):
+          freeze=True, precision='fp64', profile=False, progress=None):
     """POLAR greedy, cache backend (polar_gpu.py:180-254), on the GPU.
 
     profile=True adds `phases`: seconds per phase (bank, select, rerank_solve, rerank_pick,
     maintenance, update), measured with a device synchronisation at each phase boundary.
+    progress: optional callable(stage, info) called after the bank ('bank', {seconds, spmv}) and
+    after each round ('round', {round, edge, marginal, seconds, spmv}); it reads host values only.
 
     Returns dict(sequence, seconds, spmv, rhs, p, rounds, truncated, marginals, trace_estimate,
     solves, precision)."""
@@ -502,9 +511,9 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     t0 = time.perf_counter()                                                      # :196
     ph = _Phases(profile)
 
-    if width is None:
-        width = bank_width(n, p, nA, q, dtype)
     XY = torch.empty((p, nA, 2 * q), dtype=dtype, device=dev)
+    if width is None:
+        width = bank_width(n, p, nA, q, dtype)          # sized on what is free after the caches
 
     def bank():                                                                    # :200-206
         terms = []
@@ -527,6 +536,8 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
 
     X, Y, h, t, tr = bank()
     ph.mark('bank')
+    if progress is not None:
+        progress('bank', dict(seconds=time.perf_counter() - t0, spmv=cnt.spmv))
     traces.append(tr if want_trace else None)
     for rnd in range(k):
         if rnd and time_limit is not None and time.perf_counter() - t0 > time_limit:   # :215
@@ -551,7 +562,10 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             L.check(lib.polar_pair_rhs(dcode, n, Br, L.ptr(a_idx), L.ptr(b_idx), L.ptr(Wb), stream), 'pair_rhs')
             need_track = obj != 'logdet'
             ph.mark('select')
-            U, hh, _ = msolve(op, Wb, tau, active, tol, track=need_track, counter=cnt, freeze=freeze)[:3]
+            U = torch.empty((p, nA, Br), dtype=dtype, device=dev)
+            rw = fit_width(n, p, nA, Br, dtype)
+            U, hh, _ = msolve(op, Wb, tau, active, tol, track=need_track, counter=cnt, freeze=freeze, out=U,
+                              width=rw)[:3]
             ph.mark('rerank_solve')
             sa = torch.from_numpy(es_h[short, 0].astype(np.int32)).to(dev)
             sb = torch.from_numpy(es_h[short, 1].astype(np.int32)).to(dev)
@@ -593,6 +607,9 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
         ph.mark('maintenance')
         op.add(a, b)                                                               # :251
         ph.mark('update')
+        if progress is not None:
+            progress('round', dict(round=rnd, edge=e, marginal=marg[-1], seconds=time.perf_counter() - t0,
+                                   spmv=cnt.spmv))
     _sync()
     seconds = time.perf_counter() - t0
     if want_trace and not naive:

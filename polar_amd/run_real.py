@@ -92,6 +92,19 @@ def _sk(a, G, cand, seed=2026):
     return pg.sketches(G, cand, a.q, seed, kind=a.kind)
 
 
+def _progress(a, name, label):
+    if not a.progress:
+        return None
+
+    def show(stage, info):
+        if stage == 'bank':
+            print(f"{name} {label} bank {info['seconds']:.2f}s spmv={info['spmv']}", flush=True)
+        else:
+            print(f"{name} {label} round {info['round']} edge={info['edge']} marginal={info['marginal']:.6g} "
+                  f"{info['seconds']:.2f}s spmv={info['spmv']}", flush=True)
+    return show
+
+
 def _clean(out):
     return {k: v for k, v in out.items() if k not in ('solves',)}
 
@@ -154,11 +167,17 @@ def timing(a):
     for name in a.graphs:
         G, cand, meta = setup(name, a.N, sigma=a.sigma)
         sk = _sk(a, G, cand)
+        if torch.cuda.is_available():
+            free, total = torch.cuda.mem_get_info()
+            meta.update(device_memory_total=int(total), device_memory_free_at_start=int(free))
         res = dict(mode='time', graph=meta, params=dict(k=a.k, q=a.q, B=a.B, eps=a.eps, tol=a.tol, s=a.s[0],
                    repetitions=a.reps, precision=a.precision, poles=a.poles or 'composite'), env=env(), runs={})
         cfg = {'Naive': dict(naive=True), 'POLAR(B=1)': dict(B=1), f'POLAR(B={a.B})': dict(B=a.B)}
         if a.skip_naive:
             cfg.pop('Naive')
+        if a.only:
+            keep = {'Naive': 'Naive', 'B1': 'POLAR(B=1)', 'B': f'POLAR(B={a.B})'}
+            cfg = {k: v for k, v in cfg.items() if k in [keep[o] for o in a.only]}
         names = list(cfg)
         reps = a.reps if meta['m'] < 2e7 else 1
         res['params'].update(naive_reps=a.naive_reps, naive_limit=a.naive_limit, repetitions=reps)
@@ -170,7 +189,7 @@ def timing(a):
                 lim = a.naive_limit if nm == 'Naive' else None
                 out = pg.polar(G, cand, a.k, a.s[0], q=a.q, eps=a.eps, tol=a.tol, sketch=sk, time_limit=lim,
                                poles=_poles(a, G, a.k, a.s[0], 'frac'), precision=a.precision,
-                               **{'B': a.B, **cfg[nm]})
+                               progress=_progress(a, name, nm), **{'B': a.B, **cfg[nm]})
                 res['runs'].setdefault(nm, []).append(_clean(out))
                 print(name, nm, rep, f"{out['seconds']:.2f}s spmv={out['spmv']}", flush=True)
                 out_dir.mkdir(parents=True, exist_ok=True)
@@ -203,6 +222,8 @@ def parser():
     P.add_argument('--kind', choices=['block', 'gauss', 'rademacher'], default='block')
     P.add_argument('--precision', choices=['fp64', 'fp32'], default='fp64')
     P.add_argument('--out', default=str(OUT))
+    P.add_argument('--progress', action='store_true', help='print the bank and every round as they finish')
+    P.add_argument('--only', nargs='+', default=None, help='time only these runs (Naive, B1, B)')
     return P
 
 
