@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define POLAR_ABI_VERSION 1
+#define POLAR_ABI_VERSION 2
 
 enum polar_dtype { POLAR_F64 = 0, POLAR_F32 = 1 };
 
@@ -47,6 +47,7 @@ enum polar_status {
 #define POLAR_TRACK 1     /* track ||u_j||^2 with three scalars per pole (polar_gpu.py:114-117) */
 #define POLAR_TRACE 2     /* track w^T (M + tau_j)^{-1} w with two scalars per pole */
 #define POLAR_NOFREEZE 4  /* never freeze converged poles (synthetic/scalability/benchmark.py:72) */
+#define POLAR_GRAPH 8     /* replay the CG iterations as hipGraphs of 8 iterations (same results) */
 
 /* Operator y = (diag + shift) x - A_w x + patch(x): the graph Laplacian M0 = diag(deg+sigma) - A
  * of realworld/polar_gpu.py:47-55 stored as diagonal + off-diagonal adjacency (value-free when
@@ -71,11 +72,26 @@ typedef struct polar_op {
   const int32_t* long_base;   /* [n_long] first partial slot of each long row */
   const int32_t* extra;       /* [n_extra][2]: (long index, chunk index) */
   int32_t n_patch;            /* rows touched by added edges */
-  int32_t pad1;
+  int32_t cpl_max;            /* tuning: max columns per lane (1, 2, 4; 0 = widest 16-byte load) */
   const int32_t* patch_rows;  /* [n_patch] ascending */
   const int32_t* patch_ptr;   /* [n_patch+1] */
   const int32_t* patch_cols;  /* [patch_ptr[n_patch]] */
+  int32_t grid_max;           /* tuning: block budget of the kernels with partial sums (0 = default) */
+  int32_t pad2;
 } polar_op;
+
+/* Segments of a block-diagonal operator (the batched form: many graphs in one operator, each on
+ * its own row range, with its own poles and CG scalars).  Host arrays of S+1 ascending offsets:
+ * rows, entries of op.extra (long-row chunks), long rows (op.long_rows), active rows.  Segment g
+ * owns rows [rows[g], rows[g+1]) and its long rows, chunks and active rows lie in its ranges. */
+typedef struct polar_segs {
+  int32_t S;
+  int32_t pad;
+  const int64_t* rows;
+  const int64_t* extra;
+  const int64_t* longs;
+  const int64_t* active;
+} polar_segs;
 
 int polar_abi_version(void);
 const char* polar_last_error(void);
@@ -106,6 +122,22 @@ int polar_msolve(const polar_op* op, const void* W, int64_t ldw, int64_t R, cons
                  int32_t p, const int32_t* active, int64_t nA, double tol, int64_t maxit,
                  int32_t flags, void* Xout, int64_t x_sj, int64_t x_si, double* h_out,
                  double* te_out, int64_t* iters_out, void* ws, int64_t ws_bytes, void* stream);
+
+/* Batched form of polar_msolve: S independent systems (M_g + tau[g*p + j] I) U = W[rows of g, :R]
+ * on the segments of one block-diagonal operator, advanced together -- one launch per kernel for
+ * all segments, each (segment, column) with its own CG scalars, finished segments skipped and
+ * their blocks given to the others.  Segment g's results equal those of polar_msolve on graph g
+ * alone.  The reference runs graphs one at a time (realworld/run_real.py:114-133 loops over
+ * graphs calling polar_gpu.msolve, polar_gpu.py:88-133); this entry point has no counterpart
+ * there.  tau: device double[S][p].  active: device int32[nA], nA = segs->active[S], union row
+ * ids.  Xout (j, i, c) at Xout + j*x_sj + i*x_si + c over the nA active rows.  h_out, te_out:
+ * device double[p][S][R] or NULL; iters_out: device int64[S][R]. */
+int64_t polar_msolve_batched_ws_bytes(const polar_op* op, const polar_segs* segs, int64_t R, int32_t p,
+                                      int32_t flags);
+int polar_msolve_batched(const polar_op* op, const polar_segs* segs, const void* W, int64_t ldw, int64_t R,
+                         const double* tau, int32_t p, const int32_t* active, double tol, int64_t maxit,
+                         int32_t flags, void* Xout, int64_t x_sj, int64_t x_si, double* h_out,
+                         double* te_out, int64_t* iters_out, void* ws, int64_t ws_bytes, void* stream);
 
 /* out[j*N + e] = sum_c (X[j, ea[e], c] - X[j, eb[e], c])^2, double (polar_gpu.py:205). */
 int polar_pair_sqnorm(int32_t dtype, const void* X, int64_t x_sj, int64_t x_si, int32_t p,

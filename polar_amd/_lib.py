@@ -12,7 +12,8 @@ import torch
 LIB_PATH = Path(__file__).resolve().parent / 'libpolar_amd.so'
 
 POLAR_F64, POLAR_F32 = 0, 1
-POLAR_TRACK, POLAR_TRACE, POLAR_NOFREEZE = 1, 2, 4
+POLAR_TRACK, POLAR_TRACE, POLAR_NOFREEZE, POLAR_GRAPH = 1, 2, 4, 8
+ABI_VERSION = 2
 POLAR_E_ITERCAP, POLAR_E_ARG = -2, -3
 CHUNK = 64  # POLAR_CHUNK: default long-row chunk length
 
@@ -22,6 +23,7 @@ EXPORTS = [
     'polar_select_ws_bytes', 'polar_select_topb', 'polar_pair_rhs', 'polar_rerank_pick',
     'polar_maint_prep', 'polar_maint_rank1', 'polar_maint_cache', 'polar_rows_axpy',
     'polar_rademacher', 'polar_edge_sketch', 'polar_prof_set', 'polar_prof_get', 'polar_pair_wsum',
+    'polar_msolve_batched_ws_bytes', 'polar_msolve_batched',
 ]
 
 vp = ctypes.c_void_p
@@ -37,12 +39,18 @@ class PolarOp(ctypes.Structure):
         ('indptr', vp), ('indices', vp), ('weights', vp), ('diag', vp),
         ('n_long', i32), ('n_extra', i32), ('n_partial', i32), ('chunk', i32),
         ('long_rows', vp), ('long_base', vp), ('extra', vp),
-        ('n_patch', i32), ('pad1', i32),
+        ('n_patch', i32), ('cpl_max', i32),
         ('patch_rows', vp), ('patch_ptr', vp), ('patch_cols', vp),
+        ('grid_max', i32), ('pad2', i32),
     ]
 
 
+class PolarSegs(ctypes.Structure):
+    _fields_ = [('S', i32), ('pad', i32), ('rows', vp), ('extra', vp), ('longs', vp), ('active', vp)]
+
+
 OPP = ctypes.POINTER(PolarOp)
+SEGP = ctypes.POINTER(PolarSegs)
 
 _SIGS = {
     'polar_abi_version': (i32, []),
@@ -67,6 +75,9 @@ _SIGS = {
     'polar_prof_set': (i32, [i32]),
     'polar_pair_wsum': (i32, [i32, vp, i64, i64, i32, i64, vp, vp, i64, vp, vp, i32, vp]),
     'polar_prof_get': (i32, [i32, vp, vp, vp]),
+    'polar_msolve_batched_ws_bytes': (i64, [OPP, SEGP, i64, i32, i32]),
+    'polar_msolve_batched': (i32, [OPP, SEGP, vp, i64, i64, vp, i32, vp, f64, i64, i32, vp, i64, i64, vp, vp,
+                                   vp, vp, i64, vp]),
 }
 
 _lib = None
@@ -85,7 +96,7 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.polar_abi_version() != 1:
+    if lib.polar_abi_version() != ABI_VERSION:
         raise RuntimeError('libpolar_amd.so ABI mismatch')
     _lib = lib
     return lib

@@ -1,0 +1,345 @@
+"""Batched form: POLAR greedy on many independent graphs at once.
+
+The reference runs one graph at a time (realworld/run_real.py:114-133 loops over graphs, each
+calling polar_gpu.polar, polar_gpu.py:180-254).  Small graphs leave the GPU idle: one CG iteration
+of a 10k-row graph is a handful of microsecond-scale kernels.  Here all graphs become one
+block-diagonal operator (`UnionGraph`), each graph a *segment* with its own poles and its own CG
+scalars for every column; `polar_msolve_batched` advances every segment's solve with one launch per
+kernel, skips finished segments and re-plans the blocks over the segments still iterating, and
+(optionally) replays the CG iterations as hipGraphs.  The greedy rounds run in lockstep: one
+batched bank solve, then per round one batched re-rank solve over all graphs' shortlists; the
+per-graph proxy, shortlist, pick and Sherman-Morrison maintenance kernels work on views of the
+batched caches.
+
+Segment g's arithmetic is the single-graph arithmetic (same operator rows, same reductions), so
+in FP64 every graph's selections, marginals, trace estimates and counters equal those of
+`polar_gpu.polar` on that graph alone.
+"""
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import polar_gpu as pg
+
+
+class UnionGraph(pg.Graph):
+    """Block-diagonal union of graphs: graph g on rows [row0[g], row0[g+1])."""
+
+    def __init__(self, graphs):
+        graphs = list(graphs)
+        ns = np.array([G.n for G in graphs], dtype=np.int64)
+        row0 = np.zeros(len(graphs) + 1, dtype=np.int64)
+        row0[1:] = np.cumsum(ns)
+        edges = np.concatenate([G.edges + row0[g] for g, G in enumerate(graphs)])
+        weights = np.concatenate([G.weights for G in graphs])
+        super().__init__(int(row0[-1]), edges, 0.0, weights=weights)
+        self.graphs = graphs
+        self.row0 = row0
+        # diag = deg + sigma of each row's own graph (the same doubles as each graph alone)
+        self.sigma = np.repeat(np.array([G.sigma for G in graphs], dtype=np.float64), ns)
+        self.dmax = max(G.dmax for G in graphs)
+
+    def Lam(self, k):
+        raise ValueError('a union graph has one Lam per segment: use graphs[g].Lam(k)')
+
+    def segments(self, a0):
+        """(rows, extra, longs, active) offsets of the segments, given the active-row offsets."""
+        sc = self.schedule()
+        lr = sc['long_rows']
+        longs = np.searchsorted(lr, self.row0).astype(np.int64)
+        base = np.r_[sc['long_base'].astype(np.int64), sc['n_partial']]
+        extra = base[longs]
+        return self.row0.astype(np.int64), extra, longs, np.asarray(a0, dtype=np.int64)
+
+
+class Segments:
+    """ctypes polar_segs over host offset arrays (kept alive by this object)."""
+
+    def __init__(self, rows, extra, longs, active):
+        self.arrays = [np.ascontiguousarray(a, dtype=np.int64) for a in (rows, extra, longs, active)]
+        self.S = len(rows) - 1
+        s = L.PolarSegs()
+        s.S = self.S
+        s.rows, s.extra, s.longs, s.active = (a.ctypes.data_as(L.vp) for a in self.arrays)
+        self.c = s
+
+    @property
+    def a0(self):
+        return self.arrays[3]
+
+
+def msolve_batched(op, segs, W, tau, active, tol, maxit=100000, track=False, trace=False, out=None,
+                   width=None, graph=False, freeze=True):
+    """Batched block multi-shift CG: segment g solves (M_g + tau[g, j]) U = W[rows of g, :R].
+
+    op: Op of a UnionGraph; segs: Segments; W: (n, R) device; tau: (S, p) host or device; active:
+    device int32 union row ids (segment g's at segs.a0[g]:segs.a0[g+1]).  Returns X (p, nA, R),
+    h (p, S, R) or None, iters (S, R) host int64, and te (p, S, R) with trace=True."""
+    lib = L.require_gpu()
+    dev = pg._device()
+    dtype = op.dtype
+    W = torch.as_tensor(W, device=dev).to(dtype)
+    if W.stride(1) != 1:
+        W = W.contiguous()
+    n, R = W.shape
+    S = segs.S
+    tau_d = torch.as_tensor(tau, dtype=torch.float64, device=dev).reshape(S, -1).contiguous()
+    p = tau_d.shape[1]
+    nA = int(segs.a0[-1])
+    X = out if out is not None else torch.empty((p, nA, R), dtype=dtype, device=dev)
+    assert X.shape == (p, nA, R) and X.stride(2) == 1
+    h = torch.empty((p, S, R), dtype=torch.float64, device=dev) if track else None
+    te = torch.empty((p, S, R), dtype=torch.float64, device=dev) if trace else None
+    iters = torch.empty((S, R), dtype=torch.int64, device=dev)
+    flags = ((L.POLAR_TRACK if track else 0) | (L.POLAR_TRACE if trace else 0) |
+             (0 if freeze else L.POLAR_NOFREEZE) | (L.POLAR_GRAPH if graph else 0))
+    s = op.c_op(dtype)
+    step = min(R, pg._max_cols(dtype), width or R)
+    ws_bytes = lib.polar_msolve_batched_ws_bytes(s, segs.c, step, p, flags)
+    L.check(0 if ws_bytes > 0 else L.POLAR_E_ARG, 'polar_msolve_batched_ws_bytes')
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    act = active if nA else torch.zeros(1, dtype=torch.int32, device=dev)
+    for lo in range(0, R, step):
+        r = min(step, R - lo)
+        hb = (h if r == R else torch.empty((p, S, r), dtype=torch.float64, device=dev)) if track else None
+        tb = (te if r == R else torch.empty((p, S, r), dtype=torch.float64, device=dev)) if trace else None
+        ib = iters if r == R else torch.empty((S, r), dtype=torch.int64, device=dev)
+        rc = lib.polar_msolve_batched(s, segs.c, L.ptr(W[:, lo:]), W.stride(0), r, L.ptr(tau_d), p, L.ptr(act),
+                                      float(tol), int(maxit), flags, L.ptr(X[:, :, lo:]), X.stride(0), X.stride(1),
+                                      L.ptr(hb), L.ptr(tb), L.ptr(ib), L.ptr(ws), ws_bytes, L.stream_ptr())
+        L.check(rc, 'polar_msolve_batched')
+        if r != R:
+            iters[:, lo:lo + r] = ib
+            if track:
+                h[:, :, lo:lo + r] = hb
+            if trace:
+                te[:, :, lo:lo + r] = tb
+    res = (X, h, iters.cpu().numpy())
+    if trace:
+        res = res + (te,)
+    return res
+
+
+def _seq(x, S):
+    if isinstance(x, (list, tuple, np.ndarray)):
+        assert len(x) == S
+        return list(x)
+    return [x] * S
+
+
+def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac', sketch=None, seed=2026,
+                  kind='rademacher', poles=None, precision='fp32', graph=True, width=None, progress=None):
+    """POLAR greedy (polar_gpu.py:180-254, cache backend) on S graphs in lockstep.
+
+    graphs: list of Graph; cands: list of (N_g, 2) candidate arrays; s: one value or one per graph;
+    sketch: list of (Z, F, Gc) per graph (default: Rademacher with seed + g); poles: list of
+    (tau, om) per graph (all with the same pole count) or None for `poles(obj, s_g, ...)`.
+    Returns one result dict per graph with the keys of polar_gpu.polar (`seconds` is the whole
+    batch's wall time)."""
+    lib = L.require_gpu()
+    dev = pg._device()
+    dtype = pg._dt(precision)
+    dcode = L.dtype_code(dtype)
+    stream = L.stream_ptr()
+    S = len(graphs)
+    s_all = _seq(s, S)
+    U = UnionGraph(graphs)
+    row0 = U.row0
+    objc = 2 if obj == 'logdet' else 0
+    want_trace = obj != 'logdet'
+    taus, oms = [], []
+    for g, G in enumerate(graphs):
+        if poles is None:
+            tau, om, _ = pg.poles(obj, s_all[g], G.sigma, G.Lam(k), eps)
+        else:
+            tau, om = (np.asarray(a, float) for a in poles[g])
+        taus.append(tau)
+        oms.append(om)
+    p = len(taus[0])
+    if any(len(t) != p for t in taus):
+        raise ValueError('the batched form needs the same pole count for every graph')
+    tau_all = torch.from_numpy(np.stack(taus)).to(dev)
+    om_d = [torch.from_numpy(om).to(dev) for om in oms]
+    cands = [np.asarray(c) for c in cands]
+    if sketch is None:
+        sketch = [pg.sketches(G, cands[g], q, seed + g, kind=kind, on_device='lazy' if kind == 'rademacher' else False,
+                              precision=dtype) for g, G in enumerate(graphs)]
+    q = sketch[0].q if isinstance(sketch[0], pg.DeviceSketch) else sketch[0][0].shape[1]
+    n = U.n
+    RHS = torch.empty((n, 2 * q), dtype=dtype, device=dev)
+    actives, es, Gcs = [], [], []
+    for g in range(S):
+        Zg, Fg = RHS[row0[g]:row0[g + 1], :q], RHS[row0[g]:row0[g + 1], q:]
+        if isinstance(sketch[g], pg.DeviceSketch):
+            sketch[g].fill(Zg, Fg, pg.Op(graphs[g], precision=dtype))
+            Gc = sketch[g].rows
+        else:
+            Z, F, Gc = sketch[g]
+            Zg.copy_(torch.as_tensor(Z, device=dev).to(dtype))
+            Fg.copy_(torch.as_tensor(F, device=dev).to(dtype))
+        if isinstance(Gc, torch.Tensor):
+            Gcs.append(Gc.to(dev, torch.float64))
+        elif isinstance(Gc, pg.RademacherRows):
+            Gcs.append(Gc)
+        else:
+            Gcs.append(np.asarray(Gc, dtype=np.float64))
+        act = np.unique(cands[g])
+        actives.append(act)
+        es.append(np.searchsorted(act, cands[g]))
+    a0 = np.zeros(S + 1, dtype=np.int64)
+    a0[1:] = np.cumsum([len(a) for a in actives])
+    segs = Segments(*U.segments(a0))
+    active = torch.from_numpy(np.concatenate([a + row0[g] for g, a in enumerate(actives)]).astype(np.int32)).to(dev)
+    nA = int(a0[-1])
+    Ns = [len(c) for c in cands]
+    ea = [torch.from_numpy(e[:, 0].astype(np.int32)).to(dev) for e in es]
+    eb = [torch.from_numpy(e[:, 1].astype(np.int32)).to(dev) for e in es]
+    op = pg.Op(U, precision=dtype)
+    spmv = np.zeros(S, dtype=np.int64)
+    rhs = np.zeros(S, dtype=np.int64)
+    live = [np.ones(N, bool) for N in Ns]
+    dead = [torch.zeros(N, dtype=torch.uint8, device=dev) for N in Ns]
+    score = [torch.empty(N, dtype=torch.float64, device=dev) for N in Ns]
+    sel_ws = torch.empty(lib.polar_select_ws_bytes(max(Ns)), dtype=torch.uint8, device=dev)
+    sel_out = torch.empty(64, dtype=torch.int32, device=dev)
+    coef = torch.empty((p, 2 * q + 3), dtype=torch.float64, device=dev)
+    wX = torch.empty((p, max(len(a) for a in actives)), dtype=torch.float64, device=dev)
+    wY = torch.empty_like(wX)
+    gvec = torch.empty(max(B, 1) + 2, dtype=torch.float64, device=dev)
+    gc_row = torch.empty(q, dtype=torch.float64, device=dev)
+    seqs = [[] for _ in range(S)]
+    margs = [[] for _ in range(S)]
+    traces = [[] for _ in range(S)]
+    tr_hist = [[] for _ in range(S)]
+    pg._sync()
+    t0 = time.perf_counter()
+
+    XY = torch.empty((p, nA, 2 * q), dtype=dtype, device=dev)
+    if width is None:
+        width = pg.fit_width(n, p, nA, 2 * q, dtype)
+    terms = [[] for _ in range(S)]
+    for lo in range(0, 2 * q, width):
+        hi = min(2 * q, lo + width)
+        res = msolve_batched(op, segs, RHS[:, lo:hi], tau_all, active, tol, trace=want_trace and lo < q,
+                             out=XY[:, :, lo:hi], graph=graph)
+        spmv += res[2].sum(axis=1)
+        rhs += hi - lo
+        if want_trace and lo < q:
+            ncol = min(q, hi) - lo
+            te = res[3][:, :, :ncol].cpu().numpy()
+            for g in range(S):
+                terms[g].append((oms[g][:, None] * te[:, g, :]).ravel())
+    X, Y = XY[:, :, :q], XY[:, :, q:]
+    hs, ts = [], []
+    for g in range(S):
+        Xg, Yg = X[:, a0[g]:], Y[:, a0[g]:]
+        h = torch.empty((p, Ns[g]), dtype=torch.float64, device=dev)
+        t = torch.empty((p, Ns[g]), dtype=torch.float64, device=dev)
+        L.check(lib.polar_pair_sqnorm(dcode, L.ptr(Xg), Xg.stride(0), Xg.stride(1), p, q, L.ptr(ea[g]),
+                                      L.ptr(eb[g]), Ns[g], L.ptr(h), stream), 'pair_sqnorm')
+        L.check(lib.polar_pair_sqnorm(dcode, L.ptr(Yg), Yg.stride(0), Yg.stride(1), p, q, L.ptr(ea[g]),
+                                      L.ptr(eb[g]), Ns[g], L.ptr(t), stream), 'pair_sqnorm')
+        hs.append(h)
+        ts.append(t)
+        traces[g].append(math.fsum(np.concatenate(terms[g])) if want_trace else None)
+    if progress is not None:
+        progress('bank', dict(seconds=time.perf_counter() - t0, spmv=int(spmv.sum())))
+
+    for rnd in range(k):
+        shorts = []
+        for g in range(S):
+            L.check(lib.polar_proxy(objc, L.ptr(hs[g]), L.ptr(ts[g]), L.ptr(tau_all[g]), L.ptr(om_d[g]), p, Ns[g],
+                                    graphs[g].sigma, L.ptr(dead[g]), L.ptr(score[g]), stream), 'proxy')
+            Br = min(B, int(live[g].sum()))
+            short = np.zeros(Br, dtype=np.int32)
+            if Br == 0:                       # candidates exhausted: this graph stops selecting
+                shorts.append(short)
+                continue
+            L.check(lib.polar_select_topb(L.ptr(score[g]), Ns[g], Br, L.ptr(sel_out), short.ctypes.data_as(L.vp),
+                                          L.ptr(sel_ws), sel_ws.numel(), stream), 'select_topb')
+            shorts.append(short)
+        Bm = max(len(sh) for sh in shorts)
+        if Bm == 0:
+            break
+        rows_a, rows_b, cols = [], [], []
+        for g in range(S):
+            c = cands[g][shorts[g]]
+            rows_a.append(c[:, 0] + row0[g])
+            rows_b.append(c[:, 1] + row0[g])
+            cols.append(np.arange(len(shorts[g])))
+        ia = torch.from_numpy(np.concatenate(rows_a)).to(dev)
+        ib = torch.from_numpy(np.concatenate(rows_b)).to(dev)
+        ic = torch.from_numpy(np.concatenate(cols)).to(dev)
+        Wb = torch.zeros((n, Bm), dtype=dtype, device=dev)
+        Wb.index_put_((ia, ic), torch.ones(len(ic), dtype=dtype, device=dev))
+        Wb.index_put_((ib, ic), torch.full((len(ic),), -1.0, dtype=dtype, device=dev))
+        Ub, hh, it = msolve_batched(op, segs, Wb, tau_all, active, tol, track=obj != 'logdet', graph=graph)[:3]
+        spmv += it.sum(axis=1)
+        for g in range(S):
+            Br = len(shorts[g])
+            if Br == 0:
+                continue
+            rhs[g] += Br
+            short = shorts[g]
+            Ug = Ub[:, a0[g]:]
+            sa = torch.from_numpy(es[g][short, 0].astype(np.int32)).to(dev)
+            sb = torch.from_numpy(es[g][short, 1].astype(np.int32)).to(dev)
+            hg = hh[:, g, :Br].contiguous() if hh is not None else None
+            win = np.zeros(1, dtype=np.int32)
+            val = np.zeros(1, dtype=np.float64)
+            L.check(lib.polar_rerank_pick(objc, dcode, L.ptr(Ug), Ug.stride(0), Ug.stride(1), p, Br, L.ptr(sa),
+                                          L.ptr(sb), L.ptr(hg), L.ptr(om_d[g]), L.ptr(gvec),
+                                          win.ctypes.data_as(L.vp), val.ctypes.data_as(L.vp), stream),
+                    'rerank_pick')
+            j = 0 if Br == 1 else int(win[0])
+            e = int(short[j])
+            u = Ug[:, :, j]
+            margs[g].append(float(val[0]))
+            seqs[g].append(e)
+            live[g][e] = False
+            dead[g][e] = 1
+            aa, bb = (int(x) for x in es[g][e])
+            a, b = (int(x) for x in cands[g][e])
+            Gc = Gcs[g]
+            if isinstance(Gc, torch.Tensor):
+                gc_row.copy_(Gc[e])
+            else:
+                gc_row.copy_(torch.from_numpy(np.ascontiguousarray(Gc[e], dtype=np.float64)))
+            if rnd + 1 < k:
+                Xg, Yg = X[:, a0[g]:], Y[:, a0[g]:]
+                nAg = int(a0[g + 1] - a0[g])
+                L.check(lib.polar_maint_prep(dcode, L.ptr(Xg), L.ptr(Yg), Xg.stride(0), Xg.stride(1), p, q, L.ptr(u),
+                                             u.stride(0), u.stride(1), aa, bb, L.ptr(gc_row), L.ptr(coef), stream),
+                        'maint_prep')
+                L.check(lib.polar_maint_rank1(dcode, L.ptr(Xg), L.ptr(Yg), Xg.stride(0), Xg.stride(1), p, nAg, q,
+                                              L.ptr(u), u.stride(0), u.stride(1), L.ptr(coef), L.ptr(wX), L.ptr(wY),
+                                              stream), 'maint_rank1')
+                L.check(lib.polar_maint_cache(dcode, L.ptr(u), u.stride(0), u.stride(1), p, nAg, Ns[g], L.ptr(ea[g]),
+                                              L.ptr(eb[g]), L.ptr(coef), q, L.ptr(wX), L.ptr(wY), L.ptr(hs[g]),
+                                              L.ptr(ts[g]), stream), 'maint_cache')
+                if want_trace:
+                    tr_hist[g].append(coef[:, 2 * q:2 * q + 2].clone())
+            Fv = RHS[:, q:]
+            L.check(lib.polar_rows_axpy(dcode, L.ptr(Fv), Fv.stride(0), q, a + int(row0[g]), b + int(row0[g]),
+                                        L.ptr(gc_row), stream), 'rows_axpy')
+            op.add(a + int(row0[g]), b + int(row0[g]))
+        if progress is not None:
+            progress('round', dict(round=rnd, edges=[sq[-1] if sq else None for sq in seqs], seconds=time.perf_counter() - t0,
+                                   spmv=int(spmv.sum())))
+    pg._sync()
+    seconds = time.perf_counter() - t0
+    out = []
+    for g in range(S):
+        tr = traces[g]
+        if want_trace:
+            for dn in tr_hist[g]:
+                dn_h = dn.cpu().numpy()
+                tr.append(tr[-1] - math.fsum(oms[g] * dn_h[:, 1] / dn_h[:, 0]))
+        out.append(dict(sequence=seqs[g], seconds=seconds, spmv=int(spmv[g]), rhs=int(rhs[g]), p=p,
+                        rounds=len(seqs[g]), truncated=len(seqs[g]) < k, marginals=margs[g],
+                        trace_estimate=tr[:max(len(seqs[g]), 1)],
+                        precision='fp64' if dtype == torch.float64 else 'fp32'))
+    return out

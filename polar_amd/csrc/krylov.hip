@@ -91,14 +91,18 @@ __device__ __forceinline__ void block_colsum(const Geo& g, const DD (&acc)[CPL],
   __syncthreads();
 }
 
-// Column total of nparts double-double partial rows (parts[b*2R + c] hi, parts[b*2R + R + c] lo)
-// by one workgroup: strided double-double accumulation, then a fixed-shape tree; every thread gets
+// Column total of the double-double partial rows b in [b0, b1) u [b2, b3) (parts[b*2R + c] hi,
+// parts[b*2R + R + c] lo) by one workgroup: strided double-double accumulation, then a fixed-shape tree; every thread gets
 // the rounded total.
-__device__ double col_total(const double* __restrict__ parts, int nparts, int R, int c) {
+__device__ double col_total(const double* __restrict__ parts, int b0, int b1, int b2, int b3, int R, int c) {
   __shared__ double sh_hi[kBlock];
   __shared__ double sh_lo[kBlock];
   DD acc{0.0, 0.0};
-  for (int b = threadIdx.x; b < nparts; b += kBlock) {
+  // partial rows [b0, b1) and [b2, b3)
+  const int n01 = b1 - b0;
+  const int nall = n01 + (b3 - b2);
+  for (int k = threadIdx.x; k < nall; k += kBlock) {
+    const int b = k < n01 ? b0 + k : b2 + (k - n01);
     const double* pb = parts + (int64_t)b * 2 * R;
     acc = dd_merge(acc, DD{pb[c], pb[R + c]});
   }
@@ -197,40 +201,113 @@ struct CGState {
   double* parts;       // [2*kMaxGrid][2][R] double-double workgroup partials
   int* colmap;         // [R] slot -> caller column (active-column compaction)
   int R, p;            // R: current (compacted) width
-  int ld;              // row stride of vectors, P and per-pole scalars (the caller's width)
+  int ld;              // row stride of vectors and P (the caller's width)
   int ident;           // colmap is the identity (no compaction yet)
   double tol;
   int freeze, track, trace;
+  // Segments (batched form): S independent systems on disjoint row ranges of one block-diagonal
+  // operator, each with its own poles and its own CG scalars for every column.  Per-column arrays
+  // hold S*ld entries (segment g, column c at g*ld + c); per-pole arrays are [p][sld], sld = S*ld.
+  // Blocks of every kernel are partitioned by segment (prefix tables over blockIdx), so partial
+  // sums and finishers stay within a segment.  S == 1 needs no tables.
+  int S, sld;
+  int nb_a;                   // opmul blocks (the long-row finisher's partials follow them)
+  const double* tau;          // [S][p] poles (tau[g*p] is segment g's seed shift)
+  int* sflag;                 // [2][S] any-active flags per segment, by iteration parity
+  const int* bA;              // [S+1] block prefix: k_opmul
+  const int* bL;              // [S+1] k_longfin
+  const int* bR;              // [S+1] row kernels with partials (k_init, k_resid)
+  const int* bD;              // [S+1] k_update direction part
+  const int* bS;              // [S+1] k_update shifted part
+  const int64_t* seg_row;     // [S+1] row ranges
+  const int64_t* seg_x;       // [S+1] long-row chunk unit ranges (entries of op.extra)
+  const int64_t* seg_l;       // [S+1] long-row index ranges
+  const int64_t* seg_a;       // [S+1] active-row ranges
 };
 
-// Finisher after d.v (one workgroup per column): a = act ? rr/dv : 0.
-__global__ __launch_bounds__(kBlock) void k_alpha(CGState s, int nparts, int cur) {
-  if (s.flag[cur] == 0) return;
-  const int c = blockIdx.x;
-  const double dv = col_total(s.parts, nparts, s.R, c);
-  if (threadIdx.x == 0) s.a[c] = s.act[c] ? s.rr[c] / dv : 0.0;
+// Segment of block b of a kernel whose blocks are partitioned by the prefix table `pref`.
+struct SegPart {
+  int g, rank, nb;
+};
+
+__device__ __forceinline__ SegPart seg_part(const int* __restrict__ pref, int S, int b, int nb_total) {
+  if (S <= 1) return SegPart{0, b, nb_total};
+  int lo = 0, hi = S - 1;     // largest g with pref[g] <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pref[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return SegPart{lo, b - pref[lo], pref[lo + 1] - pref[lo]};
 }
 
-// Finisher after |rn|^2 (one workgroup per column c, threads over poles): every scalar recurrence
-// of polar_gpu.py:106-125 for column c, freezing, convergence and the any-active flag.
+__device__ __forceinline__ void seg_range(const int64_t* __restrict__ tab, int S, int g, int64_t whole,
+                                          int64_t& lo, int64_t& hi) {
+  if (S <= 1) {
+    lo = 0;
+    hi = whole;
+  } else {
+    lo = tab[g];
+    hi = tab[g + 1];
+  }
+}
+
+// The segment is still iterating (the whole-solve flag covers S == 1).
+__device__ __forceinline__ bool seg_live(const CGState& s, int cur, int g) {
+  return s.S <= 1 || s.sflag[cur * s.S + g] != 0;
+}
+
+// Partial-row ranges of segment g: the row kernels' blocks, and the operator's blocks followed by
+// the long-row finisher's (for S == 1 the first nparts rows).
+__device__ __forceinline__ void rows_parts(const CGState& s, int g, int nparts, int& b0, int& b1) {
+  if (s.S <= 1) {
+    b0 = 0;
+    b1 = nparts;
+  } else {
+    b0 = s.bR[g];
+    b1 = s.bR[g + 1];
+  }
+}
+
+// Finisher after d.v (one workgroup per (segment, column)): a = act ? rr/dv : 0.
+__global__ __launch_bounds__(kBlock) void k_alpha(CGState s, int nparts, int cur) {
+  if (s.flag[cur] == 0) return;
+  const int g = blockIdx.x / s.R, c = blockIdx.x - g * s.R;
+  if (!seg_live(s, cur, g)) return;
+  const int vc = g * s.ld + c;
+  double dv;
+  if (s.S <= 1)
+    dv = col_total(s.parts, 0, nparts, 0, 0, s.R, c);
+  else
+    dv = col_total(s.parts, s.bA[g], s.bA[g + 1], s.nb_a + s.bL[g], s.nb_a + s.bL[g + 1], s.R, c);
+  if (threadIdx.x == 0) s.a[vc] = s.act[vc] ? s.rr[vc] / dv : 0.0;
+}
+
+// Finisher after |rn|^2 (one workgroup per (segment, column), threads over poles): every scalar
+// recurrence of polar_gpu.py:106-125 for the column, freezing, convergence and the any-active
+// flags.
 __global__ __launch_bounds__(kBlock) void k_scalars(CGState s, int nparts, int cur) {
   if (s.flag[cur] == 0) return;
-  const int c = blockIdx.x;
-  const double rrn = col_total(s.parts, nparts, s.R, c);
+  const int g = blockIdx.x / s.R, c = blockIdx.x - g * s.R;
+  if (!seg_live(s, cur, g)) return;
+  const int vc = g * s.ld + c;
+  int b0, b1;
+  rows_parts(s, g, nparts, b0, b1);
+  const double rrn = col_total(s.parts, b0, b1, 0, 0, s.R, c);
   __shared__ int nlive_s;
   if (threadIdx.x == 0) nlive_s = 0;
-  const bool act = s.act[c] != 0;
-  const double a = s.a[c];
-  const double be = act ? rrn / s.rr[c] : 0.0;
-  const double cc = act ? a * s.bp[c] / s.ap[c] : 0.0;
+  const bool act = s.act[vc] != 0;
+  const double a = s.a[vc];
+  const double be = act ? rrn / s.rr[vc] : 0.0;
+  const double cc = act ? a * s.bp[vc] / s.ap[vc] : 0.0;
   const double nr = sqrt(rrn);
-  const double stop = s.stop[c];
+  const double stop = s.stop[vc];
+  const double* shift = s.shift + g * s.p;
   __syncthreads();
   int nlive = 0;
   for (int j = threadIdx.x; j < s.p; j += kBlock) {
-    const int idx = j * s.ld + c;
+    const int idx = j * s.sld + vc;
     const double z = s.z[idx], zp = s.zp[idx];
-    const double zn = z / (1.0 + a * s.shift[j] + cc * (1.0 - z / zp));
+    const double zn = z / (1.0 + a * shift[j] + cc * (1.0 - z / zp));
     const double fac = zn / z;
     const bool m = (s.live[idx] != 0) && act;
     const double aj = m ? a * fac : 0.0;
@@ -262,29 +339,35 @@ __global__ __launch_bounds__(kBlock) void k_scalars(CGState s, int nparts, int c
   if (nlive) atomicAdd(&nlive_s, nlive);
   __syncthreads();
   if (threadIdx.x == 0) {
-    s.dmask[c] = act ? 1 : 0;
-    s.be[c] = be;
+    s.dmask[vc] = act ? 1 : 0;
+    s.be[vc] = be;
     bool still = false;
     if (act) {
-      s.iters[c] += 1;
+      s.iters[vc] += 1;
       const bool done = nr <= stop;
-      s.rr[c] = rrn;
-      s.ap[c] = a;
-      s.bp[c] = be;
-      if (done) s.act[c] = 0; else still = true;
+      s.rr[vc] = rrn;
+      s.ap[vc] = a;
+      s.bp[vc] = be;
+      if (done) s.act[vc] = 0; else still = true;
       atomicAdd(&s.stats[0], 1ull);
     }
     if (nlive_s) atomicAdd(&s.stats[1], (unsigned long long)nlive_s);
-    if (still) atomicOr(&s.flag[cur ^ 1], 1);
+    if (still) {
+      atomicOr(&s.flag[cur ^ 1], 1);
+      if (s.S > 1) atomicOr(&s.sflag[(cur ^ 1) * s.S + g], 1);
+    }
   }
 }
 
-// Finisher of the init pass: rr, stop, act and every per-pole scalar of column c.
+// Finisher of the init pass: rr, stop, act and every per-pole scalar of (segment, column).
 __global__ __launch_bounds__(kBlock) void k_init_fin(CGState s, int nparts) {
-  const int c = blockIdx.x;
-  const double rr = col_total(s.parts, nparts, s.R, c);
+  const int g = blockIdx.x / s.R, c = blockIdx.x - g * s.R;
+  const int vc = g * s.ld + c;
+  int b0, b1;
+  rows_parts(s, g, nparts, b0, b1);
+  const double rr = col_total(s.parts, b0, b1, 0, 0, s.R, c);
   for (int j = threadIdx.x; j < s.p; j += kBlock) {
-    const int idx = j * s.ld + c;
+    const int idx = j * s.sld + vc;
     s.z[idx] = 1.0;
     s.zp[idx] = 1.0;
     s.zn[idx] = 1.0;
@@ -299,18 +382,22 @@ __global__ __launch_bounds__(kBlock) void k_init_fin(CGState s, int nparts) {
     s.tg[idx] = rr;
   }
   if (threadIdx.x == 0) {
-    s.rr[c] = rr;
-    s.stop[c] = s.tol * sqrt(rr);
-    s.act[c] = rr > 0.0 ? 1 : 0;
-    s.dmask[c] = 0;
-    s.a[c] = 0.0;
-    s.be[c] = 0.0;
-    s.ap[c] = 1.0;
-    s.bp[c] = 0.0;
-    s.iters[c] = 0;
+    s.rr[vc] = rr;
+    s.stop[vc] = s.tol * sqrt(rr);
+    s.act[vc] = rr > 0.0 ? 1 : 0;
+    s.dmask[vc] = 0;
+    s.a[vc] = 0.0;
+    s.be[vc] = 0.0;
+    s.ap[vc] = 1.0;
+    s.bp[vc] = 0.0;
+    s.iters[vc] = 0;
     if (rr > 0.0) {
       atomicOr(&s.flag[0], 1);
       atomicOr(&s.flag[1], 1);
+      if (s.S > 1) {
+        atomicOr(&s.sflag[g], 1);
+        atomicOr(&s.sflag[s.S + g], 1);
+      }
     }
   }
 }
@@ -322,31 +409,36 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
                                                   int64_t ldx, T* __restrict__ Y, int64_t ldy, int R,
                                                   int L, T* __restrict__ lpart, CGState s, int cur) {
   if (DOT && s.flag[cur] == 0) return;
+  const SegPart sp = seg_part(s.bA, s.S, blockIdx.x, gridDim.x);
+  if (DOT && !seg_live(s, cur, sp.g)) return;
+  if (s.S > 1) shift = s.tau[sp.g * s.p];
+  int64_t r0, r1, x0, x1;
+  seg_range(s.seg_row, s.S, sp.g, op.n, r0, r1);
+  seg_range(s.seg_x, s.S, sp.g, op.n_extra, x0, x1);
   Geo g(L);
   const int c0 = g.li * CPL;
   const bool colok = c0 < R;
-  const int64_t n = op.n;
-  const int64_t nunits = n + op.n_extra;
+  const int64_t nx = x1 - x0;
+  const int64_t nunits = nx + (r1 - r0);
   const int gpb = kWaves * g.G;
-  int64_t u = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi;
-  const int64_t stride = (int64_t)gridDim.x * gpb;
-  const int64_t nx = op.n_extra;
+  int64_t u = (int64_t)sp.rank * gpb + g.wave * g.G + g.gi;
+  const int64_t stride = (int64_t)sp.nb * gpb;
   const int CH = op.chunk;
   DD dacc[CPL];
 #pragma unroll
   for (int e = 0; e < CPL; ++e) dacc[e] = DD{0.0, 0.0};
   int pc = -1;
-  // units [0, n_extra): every chunk of every long row (scheduled first: their latency chains
-  // overlap the rest); units [n_extra, n_extra + n): the remaining rows
+  // the segment's units [0, nx): every chunk of its long rows (scheduled first: their latency
+  // chains overlap the rest); units [nx, nx + rows): the remaining rows
   for (; u < nunits; u += stride) {
     int row, beg, end, lidx = -1, chunk = 0;
     if (u >= nx) {
-      row = (int)(u - nx);
+      row = (int)(r0 + (u - nx));
       beg = op.indptr[row];
       end = op.indptr[row + 1];
       if (end - beg > CH) continue;    // long row: all its chunks are units of the chunk list
     } else {
-      const int2 ex = reinterpret_cast<const int2*>(op.extra)[u];
+      const int2 ex = reinterpret_cast<const int2*>(op.extra)[x0 + u];
       lidx = ex.x;
       chunk = ex.y;
       row = op.long_rows[lidx];
@@ -390,6 +482,11 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
                                                     int L, const T* __restrict__ lpart, CGState s,
                                                     int cur, int parts_off) {
   if (DOT && s.flag[cur] == 0) return;
+  const SegPart sp = seg_part(s.bL, s.S, blockIdx.x, gridDim.x);
+  if (DOT && !seg_live(s, cur, sp.g)) return;
+  if (s.S > 1) shift = s.tau[sp.g * s.p];
+  int64_t l0, l1;
+  seg_range(s.seg_l, s.S, sp.g, op.n_long, l0, l1);
   Geo g(L);
   const int c0 = g.li * CPL;
   const bool colok = c0 < R;
@@ -397,8 +494,7 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
   DD dacc[CPL];
 #pragma unroll
   for (int e = 0; e < CPL; ++e) dacc[e] = DD{0.0, 0.0};
-  for (int64_t l = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi; l < op.n_long;
-       l += (int64_t)gridDim.x * gpb) {
+  for (int64_t l = l0 + (int64_t)sp.rank * gpb + g.wave * g.G + g.gi; l < l1; l += (int64_t)sp.nb * gpb) {
     const int row = op.long_rows[l];
     const int deg = op.indptr[row + 1] - op.indptr[row];
     const int nch = (deg + op.chunk - 1) / op.chunk;
@@ -445,8 +541,17 @@ __global__ __launch_bounds__(kBlock) void k_resid(int64_t n, int R, int L, T* __
                                                   const T* __restrict__ v, CGState s, int cur) {
   // the finisher of this iteration raises flag[cur^1] again if a column is still active; when
   // converged, both parities stay 0 so every later launch is a no-op
-  if (blockIdx.x == 0 && threadIdx.x == 0) s.flag[cur ^ 1] = 0;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) s.flag[cur ^ 1] = 0;
+    if (s.S > 1)
+      for (int t = threadIdx.x; t < s.S; t += kBlock) s.sflag[(cur ^ 1) * s.S + t] = 0;
+  }
   if (s.flag[cur] == 0) return;
+  const SegPart sp = seg_part(s.bR, s.S, blockIdx.x, gridDim.x);
+  if (!seg_live(s, cur, sp.g)) return;
+  int64_t r0, r1;
+  seg_range(s.seg_row, s.S, sp.g, n, r0, r1);
+  const int vb = sp.g * s.ld;
   Geo g(L);
   const int c0 = g.li * CPL;
   const bool colok = c0 < R;
@@ -459,12 +564,12 @@ __global__ __launch_bounds__(kBlock) void k_resid(int64_t n, int R, int L, T* __
     bool any = false;
 #pragma unroll
     for (int e = 0; e < CPL; ++e) {
-      act[e] = s.act[c0 + e] != 0;
-      a[e] = (T)s.a[c0 + e];
+      act[e] = s.act[vb + c0 + e] != 0;
+      a[e] = (T)s.a[vb + c0 + e];
       any |= act[e];
     }
     const int gpb = kWaves * g.G;
-    for (int64_t i = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi; i < n; i += (int64_t)gridDim.x * gpb) {
+    for (int64_t i = r0 + (int64_t)sp.rank * gpb + g.wave * g.G + g.gi; i < r1; i += (int64_t)sp.nb * gpb) {
       T rv[CPL], vv[CPL];
       vload<T, CPL>(r + i * s.ld + c0, rv);
       if (any) {
@@ -494,18 +599,23 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
   const int c0 = g.li * CPL;
   const int gpb = kWaves * g.G;
   if ((int)blockIdx.x < nb_d) {
+    const SegPart sp = seg_part(s.bD, s.S, blockIdx.x, nb_d);
+    if (!seg_live(s, cur, sp.g)) return;
     if (c0 >= R) return;     // no barrier in this part
+    int64_t r0, r1;
+    seg_range(s.seg_row, s.S, sp.g, n, r0, r1);
+    const int vb = sp.g * s.ld;
     bool msk[CPL];
     T be[CPL];
     bool any = false;
 #pragma unroll
     for (int e = 0; e < CPL; ++e) {
-      msk[e] = s.dmask[c0 + e] != 0;
-      be[e] = (T)s.be[c0 + e];
+      msk[e] = s.dmask[vb + c0 + e] != 0;
+      be[e] = (T)s.be[vb + c0 + e];
       any |= msk[e];
     }
     if (!any) return;
-    for (int64_t i = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi; i < n; i += (int64_t)nb_d * gpb) {
+    for (int64_t i = r0 + (int64_t)sp.rank * gpb + g.wave * g.G + g.gi; i < r1; i += (int64_t)sp.nb * gpb) {
       T rv[CPL], dv[CPL];
       vload<T, CPL>(r + i * s.ld + c0, rv);
       vload<T, CPL>(d + i * s.ld + c0, dv);
@@ -516,7 +626,11 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
     }
     return;
   }
-  const int nb_s = gridDim.x - nb_d;
+  const SegPart sp = seg_part(s.bS, s.S, blockIdx.x - nb_d, gridDim.x - nb_d);
+  if (!seg_live(s, cur, sp.g)) return;     // uniform over the block, before any barrier
+  int64_t a0, a1;
+  seg_range(s.seg_a, s.S, sp.g, nA, a0, a1);
+  const int vb = sp.g * s.ld;
   // Stage this block's view of the per-pole scalars in LDS and list the poles that are live in
   // some column (frozen poles are skipped entirely); falls back to global loads when p*R is large.
   constexpr int kStage = STAGED ? 1024 : 1;
@@ -530,7 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
     __syncthreads();
     for (int q = threadIdx.x; q < p * R; q += kBlock) {
       const int j = q / R, c = q - j * R;
-      const int src = j * s.ld + c;
+      const int src = j * s.sld + vb + c;
       s_aj[q] = s.aj[src];
       s_bj[q] = s.bj[src];
       s_zn[q] = s.zn[src];
@@ -545,8 +659,7 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
     __syncthreads();
   }
   const int nj = staged ? s_nj : p;
-  for (int64_t i = (int64_t)(blockIdx.x - nb_d) * gpb + g.wave * g.G + g.gi; i < nA;
-       i += (int64_t)nb_s * gpb) {
+  for (int64_t i = a0 + (int64_t)sp.rank * gpb + g.wave * g.G + g.gi; i < a1; i += (int64_t)sp.nb * gpb) {
     if (c0 >= R || nj == 0) break;
     T rn[CPL];
     vload<T, CPL>(r + (int64_t)active[i] * s.ld + c0, rn);
@@ -564,7 +677,7 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
           bj[e] = (T)s_bj[q];
           zn[e] = (T)s_zn[q];
         } else {
-          const int q = j * s.ld + c0 + e;
+          const int q = j * s.sld + vb + c0 + e;
           mj[e] = s.m[q] != 0;
           aj[e] = (T)s.aj[q];
           bj[e] = (T)s.bj[q];
@@ -607,6 +720,9 @@ template <typename T, int CPL>
 __global__ __launch_bounds__(kBlock) void k_init(int64_t n, int R, int L, const T* __restrict__ W,
                                                  int64_t ldw, T* __restrict__ r, T* __restrict__ d,
                                                  CGState s) {
+  const SegPart sp = seg_part(s.bR, s.S, blockIdx.x, gridDim.x);
+  int64_t r0, r1;
+  seg_range(s.seg_row, s.S, sp.g, n, r0, r1);
   Geo g(L);
   const int c0 = g.li * CPL;
   const bool colok = c0 < R;
@@ -615,7 +731,7 @@ __global__ __launch_bounds__(kBlock) void k_init(int64_t n, int R, int L, const 
   for (int e = 0; e < CPL; ++e) dacc[e] = DD{0.0, 0.0};
   if (colok) {
     const int gpb = kWaves * g.G;
-    for (int64_t i = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi; i < n; i += (int64_t)gridDim.x * gpb) {
+    for (int64_t i = r0 + (int64_t)sp.rank * gpb + g.wave * g.G + g.gi; i < r1; i += (int64_t)sp.nb * gpb) {
       T wv[CPL];
       vload<T, CPL>(W + i * ldw + c0, wv);
       vstore<T, CPL>(r + i * s.ld + c0, wv);
@@ -650,10 +766,10 @@ __global__ __launch_bounds__(kBlock) void k_init_shift(int64_t nA, int p, int R,
   }
 }
 
-// shift[j] = tau[j] - tau[0]
-__global__ void k_shift(const double* __restrict__ tau, int p, double* __restrict__ shift) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < p) shift[j] = tau[j] - tau[0];
+// shift[g*p + j] = tau[g*p + j] - tau[g*p] (every segment's poles relative to its seed shift)
+__global__ void k_shift(const double* __restrict__ tau, int p, int S, double* __restrict__ shift) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < p * S) shift[i] = tau[i] - tau[(i / p) * p];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -672,27 +788,94 @@ static int g_prof_on = 0;
 
 struct Plan {
   int cpl, L, G;
-  int grid_a, grid_l, grid_rows;
+  int grid_a, grid_l, grid_rows;   // blocks of the kernels that write partial sums
+  int nb_d, nb_s;                  // k_update: direction part, shifted part
+  int fin;                         // finisher grid: one workgroup per (segment, column)
+  std::vector<int> tab;            // S > 1: prefix tables bA, bL, bR, bD, bS (S+1 entries each)
 };
+
+// Segment ranges on the host (S == 1: the whole operator).
+struct Segs {
+  int S = 1;
+  std::vector<int64_t> row, x, l, a;   // [S+1]: rows, long-row chunk units, long rows, active rows
+};
+
+constexpr int kBatchGrid = 4096;   // default block budget of a batched kernel with partial sums
+constexpr int kUpdGrid = 65536;    // block budget of the stream kernels without partial sums
 
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
-static Plan make_plan(const polar_op* op, int64_t R, int cpl) {
+// Blocks of one kernel over the segments: units[g] work units of segment g (0: none or finished),
+// gpb units per block and pass, about `cap` blocks in total (at least one per segment with work).
+// pref[g]..pref[g+1] are segment g's blocks.
+static void partition(const std::vector<int64_t>& units, int64_t gpb, int64_t cap, int* pref) {
+  int64_t tot = 0;
+  for (int64_t u : units) tot += u;
+  const int64_t upg = std::max<int64_t>(1, (tot + gpb * cap - 1) / (gpb * cap));
+  pref[0] = 0;
+  for (size_t g = 0; g < units.size(); ++g) {
+    const int64_t b = units[g] > 0 ? (units[g] + gpb * upg - 1) / (gpb * upg) : 0;
+    pref[g + 1] = pref[g] + (int)b;
+  }
+}
+
+static int batch_cap(const polar_op* op) { return op->grid_max > 0 ? op->grid_max : kBatchGrid; }
+
+static Plan make_plan(const polar_op* op, int64_t R, int cpl, const Segs* sg = nullptr,
+                      const std::vector<char>* live = nullptr) {
   Plan pl;
   pl.cpl = cpl;
   pl.L = lanes_for(R, cpl);
   pl.G = 64 / pl.L;
   const int64_t gpb = (int64_t)kWaves * pl.G;
-  const int64_t units = op->n + op->n_extra;
-  pl.grid_a = (int)std::min<int64_t>(kMaxGrid, std::max<int64_t>(1, (units + gpb - 1) / gpb));
-  pl.grid_l = (int)std::min<int64_t>(kMaxGrid, std::max<int64_t>(1, ((int64_t)op->n_long + gpb - 1) / gpb));
-  pl.grid_rows = (int)std::min<int64_t>(kMaxGrid, std::max<int64_t>(1, (op->n + gpb - 1) / gpb));
+  const int S = sg ? sg->S : 1;
+  pl.fin = (int)(S * R);
+  if (S == 1) {
+    const int64_t units = op->n + op->n_extra;
+    const int64_t nA = sg ? sg->a[1] : 0;
+    const int64_t cap = op->grid_max > 0 ? op->grid_max : kMaxGrid;
+    pl.grid_a = (int)std::min<int64_t>(cap, std::max<int64_t>(1, (units + gpb - 1) / gpb));
+    pl.grid_l = (int)std::min<int64_t>(cap, std::max<int64_t>(1, ((int64_t)op->n_long + gpb - 1) / gpb));
+    pl.grid_rows = (int)std::min<int64_t>(cap, std::max<int64_t>(1, (op->n + gpb - 1) / gpb));
+    pl.nb_d = (int)std::min<int64_t>(kUpdGrid, std::max<int64_t>(1, (op->n + gpb - 1) / gpb));
+    pl.nb_s = (int)std::min<int64_t>(kUpdGrid, std::max<int64_t>(1, (nA + gpb - 1) / gpb));
+    return pl;
+  }
+  const int64_t cap = batch_cap(op);
+  std::vector<int64_t> ua(S), ul(S), ur(S), us(S);
+  for (int g = 0; g < S; ++g) {
+    const bool on = !live || (*live)[g];
+    ur[g] = on ? sg->row[g + 1] - sg->row[g] : 0;
+    ua[g] = on ? ur[g] + (sg->x[g + 1] - sg->x[g]) : 0;
+    ul[g] = on ? sg->l[g + 1] - sg->l[g] : 0;
+    us[g] = on ? sg->a[g + 1] - sg->a[g] : 0;
+  }
+  pl.tab.assign(5 * (S + 1), 0);
+  int* t = pl.tab.data();
+  partition(ua, gpb, cap, t);
+  partition(ul, gpb, cap, t + (S + 1));
+  partition(ur, gpb, cap, t + 2 * (S + 1));
+  partition(ur, gpb, kUpdGrid, t + 3 * (S + 1));
+  partition(us, gpb, kUpdGrid, t + 4 * (S + 1));
+  pl.grid_a = t[S];
+  pl.grid_l = t[2 * (S + 1) - 1];
+  pl.grid_rows = t[3 * (S + 1) - 1];
+  pl.nb_d = t[4 * (S + 1) - 1];
+  pl.nb_s = t[5 * (S + 1) - 1];
   return pl;
 }
 
-static int pick_cpl(int dtype, int64_t R, std::initializer_list<std::pair<const void*, int64_t>> ptrs) {
+// Partial-sum rows the workspace holds for any plan of this operator and segmentation.
+static int64_t parts_rows(const polar_op* op, int S) {
+  if (S == 1) return 2 * (int64_t)std::max(kMaxGrid, op->grid_max);
+  return 2 * ((int64_t)batch_cap(op) + S);
+}
+
+static int pick_cpl(int dtype, int64_t R, std::initializer_list<std::pair<const void*, int64_t>> ptrs,
+                    int cpl_max = 0) {
   const int esz = elem_size(dtype);
-  const int full = 16 / esz;
+  int full = 16 / esz;
+  if (cpl_max > 0) full = std::min(full, cpl_max);
   for (int cpl = full; cpl > 1; cpl /= 2) {
     bool ok = (R % cpl) == 0 && (R + cpl - 1) / cpl <= 64;
     for (auto& pr : ptrs) {
@@ -714,6 +897,9 @@ static int check_op(const polar_op* op) {
     return set_error(POLAR_E_ARG, "patch arrays missing");
   if (op->n > INT32_MAX) return set_error(POLAR_E_ARG, "n exceeds int32 row ids");
   if (op->chunk < 8) return set_error(POLAR_E_ARG, "long-row chunk length must be >= 8");
+  if (op->cpl_max < 0 || op->cpl_max > 4 || (op->cpl_max & (op->cpl_max - 1)))
+    return set_error(POLAR_E_ARG, "cpl_max must be 0, 1, 2 or 4");
+  if (op->grid_max < 0 || op->grid_max > 65536) return set_error(POLAR_E_ARG, "grid_max out of range");
   return 0;
 }
 
@@ -730,20 +916,46 @@ struct Carver {
   }
 };
 
+// Device state of one (batched) solve: S segments of R columns, p poles, nA active rows in total.
 template <typename T>
-static void carve_state(Carver& cv, CGState& s, int64_t R, int p) {
-  s.rr = cv.take<double>(R); s.stop = cv.take<double>(R); s.a = cv.take<double>(R);
-  s.be = cv.take<double>(R); s.ap = cv.take<double>(R); s.bp = cv.take<double>(R);
-  s.act = cv.take<int>(R); s.dmask = cv.take<int>(R); s.iters = cv.take<int64_t>(R);
-  const int64_t pr = (int64_t)p * R;
+struct SolveMem {
+  CGState s{};
+  int* perm;
+  T *r, *d, *v, *t2, *P, *lpart;
+  int* dtab;           // 5 (S+1) block prefix tables
+  int64_t* dseg;       // 4 (S+1) segment ranges
+};
+
+template <typename T>
+static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int p, int64_t nA, SolveMem<T>& m) {
+  Carver cv(base);
+  CGState& s = m.s;
+  const int64_t C = (int64_t)S * R;
+  s.rr = cv.take<double>(C); s.stop = cv.take<double>(C); s.a = cv.take<double>(C);
+  s.be = cv.take<double>(C); s.ap = cv.take<double>(C); s.bp = cv.take<double>(C);
+  s.act = cv.take<int>(C); s.dmask = cv.take<int>(C); s.iters = cv.take<int64_t>(C);
+  const int64_t pr = (int64_t)p * C;
   s.z = cv.take<double>(pr); s.zp = cv.take<double>(pr); s.zn = cv.take<double>(pr);
   s.aj = cv.take<double>(pr); s.bj = cv.take<double>(pr); s.h = cv.take<double>(pr);
   s.ip = cv.take<double>(pr); s.pn = cv.take<double>(pr); s.te = cv.take<double>(pr);
-  s.tg = cv.take<double>(pr); s.shift = cv.take<double>(p);
+  s.tg = cv.take<double>(pr); s.shift = cv.take<double>((int64_t)p * S);
   s.live = cv.take<int>(pr); s.m = cv.take<int>(pr);
   s.flag = cv.take<int>(2);
   s.stats = cv.take<unsigned long long>(2);
-  s.parts = cv.take<double>((int64_t)4 * kMaxGrid * R);
+  s.parts = cv.take<double>(parts_rows(op, S) * 2 * R);
+  s.colmap = cv.take<int>(R);
+  s.sflag = cv.take<int>(2 * S);
+  m.dtab = cv.take<int>(5 * (S + 1));
+  m.dseg = cv.take<int64_t>(4 * (S + 1));
+  m.perm = cv.take<int>(R);
+  const int64_t n = op->n;
+  m.r = cv.take<T>(n * R);
+  m.d = cv.take<T>(n * R);
+  m.v = cv.take<T>(n * R);
+  m.t2 = cv.take<T>(n * R);
+  m.P = cv.take<T>((int64_t)p * nA * R);
+  m.lpart = cv.take<T>((int64_t)op->n_partial * R);
+  return cv.off + 256;
 }
 
 template <typename T, int CPL>
@@ -761,7 +973,7 @@ static int launch_opmul(const polar_op* op, const Plan& pl, double shift, const 
       hipLaunchKernelGGL((k_opmul<T, CPL, false, false>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X, ldx, Y, ldy, R, pl.L, lpart, s, cur);
   }
   POLAR_LAUNCH_CHECK();
-  if (op->n_long > 0) {
+  if (op->n_long > 0 && pl.grid_l > 0) {
     if (dot)
       hipLaunchKernelGGL((k_longfin<T, CPL, true>), dim3(pl.grid_l), dim3(kBlock), 0, st, *op, shift, X, ldx, Y, ldy, R, pl.L, (const T*)lpart, s, cur, pl.grid_a);
     else
@@ -780,6 +992,7 @@ static int spmm_impl(const polar_op* op, double shift, const T* X, int64_t ldx, 
   T* lpart = cv.take<T>((int64_t)op->n_partial * R);
   if (cv.off > ws_bytes) return set_error(POLAR_E_ARG, "spmm workspace too small (%lld < %lld)", (long long)ws_bytes, (long long)cv.off);
   CGState s{};
+  s.S = 1;
   return launch_opmul<T, CPL>(op, pl, shift, X, ldx, Y, ldy, (int)R, lpart, s, 0, false, st);
 }
 
@@ -818,15 +1031,20 @@ __global__ __launch_bounds__(kBlock) void k_compact_vec(int64_t n, int64_t ld, i
   }
 }
 
-// Write the per-column outputs of the current slots back to the caller's column order.
+// Write the per-column outputs of the current slots back to the caller's column order:
+// h_out/te_out [p][S][Rorig], iters_out [S][Rorig].
 __global__ void k_writeback(CGState s, int Rorig, double* __restrict__ h_out, double* __restrict__ te_out,
                             int64_t* __restrict__ iters_out) {
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < s.p * s.R; idx += gridDim.x * blockDim.x) {
-    const int c = idx % s.R, j = idx / s.R;
+  const int SR = s.S * s.R;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < s.p * SR; idx += gridDim.x * blockDim.x) {
+    const int j = idx / SR, rem = idx - j * SR;
+    const int g = rem / s.R, c = rem - g * s.R;
     const int col = s.colmap[c];
-    if (h_out) h_out[(int64_t)j * Rorig + col] = s.h[j * s.ld + c];
-    if (te_out) te_out[(int64_t)j * Rorig + col] = s.te[j * s.ld + c];
-    if (j == 0 && iters_out) iters_out[col] = s.iters[c];
+    const int vc = g * s.ld + c;
+    const int64_t o = ((int64_t)j * s.S + g) * Rorig + col;
+    if (h_out) h_out[o] = s.h[j * s.sld + vc];
+    if (te_out) te_out[o] = s.te[j * s.sld + vc];
+    if (j == 0 && iters_out) iters_out[(int64_t)g * Rorig + col] = s.iters[vc];
   }
 }
 
@@ -840,9 +1058,7 @@ static int run_iter(const polar_op* op, const Plan& pl, double tau0, T* r, T* d,
                     hipEvent_t* ev, hipStream_t st) {
   const int64_t n = op->n;
   const int R = s.R;
-  const int64_t gpb = (int64_t)kWaves * pl.G;
-  const int nb_d = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (n + gpb - 1) / gpb));
-  const int nb_s = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (nA + gpb - 1) / gpb));
+  const bool longs = op->n_long > 0 && pl.grid_l > 0;
   if (ev) POLAR_HIP(hipEventRecord(ev[0], st));
   if (op->weighted)
     hipLaunchKernelGGL((k_opmul<T, CPL, true, true>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, tau0, (const T*)d, (int64_t)s.ld, v, (int64_t)s.ld, R, pl.L, lpart, s, cur);
@@ -850,21 +1066,22 @@ static int run_iter(const polar_op* op, const Plan& pl, double tau0, T* r, T* d,
     hipLaunchKernelGGL((k_opmul<T, CPL, false, true>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, tau0, (const T*)d, (int64_t)s.ld, v, (int64_t)s.ld, R, pl.L, lpart, s, cur);
   POLAR_LAUNCH_CHECK();
   if (ev) POLAR_HIP(hipEventRecord(ev[1], st));
-  if (op->n_long > 0) {
+  if (longs) {
     hipLaunchKernelGGL((k_longfin<T, CPL, true>), dim3(pl.grid_l), dim3(kBlock), 0, st, *op, tau0, (const T*)d, (int64_t)s.ld, v, (int64_t)s.ld, R, pl.L, (const T*)lpart, s, cur, pl.grid_a);
     POLAR_LAUNCH_CHECK();
   }
   if (ev) POLAR_HIP(hipEventRecord(ev[2], st));
-  const int nparts_a = pl.grid_a + (op->n_long > 0 ? pl.grid_l : 0);
-  hipLaunchKernelGGL(k_alpha, dim3(R), dim3(kBlock), 0, st, s, nparts_a, cur);
+  const int nparts_a = pl.grid_a + (longs ? pl.grid_l : 0);
+  hipLaunchKernelGGL(k_alpha, dim3(pl.fin), dim3(kBlock), 0, st, s, nparts_a, cur);
   POLAR_LAUNCH_CHECK();
   if (ev) POLAR_HIP(hipEventRecord(ev[3], st));
   hipLaunchKernelGGL((k_resid<T, CPL>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, R, pl.L, r, (const T*)v, s, cur);
   POLAR_LAUNCH_CHECK();
   if (ev) POLAR_HIP(hipEventRecord(ev[4], st));
-  hipLaunchKernelGGL(k_scalars, dim3(R), dim3(kBlock), 0, st, s, pl.grid_rows, cur);
+  hipLaunchKernelGGL(k_scalars, dim3(pl.fin), dim3(kBlock), 0, st, s, pl.grid_rows, cur);
   POLAR_LAUNCH_CHECK();
   if (ev) POLAR_HIP(hipEventRecord(ev[5], st));
+  const int nb_d = pl.nb_d, nb_s = pl.nb_s;
   if (p * R <= 1024)
     hipLaunchKernelGGL((k_update<T, CPL, true>), dim3(nb_d + nb_s), dim3(kBlock), 0, st, n, nA, p, R, pl.L,
                        (const T*)r, d, active, P, Xo, sj, si, nb_d, s, cur);
@@ -903,24 +1120,51 @@ static int cpl_for(int64_t Rc, int64_t ld, int cpl_max) {
   return 1;
 }
 
+// Stream of a thread's captures (default priority; captured work runs on the caller's stream).
+static hipStream_t capture_stream(int& rc) {
+  static thread_local hipStream_t cs = nullptr;
+  static thread_local int dev = -1;
+  int cur = 0;
+  rc = (int)hipGetDevice(&cur);
+  if (rc) return nullptr;
+  if (!cs || dev != cur) {
+    hipStream_t s = nullptr;
+    rc = (int)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (rc) return nullptr;
+    cs = s;
+    dev = cur;
+  }
+  return cs;
+}
+
+// An instantiated hipGraph of 8 CG iterations (parities 0,1,...,1), rebuilt when the plan, the
+// width or the vector buffers change.
+struct IterGraph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  void drop() {
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+    x = nullptr;
+    g = nullptr;
+  }
+  ~IterGraph() { drop(); }
+};
+
 template <typename T>
-static int msolve_impl(int cpl0, const polar_op* op, const T* W, int64_t ldw, int64_t R, const double* tau, int p,
-                       const int32_t* active, int64_t nA, double tol, int64_t maxit, int flags, T* Xo,
-                       int64_t sj, int64_t si, double* h_out, double* te_out, int64_t* iters_out,
+static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W, int64_t ldw, int64_t R,
+                       const double* tau, int p, const int32_t* active, double tol, int64_t maxit, int flags,
+                       T* Xo, int64_t sj, int64_t si, double* h_out, double* te_out, int64_t* iters_out,
                        void* ws, int64_t ws_bytes, hipStream_t st) {
   const int64_t n = op->n;
-  Carver cv(ws);
-  CGState s{};
-  carve_state<T>(cv, s, R, p);
-  s.colmap = cv.take<int>(R);
-  int* perm = cv.take<int>(R);
-  T* r = cv.take<T>(n * R);
-  T* d = cv.take<T>(n * R);
-  T* v = cv.take<T>(n * R);
-  T* t2 = cv.take<T>(n * R);
-  T* P = cv.take<T>((int64_t)p * nA * R);
-  T* lpart = cv.take<T>((int64_t)op->n_partial * R);
-  if (cv.off > ws_bytes) return set_error(POLAR_E_ARG, "msolve workspace too small (%lld < %lld)", (long long)ws_bytes, (long long)cv.off);
+  const int S = sg.S;
+  const int64_t nA = sg.a[S];
+  SolveMem<T> m;
+  const int64_t need = carve_solve<T>(ws, op, S, R, p, nA, m);
+  if (need > ws_bytes) return set_error(POLAR_E_ARG, "msolve workspace too small (%lld < %lld)", (long long)ws_bytes, (long long)need);
+  CGState s = m.s;
+  T *r = m.r, *d = m.d, *v = m.v, *t2 = m.t2, *P = m.P, *lpart = m.lpart;
+  int* perm = m.perm;
   s.R = (int)R;
   s.ld = (int)R;
   s.ident = 1;
@@ -929,15 +1173,41 @@ static int msolve_impl(int cpl0, const polar_op* op, const T* W, int64_t ldw, in
   s.freeze = (flags & POLAR_NOFREEZE) ? 0 : 1;
   s.track = (flags & POLAR_TRACK) ? 1 : 0;
   s.trace = (flags & POLAR_TRACE) ? 1 : 0;
+  s.S = S;
+  s.sld = (int)(S * R);
+  s.tau = tau;
   int cpl = cpl0;
-  Plan pl = make_plan(op, R, cpl);
+  std::vector<char> live(S, 1);
+  Plan pl = make_plan(op, R, cpl, &sg, &live);
+  s.nb_a = pl.grid_a;
+  if (S > 1) {
+    std::vector<int64_t> segh(4 * (S + 1));
+    for (int g = 0; g <= S; ++g) {
+      segh[g] = sg.row[g];
+      segh[(S + 1) + g] = sg.x[g];
+      segh[2 * (S + 1) + g] = sg.l[g];
+      segh[3 * (S + 1) + g] = sg.a[g];
+    }
+    POLAR_HIP(hipMemcpyAsync(m.dseg, segh.data(), sizeof(int64_t) * segh.size(), hipMemcpyHostToDevice, st));
+    POLAR_HIP(hipMemcpyAsync(m.dtab, pl.tab.data(), sizeof(int) * pl.tab.size(), hipMemcpyHostToDevice, st));
+    s.seg_row = m.dseg;
+    s.seg_x = m.dseg + (S + 1);
+    s.seg_l = m.dseg + 2 * (S + 1);
+    s.seg_a = m.dseg + 3 * (S + 1);
+    s.bA = m.dtab;
+    s.bL = m.dtab + (S + 1);
+    s.bR = m.dtab + 2 * (S + 1);
+    s.bD = m.dtab + 3 * (S + 1);
+    s.bS = m.dtab + 4 * (S + 1);
+    POLAR_HIP(hipMemsetAsync(s.sflag, 0, 2 * S * sizeof(int), st));
+  }
   const int64_t gpb0 = (int64_t)kWaves * pl.G;
 
   POLAR_HIP(hipMemsetAsync(s.flag, 0, 2 * sizeof(int), st));
   POLAR_HIP(hipMemsetAsync(s.stats, 0, 2 * sizeof(unsigned long long), st));
   hipLaunchKernelGGL(k_iota, dim3(1), dim3(256), 0, st, s.colmap, (int)R);
   POLAR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_shift, dim3((p + 255) / 256), dim3(256), 0, st, tau, p, s.shift);
+  hipLaunchKernelGGL(k_shift, dim3((p * S + 255) / 256), dim3(256), 0, st, tau, p, S, s.shift);
   POLAR_LAUNCH_CHECK();
   if (cpl == 2) {
     hipLaunchKernelGGL((k_init<T, 2>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, pl.L, W, ldw, r, d, s);
@@ -947,52 +1217,85 @@ static int msolve_impl(int cpl0, const polar_op* op, const T* W, int64_t ldw, in
     hipLaunchKernelGGL((k_init<T, 1>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, pl.L, W, ldw, r, d, s);
   }
   POLAR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_init_fin, dim3(R), dim3(kBlock), 0, st, s, pl.grid_rows);
+  hipLaunchKernelGGL(k_init_fin, dim3(pl.fin), dim3(kBlock), 0, st, s, pl.grid_rows);
   POLAR_LAUNCH_CHECK();
-  const int grid_sh = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (nA + gpb0 - 1) / gpb0));
-  if (cpl == 2) {
-    hipLaunchKernelGGL((k_init_shift<T, 2>), dim3(grid_sh), dim3(kBlock), 0, st, nA, p, (int)R, pl.L, W, ldw, active, P, Xo, sj, si);
-  } else if (cpl == 4) {
-    if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_init_shift<T, 4>), dim3(grid_sh), dim3(kBlock), 0, st, nA, p, (int)R, pl.L, W, ldw, active, P, Xo, sj, si);
-  } else {
-    hipLaunchKernelGGL((k_init_shift<T, 1>), dim3(grid_sh), dim3(kBlock), 0, st, nA, p, (int)R, pl.L, W, ldw, active, P, Xo, sj, si);
+  if (nA > 0) {
+    const int grid_sh = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (nA + gpb0 - 1) / gpb0));
+    if (cpl == 2) {
+      hipLaunchKernelGGL((k_init_shift<T, 2>), dim3(grid_sh), dim3(kBlock), 0, st, nA, p, (int)R, pl.L, W, ldw, active, P, Xo, sj, si);
+    } else if (cpl == 4) {
+      if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_init_shift<T, 4>), dim3(grid_sh), dim3(kBlock), 0, st, nA, p, (int)R, pl.L, W, ldw, active, P, Xo, sj, si);
+    } else {
+      hipLaunchKernelGGL((k_init_shift<T, 1>), dim3(grid_sh), dim3(kBlock), 0, st, nA, p, (int)R, pl.L, W, ldw, active, P, Xo, sj, si);
+    }
+    POLAR_LAUNCH_CHECK();
   }
-  POLAR_LAUNCH_CHECK();
 
-  double tau0 = 0.0;
-  POLAR_HIP(hipMemcpyAsync(&tau0, tau, sizeof(double), hipMemcpyDeviceToHost, st));
-  POLAR_HIP(hipStreamSynchronize(st));
+  double tau0 = 0.0;          // S == 1: the seed shift is a kernel argument (segments read tau)
+  if (S == 1) {
+    POLAR_HIP(hipMemcpyAsync(&tau0, tau, sizeof(double), hipMemcpyDeviceToHost, st));
+    POLAR_HIP(hipStreamSynchronize(st));
+  }
 
   const bool prof = g_prof_on != 0;
+  const bool use_graph = (flags & POLAR_GRAPH) && !prof;
+  IterGraph graph;
+  hipStream_t cs = nullptr;
+  if (use_graph) {
+    int rc = 0;
+    cs = capture_stream(rc);
+    if (rc) return hip_fail((hipError_t)rc, "capture stream", __LINE__);
+  }
   std::vector<hipEvent_t> ev;
-  std::vector<float> kms;        // per iteration: opmul, resid, update milliseconds
+  std::vector<float> kms;        // per iteration: per-kernel milliseconds
   std::vector<int> widths;       // per iteration: processed width
   const int esz = (int)sizeof(T);
   const int align = cpl0;        // compacted widths stay multiples of the initial vector width
-  std::vector<int> acth(R), permh(R);
+  std::vector<int> acth(S * R), permh(R);
   bool converged = false;
-  for (int64_t it = 0; it < maxit; ++it) {
-    const int cur = (int)(it & 1);
-    hipEvent_t* evp = nullptr;
-    if (prof) {
-      const int slot = (int)(it % 8) * kEv;
-      while ((int)ev.size() < slot + kEv) {
-        hipEvent_t e;
-        POLAR_HIP(hipEventCreate(&e));
-        ev.push_back(e);
+  for (int64_t it0 = 0; it0 < maxit; it0 += 8) {
+    const int nit = (int)std::min<int64_t>(8, maxit - it0);
+    if (use_graph && nit == 8) {
+      if (!graph.x) {
+        POLAR_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+        int rc = 0;
+        for (int k = 0; k < 8 && !rc; ++k)
+          rc = run_iter_any<T>(cpl, op, pl, tau0, r, d, v, lpart, P, Xo, sj, si, active, nA, p, s, k & 1, nullptr, cs);
+        hipGraph_t gr = nullptr;
+        const hipError_t ec = hipStreamEndCapture(cs, &gr);
+        if (rc) {
+          if (gr) (void)hipGraphDestroy(gr);
+          return rc;
+        }
+        POLAR_HIP(ec);
+        graph.g = gr;
+        POLAR_HIP(hipGraphInstantiate(&graph.x, graph.g, nullptr, nullptr, 0));
       }
-      evp = ev.data() + slot;
-      widths.push_back(s.R);
+      POLAR_HIP(hipGraphLaunch(graph.x, st));
+    } else {
+      for (int k = 0; k < nit; ++k) {
+        const int64_t it = it0 + k;
+        hipEvent_t* evp = nullptr;
+        if (prof) {
+          const int slot = k * kEv;
+          while ((int)ev.size() < slot + kEv) {
+            hipEvent_t e;
+            POLAR_HIP(hipEventCreate(&e));
+            ev.push_back(e);
+          }
+          evp = ev.data() + slot;
+          widths.push_back(s.R);
+        }
+        int rc = run_iter_any<T>(cpl, op, pl, tau0, r, d, v, lpart, P, Xo, sj, si, active, nA, p, s, (int)(it & 1), evp, st);
+        if (rc) return rc;
+      }
     }
-    int rc = run_iter_any<T>(cpl, op, pl, tau0, r, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, evp, st);
-    if (rc) return rc;
-    if (it % 8 != 7) continue;
     int flag = 1;
-    POLAR_HIP(hipMemcpyAsync(&flag, s.flag + ((it + 1) & 1), sizeof(int), hipMemcpyDeviceToHost, st));
-    POLAR_HIP(hipMemcpyAsync(acth.data(), s.act, sizeof(int) * s.R, hipMemcpyDeviceToHost, st));
+    POLAR_HIP(hipMemcpyAsync(&flag, s.flag + ((it0 + nit) & 1), sizeof(int), hipMemcpyDeviceToHost, st));
+    POLAR_HIP(hipMemcpyAsync(acth.data(), s.act, sizeof(int) * (S > 1 ? S * R : s.R), hipMemcpyDeviceToHost, st));
     POLAR_HIP(hipStreamSynchronize(st));
     if (prof) {
-      for (int k = 0; k < 8; ++k)
+      for (int k = 0; k < nit; ++k)
         for (int c = 0; c < kEv - 1; ++c) {
           float ms = 0.f;
           POLAR_HIP(hipEventElapsedTime(&ms, ev[k * kEv + c], ev[k * kEv + c + 1]));
@@ -1003,9 +1306,27 @@ static int msolve_impl(int cpl0, const polar_op* op, const T* W, int64_t ldw, in
       converged = true;
       break;
     }
-    // Active-column compaction: when at most half of the current slots are still active, move
-    // them (padded to the vector width with finished slots) to the front.  Columns are
-    // independent, so every column's arithmetic is unchanged.
+    if (S > 1) {
+      // Re-plan over the segments still iterating: finished segments get no blocks, the others
+      // share the whole budget (their kernels skip finished segments either way).
+      bool changed = false;
+      for (int g = 0; g < S; ++g) {
+        char on = 0;
+        for (int c = 0; c < R; ++c) on |= acth[g * R + c] != 0;
+        changed |= on != live[g];
+        live[g] = on;
+      }
+      if (changed) {
+        pl = make_plan(op, R, cpl, &sg, &live);
+        s.nb_a = pl.grid_a;
+        POLAR_HIP(hipMemcpyAsync(m.dtab, pl.tab.data(), sizeof(int) * pl.tab.size(), hipMemcpyHostToDevice, st));
+        graph.drop();
+      }
+      continue;
+    }
+    // Active-column compaction (one segment): when at most half of the current slots are still
+    // active, move them (padded to the vector width with finished slots) to the front.  Columns
+    // are independent, so every column's arithmetic is unchanged.
     int count = 0;
     for (int c = 0; c < s.R; ++c) count += acth[c] != 0;
     const int Rn = std::min(s.R, (int)align_up(std::max(count, 1), align));
@@ -1039,57 +1360,96 @@ static int msolve_impl(int cpl0, const polar_op* op, const T* W, int64_t ldw, in
       s.R = Rn;
       s.ident = 0;
       cpl = cpl_for<T>(Rn, R, cpl0);
-      pl = make_plan(op, Rn, cpl);
+      pl = make_plan(op, Rn, cpl, &sg, &live);
+      s.nb_a = pl.grid_a;
+      graph.drop();
     }
   }
   if (prof) {
     // count only the iterations that did work (launches after convergence return at once)
-    std::vector<int64_t> ih(s.R);
+    std::vector<int64_t> ih((size_t)S * s.R);
     unsigned long long st2[2];
-    POLAR_HIP(hipMemcpyAsync(ih.data(), s.iters, sizeof(int64_t) * s.R, hipMemcpyDeviceToHost, st));
+    if (S == 1) {
+      POLAR_HIP(hipMemcpyAsync(ih.data(), s.iters, sizeof(int64_t) * s.R, hipMemcpyDeviceToHost, st));
+    } else {
+      POLAR_HIP(hipMemcpyAsync(ih.data(), s.iters, sizeof(int64_t) * S * R, hipMemcpyDeviceToHost, st));
+    }
     POLAR_HIP(hipMemcpyAsync(st2, s.stats, sizeof(st2), hipMemcpyDeviceToHost, st));
     POLAR_HIP(hipStreamSynchronize(st));
     int64_t real = 0;
-    for (int c = 0; c < s.R; ++c) real = std::max(real, ih[c]);
+    for (int64_t v2 : ih) real = std::max(real, v2);
     real = std::min<int64_t>(real, (int64_t)kms.size() / (kEv - 1));
     int32_t nnz_h = 0;
     POLAR_HIP(hipMemcpy(&nnz_h, op->indptr + op->n, sizeof(int32_t), hipMemcpyDeviceToHost));
     for (int64_t k = 0; k < real; ++k) {
       for (int c = 0; c < kEv - 1; ++c) g_prof[c].ms += kms[(kEv - 1) * k + c];
-      const double w = widths[k];
+      const double w = (double)widths[k] * S;
       g_prof[0].bytes += 4.0 * (n + 1) + 4.0 * nnz_h + (op->weighted ? (double)esz * nnz_h : 0.0) +
-                         (double)esz * n + 2.0 * esz * (double)n * w;
-      g_prof[1].bytes += (double)esz * w * ((double)op->n_partial + 2.0 * op->n_long);
-      g_prof[2].bytes += 16.0 * w * (pl.grid_a + pl.grid_l);
-      g_prof[4].bytes += 16.0 * w * pl.grid_rows + 120.0 * w * p;
+                         (double)esz * n + 2.0 * esz * (double)n * widths[k];
+      g_prof[1].bytes += (double)esz * widths[k] * ((double)op->n_partial + 2.0 * op->n_long);
+      g_prof[2].bytes += 16.0 * widths[k] * (pl.grid_a + pl.grid_l);
+      g_prof[4].bytes += 16.0 * widths[k] * pl.grid_rows + 120.0 * w * p;
     }
     for (int c = 0; c < kEv - 1; ++c) g_prof[c].launches += real;
-    g_prof[3].bytes += 3.0 * esz * n * (double)st2[0];
-    g_prof[5].bytes += esz * (3.0 * n + nA) * (double)st2[0] + 4.0 * esz * nA * (double)st2[1];
+    // active columns summed over iterations; with segments a column-iteration covers its
+    // segment's rows only, so the row streams are counted at the mean segment length
+    const double nrows = (double)n / S, narows = (double)nA / S;
+    g_prof[3].bytes += 3.0 * esz * nrows * (double)st2[0];
+    g_prof[5].bytes += esz * (3.0 * nrows + narows) * (double)st2[0] + 4.0 * esz * narows * (double)st2[1];
     g_prof[6].launches += 1;
     g_prof[6].bytes += (double)st2[1];
     for (auto e : ev) (void)hipEventDestroy(e);
   }
-  hipLaunchKernelGGL(k_writeback, dim3((p * s.R + 255) / 256), dim3(256), 0, st, s, (int)R, h_out, te_out, iters_out);
+  hipLaunchKernelGGL(k_writeback, dim3((p * S * s.R + 255) / 256), dim3(256), 0, st, s, (int)R, h_out, te_out, iters_out);
   POLAR_LAUNCH_CHECK();
   if (!converged) return set_error(POLAR_E_ITERCAP, "multi-shift CG iteration cap");
   return 0;
 }
 
 template <typename T>
-static int64_t msolve_ws(const polar_op* op, int64_t R, int p, int64_t nA) {
-  Carver cv(nullptr);
-  CGState s{};
-  carve_state<T>(cv, s, R, p);
-  cv.take<int>(R);
-  cv.take<int>(R);
-  cv.take<T>(op->n * R);
-  cv.take<T>(op->n * R);
-  cv.take<T>(op->n * R);
-  cv.take<T>(op->n * R);
-  cv.take<T>((int64_t)p * nA * R);
-  cv.take<T>((int64_t)op->n_partial * R);
-  return cv.off + 256;
+static int64_t msolve_ws(const polar_op* op, int S, int64_t R, int p, int64_t nA) {
+  SolveMem<T> m;
+  return carve_solve<T>(nullptr, op, S, R, p, nA, m);
+}
+
+// Segment ranges from the caller's table, checked against the operator.
+static int segs_from(const polar_op* op, const polar_segs* in, Segs& sg) {
+  if (!in || in->S < 1 || !in->rows || !in->extra || !in->longs || !in->active)
+    return set_error(POLAR_E_ARG, "invalid polar_segs");
+  const int S = in->S;
+  sg.S = S;
+  sg.row.assign(in->rows, in->rows + S + 1);
+  sg.x.assign(in->extra, in->extra + S + 1);
+  sg.l.assign(in->longs, in->longs + S + 1);
+  sg.a.assign(in->active, in->active + S + 1);
+  if (sg.row[0] != 0 || sg.row[S] != op->n || sg.x[0] != 0 || sg.x[S] != op->n_extra || sg.l[0] != 0 ||
+      sg.l[S] != op->n_long || sg.a[0] != 0)
+    return set_error(POLAR_E_ARG, "segment ranges do not cover the operator");
+  for (int g = 0; g < S; ++g)
+    if (sg.row[g + 1] <= sg.row[g] || sg.x[g + 1] < sg.x[g] || sg.l[g + 1] < sg.l[g] || sg.a[g + 1] < sg.a[g])
+      return set_error(POLAR_E_ARG, "segment %d: empty or decreasing range", g);
+  return 0;
+}
+
+static Segs one_seg(const polar_op* op, int64_t nA) {
+  Segs sg;
+  sg.S = 1;
+  sg.row = {0, op->n};
+  sg.x = {0, op->n_extra};
+  sg.l = {0, op->n_long};
+  sg.a = {0, nA};
+  return sg;
+}
+
+template <typename T>
+static int msolve_dispatch(const polar_op* op, const Segs& sg, const void* W, int64_t ldw, int64_t R,
+                           const double* tau, int32_t p, const int32_t* active, double tol, int64_t maxit,
+                           int32_t flags, void* Xout, int64_t x_sj, int64_t x_si, double* h_out,
+                           double* te_out, int64_t* iters_out, void* ws, int64_t ws_bytes, hipStream_t st) {
+  const int cpl = pick_cpl(op->dtype, R, {{W, ldw}, {Xout, x_si}, {(const char*)Xout + 0, x_sj}}, op->cpl_max);
+  if (lanes_for(R, cpl) > 64) return set_error(POLAR_E_ARG, "R=%lld too wide for one wave at this alignment", (long long)R);
+  return msolve_impl<T>(cpl, op, sg, (const T*)W, ldw, R, tau, p, active, tol, maxit, flags, (T*)Xout, x_sj, x_si,
+                        h_out, te_out, iters_out, ws, ws_bytes, st);
 }
 
 }  // namespace polar
@@ -1126,7 +1486,7 @@ int polar_spmm(const polar_op* op, double shift, const void* X, int64_t ldx, voi
   if (int rc = check_op(op)) return rc;
   if (R <= 0 || ldx < R || ldy < R) return set_error(POLAR_E_ARG, "bad R/ld");
   hipStream_t st = (hipStream_t)stream;
-  const int cpl = pick_cpl(op->dtype, R, {{X, ldx}, {Y, ldy}, {ws, 0}});
+  const int cpl = pick_cpl(op->dtype, R, {{X, ldx}, {Y, ldy}, {ws, 0}}, op->cpl_max);
   if (lanes_for(R, cpl) > 64) return set_error(POLAR_E_ARG, "R=%lld too wide for one wave (max %d)", (long long)R, 64 * cpl);
   if (op->dtype == POLAR_F64) {
     if (cpl == 2) return spmm_impl<double, 2>(op, shift, (const double*)X, ldx, (double*)Y, ldy, R, ws, ws_bytes, st);
@@ -1140,7 +1500,7 @@ int polar_spmm(const polar_op* op, double shift, const void* X, int64_t ldx, voi
 int64_t polar_msolve_ws_bytes(const polar_op* op, int64_t R, int32_t p, int64_t nA, int32_t flags) {
   (void)flags;
   if (check_op(op)) return -1;
-  return op->dtype == POLAR_F64 ? msolve_ws<double>(op, R, p, nA) : msolve_ws<float>(op, R, p, nA);
+  return op->dtype == POLAR_F64 ? msolve_ws<double>(op, 1, R, p, nA) : msolve_ws<float>(op, 1, R, p, nA);
 }
 
 int polar_msolve(const polar_op* op, const void* W, int64_t ldw, int64_t R, const double* tau,
@@ -1151,13 +1511,40 @@ int polar_msolve(const polar_op* op, const void* W, int64_t ldw, int64_t R, cons
   if (R <= 0 || ldw < R || p <= 0 || nA <= 0 || !W || !tau || !active || !Xout || !ws)
     return set_error(POLAR_E_ARG, "bad msolve arguments");
   if (R > 512) return set_error(POLAR_E_ARG, "R=%lld exceeds 512 columns per block solve", (long long)R);
+  const Segs sg = one_seg(op, nA);
   hipStream_t st = (hipStream_t)stream;
-  const int cpl = pick_cpl(op->dtype, R, {{W, ldw}, {Xout, x_si}, {(const char*)Xout + 0, x_sj}});
-  if (lanes_for(R, cpl) > 64) return set_error(POLAR_E_ARG, "R=%lld too wide for one wave at this alignment", (long long)R);
-  if (op->dtype == POLAR_F64) {
-    return msolve_impl<double>(cpl, op, (const double*)W, ldw, R, tau, p, active, nA, tol, maxit, flags, (double*)Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
-  }
-  return msolve_impl<float>(cpl, op, (const float*)W, ldw, R, tau, p, active, nA, tol, maxit, flags, (float*)Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
+  if (op->dtype == POLAR_F64)
+    return msolve_dispatch<double>(op, sg, W, ldw, R, tau, p, active, tol, maxit, flags, Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
+  return msolve_dispatch<float>(op, sg, W, ldw, R, tau, p, active, tol, maxit, flags, Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
+}
+
+int64_t polar_msolve_batched_ws_bytes(const polar_op* op, const polar_segs* segs, int64_t R, int32_t p,
+                                      int32_t flags) {
+  (void)flags;
+  if (check_op(op)) return -1;
+  Segs sg;
+  if (segs_from(op, segs, sg)) return -1;
+  const int64_t nA = sg.a[sg.S];
+  return op->dtype == POLAR_F64 ? msolve_ws<double>(op, sg.S, R, p, nA) : msolve_ws<float>(op, sg.S, R, p, nA);
+}
+
+int polar_msolve_batched(const polar_op* op, const polar_segs* segs, const void* W, int64_t ldw, int64_t R,
+                         const double* tau, int32_t p, const int32_t* active, double tol, int64_t maxit,
+                         int32_t flags, void* Xout, int64_t x_sj, int64_t x_si, double* h_out,
+                         double* te_out, int64_t* iters_out, void* ws, int64_t ws_bytes, void* stream) {
+  if (int rc = check_op(op)) return rc;
+  Segs sg;
+  if (int rc = segs_from(op, segs, sg)) return rc;
+  const int64_t nA = sg.a[sg.S];
+  if (R <= 0 || ldw < R || p <= 0 || !W || !tau || (nA > 0 && !active) || !Xout || !ws)
+    return set_error(POLAR_E_ARG, "bad msolve arguments");
+  if (R > 512) return set_error(POLAR_E_ARG, "R=%lld exceeds 512 columns per block solve", (long long)R);
+  if ((int64_t)sg.S * R > (int64_t)INT32_MAX / 64 || (int64_t)sg.S * R * p > (int64_t)INT32_MAX / 2)
+    return set_error(POLAR_E_ARG, "too many segment columns");
+  hipStream_t st = (hipStream_t)stream;
+  if (op->dtype == POLAR_F64)
+    return msolve_dispatch<double>(op, sg, W, ldw, R, tau, p, active, tol, maxit, flags, Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
+  return msolve_dispatch<float>(op, sg, W, ldw, R, tau, p, active, tol, maxit, flags, Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
 }
 
 }  // extern "C"

@@ -25,7 +25,7 @@ from .quadrature import composite, fixed_rule, poles  # noqa: F401  (re-exported
 
 __all__ = ['composite', 'fixed_rule', 'poles', 'Graph', 'Op', 'Counter', 'msolve', 'sketches',
            'chunk_width', 'polar', 'degree_greedy', 'random_set', 'true_marginals', 'spmm',
-           'RademacherRows']
+           'RademacherRows', 'DeviceSketch']
 
 
 def _device():
@@ -237,13 +237,14 @@ def _max_cols(dtype):
 
 
 def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, freeze=True,
-           trace=False, out=None, width=None):
+           trace=False, out=None, width=None, graph=False):
     """Block multi-shift CG for (M + tau_j I) U = W on all poles (polar_gpu.py:88-133).
 
     Returns (X[p, nA, R], h[p, R] or None, iters[R] host int64) and, with trace=True, a 4th value
     te[p, R] = w^T (M + tau_j)^{-1} w.  `out` may be a (p, nA, R) view with unit column stride
     into which X is written (the bank).  `width` caps the columns solved together (the workspace
-    scales with it); columns are independent, so the results do not depend on it."""
+    scales with it); columns are independent, so the results do not depend on it.  graph=True
+    replays the CG iterations as hipGraphs of 8 iterations (same results, fewer launch gaps)."""
     lib = L.require_gpu()
     dev = _device()
     dtype = op.dtype
@@ -268,7 +269,8 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
     h = torch.empty((p, R), dtype=torch.float64, device=dev) if track else None
     te = torch.empty((p, R), dtype=torch.float64, device=dev) if trace else None
     iters = torch.empty(R, dtype=torch.int64, device=dev)
-    flags = (L.POLAR_TRACK if track else 0) | (L.POLAR_TRACE if trace else 0) | (0 if freeze else L.POLAR_NOFREEZE)
+    flags = ((L.POLAR_TRACK if track else 0) | (L.POLAR_TRACE if trace else 0) |
+             (0 if freeze else L.POLAR_NOFREEZE) | (L.POLAR_GRAPH if graph else 0))
     s = op.c_op(dtype)
     step = min(R, _max_cols(dtype), width or R)
     ws_bytes = lib.polar_msolve_ws_bytes(s, step, p, nA, flags)
@@ -345,14 +347,35 @@ class RademacherRows:
         return _rad_rows(self.seed, 2, np.arange(self.N), self.q) / math.sqrt(self.q)
 
 
+class DeviceSketch:
+    """A counter-hash Rademacher sketch (Z, F, Gc) that is generated on the GPU where polar() needs
+    it, so no resident copy is held beside the solver's own right-hand-side blocks."""
+
+    def __init__(self, G, seed, q, N):
+        self.G, self.seed, self.q = G, int(seed), int(q)
+        self.rows = RademacherRows(seed, q, N)
+
+    def fill(self, Zout, Fout, op):
+        """Write Z and F (n x q views with unit column stride) in op's dtype."""
+        lib = L.require_gpu()
+        scale = 1.0 / math.sqrt(self.q)
+        dt = L.dtype_code(Zout.dtype)
+        L.check(lib.polar_rademacher(dt, self.seed, 0, None, self.G.n, self.q, scale, L.ptr(Zout), Zout.stride(0),
+                                     L.stream_ptr()), 'polar_rademacher')
+        L.check(lib.polar_edge_sketch(op.c_op(Fout.dtype), self.seed, self.q, scale, L.ptr(Fout), Fout.stride(0),
+                                      L.stream_ptr()), 'polar_edge_sketch')
+
+
 def sketches(G, cand, q, seed, kind='block', chunk=4_000_000, on_device=False, precision='fp64'):
     """(Z n x q, F = C0^T U n x q, Gc N x q) of polar_gpu.py:148-165.  kind 'block' / 'gauss' follow
     the reference on the host; kind 'rademacher' uses the counter hash (+-1/sqrt(q)), which the
     GPU can generate in place (on_device=True: Z and F are device tensors, Gc rows are generated
-    per use)."""
+    per use; on_device='lazy': a DeviceSketch that polar() generates into its own blocks)."""
     n = G.n
     if kind == 'rademacher':
         scale = 1.0 / math.sqrt(q)
+        if on_device == 'lazy':
+            return DeviceSketch(G, seed, q, len(cand))
         if on_device:
             lib = L.require_gpu()
             dev = _device()
@@ -447,11 +470,12 @@ class _Phases:
 # ------------------------------------------------------------------------------------------------
 def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=None, naive=False,
           seed=2026, kind='block', trace=None, time_limit=None, poles=None, width=None,
-          freeze=True, precision='fp64', profile=False, progress=None):
+          freeze=True, precision='fp64', profile=False, progress=None, graph=False):
     """POLAR greedy, cache backend (polar_gpu.py:180-254), on the GPU.
 
     profile=True adds `phases`: seconds per phase (bank, select, rerank_solve, rerank_pick,
     maintenance, update), measured with a device synchronisation at each phase boundary.
+    graph=True replays the CG iterations of every solve as hipGraphs (same results).
     progress: optional callable(stage, info) called after the bank ('bank', {seconds, spmv}) and
     after each round ('round', {round, edge, marginal, seconds, spmv}); it reads host values only.
 
@@ -471,8 +495,14 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     p = len(tau)
     if sketch is None:
         sketch = sketches(G, cand, q, seed, kind)
-    Z, F, Gc = sketch
-    q = Z.shape[1]
+    lazy = isinstance(sketch, DeviceSketch)
+    if lazy:
+        Z = F = None
+        Gc = sketch.rows
+        q = sketch.q
+    else:
+        Z, F, Gc = sketch
+        q = Z.shape[1]
     cand = np.asarray(cand)
     active_h = np.unique(cand)                                                    # :191
     es_h = np.searchsorted(active_h, cand)
@@ -482,15 +512,31 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     eb = torch.from_numpy(es_h[:, 1].astype(np.int32)).to(dev)
     tau_d = torch.from_numpy(tau).to(dev)
     om_d = torch.from_numpy(om).to(dev)
-    RHS = torch.empty((n, 2 * q), dtype=dtype, device=dev)                         # [Z | F]
-    RHS[:, :q] = torch.as_tensor(Z, device=dev).to(dtype)
-    RHS[:, q:] = torch.as_tensor(F, device=dev).to(dtype)
+    # Bank right-hand sides [Z | F]: one n x 2q block when one block solve covers all 2q columns
+    # with room to spare, else separate Z and F blocks (Z is released after the bank).  Columns
+    # are independent, so the split does not change any result.
+    esz = 8 if dtype == torch.float64 else 4
+    free = torch.cuda.mem_get_info()[0]
+    joint = (p * nA * 2 * q + 2 * n * q + 2 * q * (4 * n + p * nA)) * esz < 0.8 * free
+    if joint:
+        RHS = torch.empty((n, 2 * q), dtype=dtype, device=dev)
+        Zb, Fb = RHS[:, :q], RHS[:, q:]
+    else:
+        Zb = torch.empty((n, q), dtype=dtype, device=dev)
+        Fb = torch.empty((n, q), dtype=dtype, device=dev)
+    op = Op(G, precision=dtype)
+    if lazy:
+        sketch.fill(Zb, Fb, op)
+    else:
+        Zb.copy_(torch.as_tensor(Z, device=dev).to(dtype))
+        Fb.copy_(torch.as_tensor(F, device=dev).to(dtype))
+    Zbox = [Zb]
+    del Zb
     Gc_dev = None
     if isinstance(Gc, torch.Tensor):
         Gc_dev = Gc.to(dev, torch.float64)
     elif not isinstance(Gc, RademacherRows):
         Gc = np.asarray(Gc, dtype=np.float64)
-    op = Op(G, precision=dtype)
     cnt = Counter()
     live = np.ones(N, bool)
     dead = torch.zeros(N, dtype=torch.uint8, device=dev)
@@ -515,12 +561,17 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     if width is None:
         width = bank_width(n, p, nA, q, dtype)          # sized on what is free after the caches
 
+    if joint:
+        blocks = [(lambda: RHS, 0, 2 * q)]
+    else:
+        blocks = [(lambda: Zbox[0], 0, q), (lambda: Fb, q, 2 * q)]
+
     def bank():                                                                    # :200-206
         terms = []
-        for lo in range(0, 2 * q, width):
-            hi = min(2 * q, lo + width)
-            res = msolve(op, RHS[:, lo:hi], tau, active, tol, counter=cnt, freeze=freeze,
-                         trace=want_trace and lo < q, out=XY[:, :, lo:hi])
+        chunks = [(src, c0, lo, min(c1, lo + width)) for src, c0, c1 in blocks for lo in range(c0, c1, width)]
+        for src, c0, lo, hi in chunks:
+            res = msolve(op, src()[:, lo - c0:hi - c0], tau, active, tol, counter=cnt, freeze=freeze,
+                         trace=want_trace and lo < q, out=XY[:, :, lo:hi], graph=graph)
             if want_trace and lo < q:
                 ncol = min(q, hi) - lo
                 terms.append((om[:, None] * res[3][:, :ncol].cpu().numpy()).ravel())
@@ -534,7 +585,10 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
                                       L.ptr(t), stream), 'pair_sqnorm')
         return X, Y, h, t, tr
 
+    Ubuf = None
     X, Y, h, t, tr = bank()
+    if not naive and not joint:
+        Zbox[0] = None                      # Z is not needed again
     ph.mark('bank')
     if progress is not None:
         progress('bank', dict(seconds=time.perf_counter() - t0, spmv=cnt.spmv))
@@ -562,10 +616,12 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             L.check(lib.polar_pair_rhs(dcode, n, Br, L.ptr(a_idx), L.ptr(b_idx), L.ptr(Wb), stream), 'pair_rhs')
             need_track = obj != 'logdet'
             ph.mark('select')
-            U = torch.empty((p, nA, Br), dtype=dtype, device=dev)
+            if Ubuf is None:                # one re-rank block for all rounds
+                Ubuf = torch.empty((p, nA, max(B, 1)), dtype=dtype, device=dev)
+            U = Ubuf[:, :, :Br]
             rw = fit_width(n, p, nA, Br, dtype)
             U, hh, _ = msolve(op, Wb, tau, active, tol, track=need_track, counter=cnt, freeze=freeze, out=U,
-                              width=rw)[:3]
+                              width=rw, graph=graph)[:3]
             ph.mark('rerank_solve')
             sa = torch.from_numpy(es_h[short, 0].astype(np.int32)).to(dev)
             sb = torch.from_numpy(es_h[short, 1].astype(np.int32)).to(dev)
@@ -602,7 +658,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
                     'maint_cache')
             if want_trace:
                 tr_hist.append(coef[:, 2 * q:2 * q + 2].clone())
-        Fv = RHS[:, q:]
+        Fv = Fb
         L.check(lib.polar_rows_axpy(dcode, L.ptr(Fv), Fv.stride(0), q, a, b, L.ptr(gc_row), stream), 'rows_axpy')  # :250
         ph.mark('maintenance')
         op.add(a, b)                                                               # :251

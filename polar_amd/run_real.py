@@ -88,7 +88,7 @@ def evaluate(G, cand, seq, s, obj, use_dense, Md):
 
 def _sk(a, G, cand, seed=2026):
     if a.kind == 'rademacher':
-        return pg.sketches(G, cand, a.q, seed, kind='rademacher', on_device=True)
+        return pg.sketches(G, cand, a.q, seed, kind='rademacher', on_device='lazy')
     return pg.sketches(G, cand, a.q, seed, kind=a.kind)
 
 

@@ -28,15 +28,18 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, s), s
     assert set(declared_symbols()) == set(_lib.EXPORTS)
     loaded = _lib.load()
-    assert loaded.polar_abi_version() == 1
+    assert loaded.polar_abi_version() == _lib.ABI_VERSION == 2
 
 
 def test_polar_op_struct_layout():
     from polar_amd._lib import PolarOp
     off = {name: getattr(PolarOp, name).offset for name, _ in PolarOp._fields_}
     assert off['indptr'] == 16 and off['diag'] == 40 and off['n_long'] == 48 and off['chunk'] == 60
-    assert off['long_rows'] == 64 and off['n_patch'] == 88 and off['patch_cols'] == 112
-    assert ctypes.sizeof(PolarOp) == 120
+    assert off['long_rows'] == 64 and off['n_patch'] == 88 and off['cpl_max'] == 92
+    assert off['patch_cols'] == 112 and off['grid_max'] == 120
+    assert ctypes.sizeof(PolarOp) == 128
+    from polar_amd._lib import PolarSegs
+    assert PolarSegs.rows.offset == 8 and PolarSegs.active.offset == 32 and ctypes.sizeof(PolarSegs) == 40
 
 
 def test_product_path_has_no_cpu_fallback(monkeypatch):

@@ -1,0 +1,122 @@
+"""Batched form: many graphs as segments of one block-diagonal operator.
+
+Segment g's arithmetic is the single-graph arithmetic, so batched results must equal the
+per-graph path exactly (bit for bit, FP64 and FP32): X blocks, tracked norms, trace terms,
+iteration counts, and the whole greedy (selections, marginals, trace estimates, counters).
+Also covered: hipGraph replay (identical), segments that finish at different iterations
+(re-planning), a weighted graph, long rows, added edges (row patch) and uneven shortlists."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip('torch')
+pytestmark = pytest.mark.gpu
+
+from polar_amd import batched, graphs  # noqa: E402
+from polar_amd import polar_gpu as pg  # noqa: E402
+
+
+def _graphs():
+    out = []
+    e = graphs.barabasi_albert(400, 3, 7)
+    out.append(pg.Graph(400, e, 0.1))
+    e = graphs.chung_lu(3000, 2.2, 10.0, 3)                      # hub rows longer than one chunk
+    out.append(pg.Graph(3000, e, 0.05))
+    e = graphs.chung_lu(1500, 2.5, 6.0, 4)
+    w = np.random.default_rng(1).uniform(0.5, 1.5, len(e))
+    out.append(pg.Graph(1500, e, 0.2, weights=w))
+    e = graphs.rmat(2048, 0.57, 0.19, 0.19, 0.05, 8, 5)
+    out.append(pg.Graph(2048, e, 0.05))
+    return out
+
+
+def _poles(Gs, p, k=4):
+    return [pg.fixed_rule(s, p, G.sigma, G.Lam(k))[:2] for s, G in zip([0.25, 0.5, 0.75, 0.5], Gs)]
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+@pytest.mark.parametrize('graph', [False, True])
+def test_msolve_batched_equals_single(precision, graph):
+    Gs = _graphs()
+    S, R, p = len(Gs), 8, 6
+    U = batched.UnionGraph(Gs)
+    dt = pg._dt(precision)
+    rng = np.random.default_rng(11)
+    Ws, acts = [], []
+    for G in Gs:
+        Ws.append(rng.standard_normal((G.n, R)))
+        acts.append(np.sort(rng.choice(G.n, G.n // 3, replace=False)))
+    Ws[1][:, 5] = 0.0                                             # an inactive column
+    poles = _poles(Gs, p)
+    tau = np.stack([t for t, _ in poles])
+    a0 = np.r_[0, np.cumsum([len(a) for a in acts])]
+    segs = batched.Segments(*U.segments(a0))
+    active = torch.from_numpy(np.concatenate([a + U.row0[g] for g, a in enumerate(acts)]).astype(np.int32)).cuda()
+    W = torch.from_numpy(np.concatenate(Ws)).cuda().to(dt)
+    op = pg.Op(U, precision=dt)
+    X, h, it, te = batched.msolve_batched(op, segs, W, tau, active, 1e-9, track=True, trace=True, graph=graph)
+    assert it[1, 5] == 0 and np.all(it[:, :5] > 0)
+    for g, G in enumerate(Gs):
+        Xs, hs, its, tes = pg.msolve(pg.Op(G, precision=dt), torch.from_numpy(Ws[g]).cuda().to(dt), tau[g], acts[g],
+                                     1e-9, track=True, trace=True)
+        assert np.array_equal(it[g], its), g
+        assert torch.equal(X[:, a0[g]:a0[g + 1]], Xs), g
+        assert torch.equal(h[:, g], hs) and torch.equal(te[:, g], tes), g
+
+
+def test_msolve_batched_graph_replay_equals_direct():
+    """Segments finish at different iterations: the plan shrinks and the captured graphs are rebuilt."""
+    Gs = _graphs()
+    U = batched.UnionGraph(Gs)
+    R, p = 16, 8
+    rng = np.random.default_rng(2)
+    W = torch.from_numpy(rng.standard_normal((U.n, R))).cuda()
+    tau = np.stack([t for t, _ in _poles(Gs, p)])
+    acts = [np.arange(G.n) for G in Gs]
+    a0 = np.r_[0, np.cumsum([G.n for G in Gs])]
+    segs = batched.Segments(*U.segments(a0))
+    active = torch.arange(U.n, dtype=torch.int32).cuda()
+    op = pg.Op(U)
+    res = [batched.msolve_batched(op, segs, W, tau, active, 1e-10, track=True, graph=gr) for gr in (False, True)]
+    assert len({tuple(r[2].max(axis=1)) for r in res}) == 1
+    assert len(set(res[0][2].max(axis=1))) > 1                   # segments converge at different counts
+    assert torch.equal(res[0][0], res[1][0]) and torch.equal(res[0][1], res[1][1])
+    del acts
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_polar_batched_equals_per_graph(precision):
+    Gs = _graphs()
+    k, B, q, p = 5, 3, 16, 8
+    cands = [graphs.candidates(G.n, G.edges, 300 + 50 * g, 20 + g) for g, G in enumerate(Gs)]
+    cands[0] = cands[0][:6]                             # last round's shortlist is shorter than B (uneven widths)
+    poles = _poles(Gs, p, k)
+    sk = [pg.sketches(G, cands[g], q, 40 + g, kind='rademacher', on_device=True, precision=precision)
+          for g, G in enumerate(Gs)]
+    kk = [min(k, len(c)) for c in cands]
+    out = batched.polar_batched(Gs, cands, k, 0.5, q=q, B=B, sketch=sk, poles=poles, precision=precision)
+    for g, G in enumerate(Gs):
+        ref = pg.polar(G, cands[g], kk[g], 0.5, q=q, B=B, sketch=sk[g], poles=poles[g], precision=precision)
+        got = out[g]
+        assert got['sequence'][:kk[g]] == ref['sequence'], g
+        assert got['marginals'][:kk[g]] == ref['marginals'], g
+        assert got['trace_estimate'][:kk[g]] == ref['trace_estimate'][:kk[g]], g
+        if kk[g] == k:
+            assert (got['spmv'], got['rhs']) == (ref['spmv'], ref['rhs']), g
+
+
+def test_lazy_sketch_and_split_bank_blocks_change_nothing(monkeypatch):
+    """polar() with a DeviceSketch (generated in place) and with separate Z / F bank blocks (the
+    memory-tight layout, forced by reporting little free memory) equals the resident-tensor run."""
+    G = _graphs()[1]
+    cand = graphs.candidates(G.n, G.edges, 400, 3)
+    poles = pg.fixed_rule(0.5, 8, G.sigma, G.Lam(4))[:2]
+    sk = pg.sketches(G, cand, 16, 9, kind='rademacher', on_device=True)
+    ref = pg.polar(G, cand, 4, 0.5, q=16, B=3, sketch=sk, poles=poles)
+    lazy = pg.polar(G, cand, 4, 0.5, q=16, B=3, sketch=pg.sketches(G, cand, 16, 9, kind='rademacher', on_device='lazy'),
+                    poles=poles)
+    total = torch.cuda.mem_get_info()[1]
+    monkeypatch.setattr(pg.torch.cuda, 'mem_get_info', lambda *a: (1 << 20, total))
+    split = pg.polar(G, cand, 4, 0.5, q=16, B=3, sketch=sk, poles=poles, width=8)
+    for out in (lazy, split):
+        assert out['sequence'] == ref['sequence'] and out['marginals'] == ref['marginals']
+        assert out['trace_estimate'] == ref['trace_estimate'] and out['spmv'] == ref['spmv']
