@@ -72,7 +72,7 @@ class Segments:
 
 
 def msolve_batched(op, segs, W, tau, active, tol, maxit=100000, track=False, trace=False, out=None,
-                   width=None, graph=False, freeze=True):
+                   width=None, graph=None, freeze=True):
     """Batched block multi-shift CG: segment g solves (M_g + tau[g, j]) U = W[rows of g, :R].
 
     op: Op of a UnionGraph; segs: Segments; W: (n, R) device; tau: (S, p) host or device; active:
@@ -94,6 +94,8 @@ def msolve_batched(op, segs, W, tau, active, tol, maxit=100000, track=False, tra
     h = torch.empty((p, S, R), dtype=torch.float64, device=dev) if track else None
     te = torch.empty((p, S, R), dtype=torch.float64, device=dev) if trace else None
     iters = torch.empty((S, R), dtype=torch.int64, device=dev)
+    if graph is None:
+        graph = getattr(op, 'use_graph', False)
     flags = ((L.POLAR_TRACK if track else 0) | (L.POLAR_TRACE if trace else 0) |
              (0 if freeze else L.POLAR_NOFREEZE) | (L.POLAR_GRAPH if graph else 0))
     s = op.c_op(dtype)
@@ -131,12 +133,15 @@ def _seq(x, S):
 
 
 def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac', sketch=None, seed=2026,
-                  kind='rademacher', poles=None, precision='fp32', graph=True, width=None, progress=None):
+                  kind='rademacher', poles=None, precision='fp32', graph=None, width=None, progress=None,
+                  tune=None):
     """POLAR greedy (polar_gpu.py:180-254, cache backend) on S graphs in lockstep.
 
     graphs: list of Graph; cands: list of (N_g, 2) candidate arrays; s: one value or one per graph;
     sketch: list of (Z, F, Gc) per graph (default: Rademacher with seed + g); poles: list of
     (tau, om) per graph (all with the same pole count) or None for `poles(obj, s_g, ...)`.
+    graph: hipGraph replay (None: the tuned choice, else True); tune: a launch-shape dict from
+    polar_amd.autotune or True to tune the batched operator here (before the timed region).
     Returns one result dict per graph with the keys of polar_gpu.polar (`seconds` is the whole
     batch's wall time)."""
     lib = L.require_gpu()
@@ -198,6 +203,14 @@ def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac'
     ea = [torch.from_numpy(e[:, 0].astype(np.int32)).to(dev) for e in es]
     eb = [torch.from_numpy(e[:, 1].astype(np.int32)).to(dev) for e in es]
     op = pg.Op(U, precision=dtype)
+    if tune is True:
+        from . import autotune
+        tune = autotune.tune(op, min(2 * q, pg._max_cols(dtype)), tau_all, active, segs=segs)
+    if tune:
+        from . import autotune
+        autotune.apply(op, tune)
+    if graph is None:
+        graph = op.use_graph if tune else True
     spmv = np.zeros(S, dtype=np.int64)
     rhs = np.zeros(S, dtype=np.int64)
     live = [np.ones(N, bool) for N in Ns]

@@ -141,6 +141,9 @@ class Op:
         self.dtype = _dt(precision)
         self._patch = None
         self._struct = None
+        self.cpl_max = 0          # launch-shape tuning (polar_amd.autotune); 0 = library default
+        self.grid_max = 0
+        self.use_graph = False
 
     def add(self, a, b):
         self.A.append(int(a))
@@ -186,6 +189,8 @@ class Op:
         s.n_extra = len(sc['extra'])
         s.n_partial = sc['n_partial']
         s.chunk = L.CHUNK
+        s.cpl_max = self.cpl_max
+        s.grid_max = self.grid_max
         s.long_rows = L.ptr(idx['long_rows']) if s.n_long else None
         s.long_base = L.ptr(idx['long_base']) if s.n_long else None
         s.extra = L.ptr(idx['extra']) if s.n_extra else None
@@ -237,14 +242,15 @@ def _max_cols(dtype):
 
 
 def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, freeze=True,
-           trace=False, out=None, width=None, graph=False):
+           trace=False, out=None, width=None, graph=None):
     """Block multi-shift CG for (M + tau_j I) U = W on all poles (polar_gpu.py:88-133).
 
     Returns (X[p, nA, R], h[p, R] or None, iters[R] host int64) and, with trace=True, a 4th value
     te[p, R] = w^T (M + tau_j)^{-1} w.  `out` may be a (p, nA, R) view with unit column stride
     into which X is written (the bank).  `width` caps the columns solved together (the workspace
     scales with it); columns are independent, so the results do not depend on it.  graph=True
-    replays the CG iterations as hipGraphs of 8 iterations (same results, fewer launch gaps)."""
+    replays the CG iterations as hipGraphs of 8 iterations (same results, fewer launch gaps);
+    None takes the operator's tuned choice (op.use_graph, see polar_amd.autotune)."""
     lib = L.require_gpu()
     dev = _device()
     dtype = op.dtype
@@ -269,6 +275,8 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
     h = torch.empty((p, R), dtype=torch.float64, device=dev) if track else None
     te = torch.empty((p, R), dtype=torch.float64, device=dev) if trace else None
     iters = torch.empty(R, dtype=torch.int64, device=dev)
+    if graph is None:
+        graph = getattr(op, 'use_graph', False)
     flags = ((L.POLAR_TRACK if track else 0) | (L.POLAR_TRACE if trace else 0) |
              (0 if freeze else L.POLAR_NOFREEZE) | (L.POLAR_GRAPH if graph else 0))
     s = op.c_op(dtype)
@@ -470,12 +478,14 @@ class _Phases:
 # ------------------------------------------------------------------------------------------------
 def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=None, naive=False,
           seed=2026, kind='block', trace=None, time_limit=None, poles=None, width=None,
-          freeze=True, precision='fp64', profile=False, progress=None, graph=False):
+          freeze=True, precision='fp64', profile=False, progress=None, graph=None, tune=None):
     """POLAR greedy, cache backend (polar_gpu.py:180-254), on the GPU.
 
     profile=True adds `phases`: seconds per phase (bank, select, rerank_solve, rerank_pick,
     maintenance, update), measured with a device synchronisation at each phase boundary.
     graph=True replays the CG iterations of every solve as hipGraphs (same results).
+    tune: a launch-shape dict from polar_amd.autotune (same results) or True to tune here (before
+    the timed region).
     progress: optional callable(stage, info) called after the bank ('bank', {seconds, spmv}) and
     after each round ('round', {round, edge, marginal, seconds, spmv}); it reads host values only.
 
@@ -537,6 +547,12 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
         Gc_dev = Gc.to(dev, torch.float64)
     elif not isinstance(Gc, RademacherRows):
         Gc = np.asarray(Gc, dtype=np.float64)
+    if tune is True:
+        from . import autotune
+        tune = autotune.tune(op, min(2 * q, _max_cols(dtype)), tau, active)
+    if tune:
+        from . import autotune
+        autotune.apply(op, tune)
     cnt = Counter()
     live = np.ones(N, bool)
     dead = torch.zeros(N, dtype=torch.uint8, device=dev)

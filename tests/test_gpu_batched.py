@@ -120,3 +120,24 @@ def test_lazy_sketch_and_split_bank_blocks_change_nothing(monkeypatch):
     for out in (lazy, split):
         assert out['sequence'] == ref['sequence'] and out['marginals'] == ref['marginals']
         assert out['trace_estimate'] == ref['trace_estimate'] and out['spmv'] == ref['spmv']
+
+
+def test_autotune_keeps_results_and_run_batched_cli(tmp_path, monkeypatch):
+    """The tuned launch shape changes no result; the c4 CLI (3 graphs, shrunk) agrees across modes."""
+    from polar_amd import autotune, run_batched
+    monkeypatch.setenv('POLAR_TUNE_CACHE', str(tmp_path / 'tune.json'))
+    G = _graphs()[3]
+    cand = graphs.candidates(G.n, G.edges, 300, 8)
+    poles = pg.fixed_rule(0.5, 8, G.sigma, G.Lam(4))[:2]
+    sk = pg.sketches(G, cand, 16, 5, kind='rademacher', on_device='lazy')
+    ref = pg.polar(G, cand, 4, 0.5, q=16, B=3, sketch=sk, poles=poles)
+    op = pg.Op(G)
+    act = torch.from_numpy(np.unique(cand).astype(np.int32)).cuda()
+    cfg = autotune.tune(op, 32, poles[0], act, iters=8, reps=1)
+    assert set(cfg) >= {'cpl_max', 'grid_max', 'graph'} and (tmp_path / 'tune.json').exists()
+    for forced in (cfg, dict(cpl_max=1, grid_max=256, graph=True)):
+        out = pg.polar(G, cand, 4, 0.5, q=16, B=3, sketch=sk, poles=poles, tune=forced)
+        assert out['sequence'] == ref['sequence'] and out['marginals'] == ref['marginals']
+        assert out['spmv'] == ref['spmv']
+    res = run_batched.main(['--graphs', '3', '--scale', '0.03', '--k', '3', '--tune'])
+    assert res['selections_identical'] and set(res['modes']) == {'batched', 'nograph', 'sequential'}
