@@ -153,7 +153,7 @@ def main():
     t = time.time()
     inp = configs.make(a.config, a.scale)
     G = pg.Graph(inp['n'], inp['edges'], inp['sigma'])
-    sk = pg.sketches(G, inp['cand'], inp['q'], inp['seed'], kind='rademacher', on_device=True, precision='fp64')
+    sk = pg.sketches(G, inp['cand'], inp['q'], inp['seed'], kind='rademacher', on_device='lazy')
     torch.cuda.synchronize()
     log(f'{a.config}: n={inp["n"]} m={inp["m"]} N={inp["N"]} nA={len(np.unique(inp["cand"]))} '
         f'dmax={inp["dmax"]} inputs in {time.time() - t:.1f}s')

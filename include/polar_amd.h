@@ -48,6 +48,9 @@ enum polar_status {
 #define POLAR_TRACE 2     /* track w^T (M + tau_j)^{-1} w with two scalars per pole */
 #define POLAR_NOFREEZE 4  /* never freeze converged poles (synthetic/scalability/benchmark.py:72) */
 #define POLAR_GRAPH 8     /* replay the CG iterations as hipGraphs of 8 iterations (same results) */
+#define POLAR_BLOCK8 16   /* shifted x/p updates once per 8 iterations from the stored residuals:
+                             same CG scalars and iteration counts, X within rounding of the
+                             per-iteration updates, about 8x fewer passes over the shifted blocks */
 
 /* Operator y = (diag + shift) x - A_w x + patch(x): the graph Laplacian M0 = diag(deg+sigma) - A
  * of realworld/polar_gpu.py:47-55 stored as diagonal + off-diagonal adjacency (value-free when

@@ -223,6 +223,13 @@ struct CGState {
   const int64_t* seg_x;       // [S+1] long-row chunk unit ranges (entries of op.extra)
   const int64_t* seg_l;       // [S+1] long-row index ranges
   const int64_t* seg_a;       // [S+1] active-row ranges
+  // Blocked shifted update (T > 1): the residuals of a window of T iterations are kept (ring of
+  // T vectors) and X_j, P_j are updated once per window from per-(pole, column) coefficients:
+  //   X_j <- X_j + cX[0] P_j + sum_t cX[t] r_t,   P_j <- cP[0] P_j + sum_t cP[t] r_t,
+  // built by the finisher from the same aj, bj, zn the per-iteration update would apply.
+  int T;
+  double *cX, *cP;            // [p][T+1][sld]
+  int* dirty;                 // [p][sld] (pole, column) changed in the current window
 };
 
 // Segment of block b of a kernel whose blocks are partitioned by the prefix table `pref`.
@@ -285,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_alpha(CGState s, int nparts, int cur
 // Finisher after |rn|^2 (one workgroup per (segment, column), threads over poles): every scalar
 // recurrence of polar_gpu.py:106-125 for the column, freezing, convergence and the any-active
 // flags.
-__global__ __launch_bounds__(kBlock) void k_scalars(CGState s, int nparts, int cur) {
+__global__ __launch_bounds__(kBlock) void k_scalars(CGState s, int nparts, int cur, int wt) {
   if (s.flag[cur] == 0) return;
   const int g = blockIdx.x / s.R, c = blockIdx.x - g * s.R;
   if (!seg_live(s, cur, g)) return;
@@ -333,6 +340,19 @@ __global__ __launch_bounds__(kBlock) void k_scalars(CGState s, int nparts, int c
     if (m) {
       s.zp[idx] = z;
       s.z[idx] = zn;
+    }
+    if (s.T > 1 && m) {
+      // window step t = wt + 1: X += aj P, P = zn r_t + bj P in the window basis {P_j, r_1..r_T}
+      const int t = wt + 1;
+      double* cx = s.cX + (int64_t)j * (s.T + 1) * s.sld + vc;
+      double* cp = s.cP + (int64_t)j * (s.T + 1) * s.sld + vc;
+      for (int k = 0; k < t; ++k) {
+        const double c = cp[(int64_t)k * s.sld];
+        cx[(int64_t)k * s.sld] += aj * c;
+        cp[(int64_t)k * s.sld] = bj * c;
+      }
+      cp[(int64_t)t * s.sld] = zn;
+      s.dirty[idx] = 1;
     }
     if (s.freeze && m && fabs(zn) * nr <= stop) s.live[idx] = 0;
   }
@@ -537,7 +557,7 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
 
 // Kernel B: rn = r - a v (active columns) and |rn|^2 partials (finished by k_scalars).
 template <typename T, int CPL>
-__global__ __launch_bounds__(kBlock) void k_resid(int64_t n, int R, int L, T* __restrict__ r,
+__global__ __launch_bounds__(kBlock) void k_resid(int64_t n, int R, int L, const T* rin, T* rout,
                                                   const T* __restrict__ v, CGState s, int cur) {
   // the finisher of this iteration raises flag[cur^1] again if a column is still active; when
   // converged, both parities stay 0 so every later launch is a no-op
@@ -571,14 +591,14 @@ __global__ __launch_bounds__(kBlock) void k_resid(int64_t n, int R, int L, T* __
     const int gpb = kWaves * g.G;
     for (int64_t i = r0 + (int64_t)sp.rank * gpb + g.wave * g.G + g.gi; i < r1; i += (int64_t)sp.nb * gpb) {
       T rv[CPL], vv[CPL];
-      vload<T, CPL>(r + i * s.ld + c0, rv);
+      vload<T, CPL>(rin + i * s.ld + c0, rv);
       if (any) {
         vload<T, CPL>(v + i * s.ld + c0, vv);
 #pragma unroll
         for (int e = 0; e < CPL; ++e)
           if (act[e]) rv[e] = rv[e] - (a[e] * vv[e]);
-        vstore<T, CPL>(r + i * s.ld + c0, rv);
       }
+      if (any || rin != rout) vstore<T, CPL>(rout + i * s.ld + c0, rv);
 #pragma unroll
       for (int e = 0; e < CPL; ++e) dd_add(dacc[e], (double)rv[e] * (double)rv[e]);
     }
@@ -713,6 +733,124 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
       vstore<T, CPL>(pp, pv);
     }
   }
+}
+
+// Blocked shifted update, once per window of kW iterations (after the window's last iteration):
+// X_j += cX0 P_j + sum_t cXt r_t;  P_j = cP0 P_j + sum_t cPt r_t  on the active rows, for the
+// poles changed in the window.  Each thread keeps its rows' window residuals in registers; one
+// pole's coefficients at a time are staged in LDS and shared by the block's rows.
+constexpr int kW = 8;            // window length of the blocked update
+constexpr int kFlushRows = 2;    // rows per thread per tile
+constexpr int kMaxPolesFlush = 1024;
+
+template <typename T> struct Ring {
+  const T* s[kW];                // r_t is in s[t % kW]
+};
+
+template <typename T, int CPL>
+__global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int L, Ring<T> ring,
+                                                  const int32_t* __restrict__ active, T* __restrict__ P,
+                                                  T* __restrict__ Xo, int64_t sj, int64_t si, CGState s) {
+  const SegPart sp = seg_part(s.bS, s.S, blockIdx.x, gridDim.x);
+  int64_t a0, a1;
+  seg_range(s.seg_a, s.S, sp.g, nA, a0, a1);
+  const int vb = sp.g * s.ld;
+  __shared__ int s_jl[kMaxPolesFlush];
+  __shared__ int s_nj;
+  __shared__ T s_cx[(kW + 1) * 256], s_cp[(kW + 1) * 256];
+  if (threadIdx.x == 0) s_nj = 0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < p; j += kBlock) {
+    bool any = false;
+    for (int c = 0; c < R; ++c) any |= s.dirty[(int64_t)j * s.sld + vb + c] != 0;
+    if (any) s_jl[atomicAdd(&s_nj, 1)] = j;     // pole order is irrelevant: poles are independent
+  }
+  __syncthreads();
+  const int nj = s_nj;
+  if (nj == 0) return;                           // uniform over the block
+  Geo g(L);
+  const int c0 = g.li * CPL;
+  const bool colok = c0 < R;
+  const int gpb = kWaves * g.G;
+  const int64_t tile = (int64_t)gpb * kFlushRows;
+  for (int64_t base = a0 + (int64_t)sp.rank * tile; base < a1; base += (int64_t)sp.nb * tile) {
+    T rr[kFlushRows][kW][CPL];
+    int64_t row[kFlushRows];
+    bool ok[kFlushRows];
+#pragma unroll
+    for (int k = 0; k < kFlushRows; ++k) {
+      row[k] = base + g.wave * g.G + g.gi + (int64_t)k * gpb;
+      ok[k] = colok && row[k] < a1;
+      if (ok[k]) {
+        const int64_t ar = active[row[k]];
+#pragma unroll
+        for (int t = 1; t <= kW; ++t) vload<T, CPL>(ring.s[t % kW] + ar * s.ld + c0, rr[k][t - 1]);
+      }
+    }
+    for (int jj = 0; jj < nj; ++jj) {
+      const int j = s_jl[jj];
+      __syncthreads();                           // the previous pole's coefficients are consumed
+      for (int q = threadIdx.x; q < (kW + 1) * R; q += kBlock) {
+        const int t = q / R, c = q - t * R;
+        const int64_t src = ((int64_t)j * (kW + 1) + t) * s.sld + vb + c;
+        s_cx[q] = (T)s.cX[src];
+        s_cp[q] = (T)s.cP[src];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kFlushRows; ++k) {
+        if (!ok[k]) continue;
+        T* pp = P + ((int64_t)j * nA + row[k]) * s.ld + c0;
+        T* xrow = Xo + (int64_t)j * sj + row[k] * si;
+        T pv[CPL], xv[CPL];
+        vload<T, CPL>(pp, pv);
+        if (s.ident) {
+          vload<T, CPL>(xrow + c0, xv);
+        } else {
+#pragma unroll
+          for (int e = 0; e < CPL; ++e) xv[e] = xrow[s.colmap[c0 + e]];
+        }
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) {
+          T ax = s_cx[c0 + e] * pv[e];
+          T ap = s_cp[c0 + e] * pv[e];
+#pragma unroll
+          for (int t = 1; t <= kW; ++t) {
+            ax = ax + s_cx[t * R + c0 + e] * rr[k][t - 1][e];
+            ap = ap + s_cp[t * R + c0 + e] * rr[k][t - 1][e];
+          }
+          xv[e] = xv[e] + ax;
+          pv[e] = ap;
+        }
+        if (s.ident) {
+          vstore<T, CPL>(xrow + c0, xv);
+        } else {
+#pragma unroll
+          for (int e = 0; e < CPL; ++e) xrow[s.colmap[c0 + e]] = xv[e];
+        }
+        vstore<T, CPL>(pp, pv);
+      }
+    }
+  }
+}
+
+// Start a window: identity coefficients (cP0 = 1, the rest 0), nothing dirty.  Counts the dirty
+// (pole, column) pairs of the window that ends (stats[2]).
+__global__ void k_coef_reset(CGState s) {
+  const int64_t tot = (int64_t)s.p * s.sld;
+  unsigned long long nd = 0;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = idx / s.sld, vc = idx - j * s.sld;
+    double* cx = s.cX + j * (s.T + 1) * s.sld + vc;
+    double* cp = s.cP + j * (s.T + 1) * s.sld + vc;
+    for (int t = 0; t <= s.T; ++t) {
+      cx[(int64_t)t * s.sld] = 0.0;
+      cp[(int64_t)t * s.sld] = t == 0 ? 1.0 : 0.0;
+    }
+    nd += s.dirty[idx] != 0;
+    s.dirty[idx] = 0;
+  }
+  if (nd) atomicAdd(&s.stats[2], nd);
 }
 
 // Init: r = d = W[:, :R]; |w|^2 partials; last block initialises every scalar.
@@ -921,13 +1059,14 @@ template <typename T>
 struct SolveMem {
   CGState s{};
   int* perm;
-  T *r, *d, *v, *t2, *P, *lpart;
+  T *ring, *d, *v, *t2, *P, *lpart;   // ring: Tw residual vectors (Tw = 1: the residual itself)
   int* dtab;           // 5 (S+1) block prefix tables
   int64_t* dseg;       // 4 (S+1) segment ranges
 };
 
 template <typename T>
-static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int p, int64_t nA, SolveMem<T>& m) {
+static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int p, int64_t nA, int Tw,
+                           SolveMem<T>& m) {
   Carver cv(base);
   CGState& s = m.s;
   const int64_t C = (int64_t)S * R;
@@ -941,15 +1080,20 @@ static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int
   s.tg = cv.take<double>(pr); s.shift = cv.take<double>((int64_t)p * S);
   s.live = cv.take<int>(pr); s.m = cv.take<int>(pr);
   s.flag = cv.take<int>(2);
-  s.stats = cv.take<unsigned long long>(2);
+  s.stats = cv.take<unsigned long long>(3);
   s.parts = cv.take<double>(parts_rows(op, S) * 2 * R);
+  if (Tw > 1) {
+    s.cX = cv.take<double>(pr * (Tw + 1));
+    s.cP = cv.take<double>(pr * (Tw + 1));
+    s.dirty = cv.take<int>(pr);
+  }
   s.colmap = cv.take<int>(R);
   s.sflag = cv.take<int>(2 * S);
   m.dtab = cv.take<int>(5 * (S + 1));
   m.dseg = cv.take<int64_t>(4 * (S + 1));
   m.perm = cv.take<int>(R);
   const int64_t n = op->n;
-  m.r = cv.take<T>(n * R);
+  m.ring = cv.take<T>(n * R * Tw);
   m.d = cv.take<T>(n * R);
   m.v = cv.take<T>(n * R);
   m.t2 = cv.take<T>(n * R);
@@ -1052,10 +1196,13 @@ __global__ void k_iota(int* __restrict__ a, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = i;
 }
 
+// One CG iteration: rin is the current residual, rout receives the next one (the same vector
+// unless the shifted update is blocked; then the k_update launch only updates the direction and
+// wt is the iteration's position in its window).
 template <typename T, int CPL>
-static int run_iter(const polar_op* op, const Plan& pl, double tau0, T* r, T* d, T* v, T* lpart, T* P, T* Xo,
-                    int64_t sj, int64_t si, const int32_t* active, int64_t nA, int p, const CGState& s, int cur,
-                    hipEvent_t* ev, hipStream_t st) {
+static int run_iter(const polar_op* op, const Plan& pl, double tau0, const T* rin, T* rout, T* d, T* v, T* lpart,
+                    T* P, T* Xo, int64_t sj, int64_t si, const int32_t* active, int64_t nA, int p, const CGState& s,
+                    int cur, int wt, hipEvent_t* ev, hipStream_t st) {
   const int64_t n = op->n;
   const int R = s.R;
   const bool longs = op->n_long > 0 && pl.grid_l > 0;
@@ -1075,33 +1222,53 @@ static int run_iter(const polar_op* op, const Plan& pl, double tau0, T* r, T* d,
   hipLaunchKernelGGL(k_alpha, dim3(pl.fin), dim3(kBlock), 0, st, s, nparts_a, cur);
   POLAR_LAUNCH_CHECK();
   if (ev) POLAR_HIP(hipEventRecord(ev[3], st));
-  hipLaunchKernelGGL((k_resid<T, CPL>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, R, pl.L, r, (const T*)v, s, cur);
+  hipLaunchKernelGGL((k_resid<T, CPL>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, R, pl.L, rin, rout, (const T*)v, s, cur);
   POLAR_LAUNCH_CHECK();
   if (ev) POLAR_HIP(hipEventRecord(ev[4], st));
-  hipLaunchKernelGGL(k_scalars, dim3(pl.fin), dim3(kBlock), 0, st, s, pl.grid_rows, cur);
+  hipLaunchKernelGGL(k_scalars, dim3(pl.fin), dim3(kBlock), 0, st, s, pl.grid_rows, cur, wt);
   POLAR_LAUNCH_CHECK();
   if (ev) POLAR_HIP(hipEventRecord(ev[5], st));
-  const int nb_d = pl.nb_d, nb_s = pl.nb_s;
+  const int nb_d = pl.nb_d, nb_s = s.T > 1 ? 0 : pl.nb_s;    // blocked: the window flush updates X, P
   if (p * R <= 1024)
     hipLaunchKernelGGL((k_update<T, CPL, true>), dim3(nb_d + nb_s), dim3(kBlock), 0, st, n, nA, p, R, pl.L,
-                       (const T*)r, d, active, P, Xo, sj, si, nb_d, s, cur);
+                       (const T*)rout, d, active, P, Xo, sj, si, nb_d, s, cur);
   else
     hipLaunchKernelGGL((k_update<T, CPL, false>), dim3(nb_d + nb_s), dim3(kBlock), 0, st, n, nA, p, R, pl.L,
-                       (const T*)r, d, active, P, Xo, sj, si, nb_d, s, cur);
+                       (const T*)rout, d, active, P, Xo, sj, si, nb_d, s, cur);
   POLAR_LAUNCH_CHECK();
   if (ev) POLAR_HIP(hipEventRecord(ev[6], st));
   return 0;
 }
 
 template <typename T>
-static int run_iter_any(int cpl, const polar_op* op, const Plan& pl, double tau0, T* r, T* d, T* v, T* lpart,
-                        T* P, T* Xo, int64_t sj, int64_t si, const int32_t* active, int64_t nA, int p,
-                        const CGState& s, int cur, hipEvent_t* ev, hipStream_t st) {
-  if (cpl == 2) return run_iter<T, 2>(op, pl, tau0, r, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, ev, st);
+static int run_iter_any(int cpl, const polar_op* op, const Plan& pl, double tau0, const T* rin, T* rout, T* d, T* v,
+                        T* lpart, T* P, T* Xo, int64_t sj, int64_t si, const int32_t* active, int64_t nA, int p,
+                        const CGState& s, int cur, int wt, hipEvent_t* ev, hipStream_t st) {
+  if (cpl == 2) return run_iter<T, 2>(op, pl, tau0, rin, rout, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, wt, ev, st);
   if constexpr (sizeof(T) == 4) {
-    if (cpl == 4) return run_iter<T, 4>(op, pl, tau0, r, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, ev, st);
+    if (cpl == 4) return run_iter<T, 4>(op, pl, tau0, rin, rout, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, wt, ev, st);
   }
-  return run_iter<T, 1>(op, pl, tau0, r, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, ev, st);
+  return run_iter<T, 1>(op, pl, tau0, rin, rout, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, wt, ev, st);
+}
+
+// End of a window of the blocked update: apply the coefficients to X, P, then start a new window.
+template <typename T>
+static int flush_window(int cpl, const Plan& pl, const Ring<T>& ring, const int32_t* active, int64_t nA, int p,
+                        T* P, T* Xo, int64_t sj, int64_t si, const CGState& s, hipStream_t st) {
+  if (nA > 0 && pl.nb_s > 0) {
+    if (cpl == 2) {
+      hipLaunchKernelGGL((k_flush<T, 2>), dim3(pl.nb_s), dim3(kBlock), 0, st, nA, p, s.R, pl.L, ring, active, P, Xo, sj, si, s);
+    } else if (cpl == 4) {
+      if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_flush<T, 4>), dim3(pl.nb_s), dim3(kBlock), 0, st, nA, p, s.R, pl.L, ring, active, P, Xo, sj, si, s);
+    } else {
+      hipLaunchKernelGGL((k_flush<T, 1>), dim3(pl.nb_s), dim3(kBlock), 0, st, nA, p, s.R, pl.L, ring, active, P, Xo, sj, si, s);
+    }
+    POLAR_LAUNCH_CHECK();
+  }
+  const int64_t tot = (int64_t)p * s.sld;
+  hipLaunchKernelGGL(k_coef_reset, dim3((int)std::min<int64_t>(1024, (tot + 255) / 256)), dim3(256), 0, st, s);
+  POLAR_LAUNCH_CHECK();
+  return 0;
 }
 
 template <typename T>
@@ -1151,6 +1318,10 @@ struct IterGraph {
   ~IterGraph() { drop(); }
 };
 
+// Window length of the shifted update: kW with POLAR_BLOCK8 (when the pole list fits the flush
+// kernel's LDS), else 1 (per-iteration updates).
+static int window_of(int flags, int p) { return ((flags & POLAR_BLOCK8) && p <= kMaxPolesFlush) ? kW : 1; }
+
 template <typename T>
 static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W, int64_t ldw, int64_t R,
                        const double* tau, int p, const int32_t* active, double tol, int64_t maxit, int flags,
@@ -1160,11 +1331,15 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   const int S = sg.S;
   const int64_t nA = sg.a[S];
   SolveMem<T> m;
-  const int64_t need = carve_solve<T>(ws, op, S, R, p, nA, m);
+  const int Tw = window_of(flags, p);
+  const int64_t need = carve_solve<T>(ws, op, S, R, p, nA, Tw, m);
   if (need > ws_bytes) return set_error(POLAR_E_ARG, "msolve workspace too small (%lld < %lld)", (long long)ws_bytes, (long long)need);
   CGState s = m.s;
-  T *r = m.r, *d = m.d, *v = m.v, *t2 = m.t2, *P = m.P, *lpart = m.lpart;
+  T *d = m.d, *v = m.v, *t2 = m.t2, *P = m.P, *lpart = m.lpart;
+  T* slot[kW];                       // residual ring: r_t of a window in slot[t % Tw]
+  for (int t = 0; t < kW; ++t) slot[t] = m.ring + (int64_t)(t % Tw) * n * R;
   int* perm = m.perm;
+  s.T = Tw;
   s.R = (int)R;
   s.ld = (int)R;
   s.ident = 1;
@@ -1204,17 +1379,23 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   const int64_t gpb0 = (int64_t)kWaves * pl.G;
 
   POLAR_HIP(hipMemsetAsync(s.flag, 0, 2 * sizeof(int), st));
-  POLAR_HIP(hipMemsetAsync(s.stats, 0, 2 * sizeof(unsigned long long), st));
+  POLAR_HIP(hipMemsetAsync(s.stats, 0, 3 * sizeof(unsigned long long), st));
+  if (Tw > 1) {
+    // every ring slot holds finite values before a flush reads it (with zero weight or not)
+    POLAR_HIP(hipMemsetAsync(m.ring, 0, sizeof(T) * (size_t)n * R * Tw, st));
+    hipLaunchKernelGGL(k_coef_reset, dim3((int)std::min<int64_t>(1024, ((int64_t)p * S * R + 255) / 256)), dim3(256), 0, st, s);
+    POLAR_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(k_iota, dim3(1), dim3(256), 0, st, s.colmap, (int)R);
   POLAR_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_shift, dim3((p * S + 255) / 256), dim3(256), 0, st, tau, p, S, s.shift);
   POLAR_LAUNCH_CHECK();
   if (cpl == 2) {
-    hipLaunchKernelGGL((k_init<T, 2>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, pl.L, W, ldw, r, d, s);
+    hipLaunchKernelGGL((k_init<T, 2>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, pl.L, W, ldw, slot[0], d, s);
   } else if (cpl == 4) {
-    if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_init<T, 4>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, pl.L, W, ldw, r, d, s);
+    if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_init<T, 4>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, pl.L, W, ldw, slot[0], d, s);
   } else {
-    hipLaunchKernelGGL((k_init<T, 1>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, pl.L, W, ldw, r, d, s);
+    hipLaunchKernelGGL((k_init<T, 1>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, pl.L, W, ldw, slot[0], d, s);
   }
   POLAR_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_init_fin, dim3(pl.fin), dim3(kBlock), 0, st, s, pl.grid_rows);
@@ -1252,6 +1433,13 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   const int esz = (int)sizeof(T);
   const int align = cpl0;        // compacted widths stay multiples of the initial vector width
   std::vector<int> acth(S * R), permh(R);
+  hipEvent_t evf[2] = {nullptr, nullptr};   // profiling of the window flushes
+  float flush_ms = 0.f;
+  auto ring_of = [&]() {
+    Ring<T> rg;
+    for (int t = 0; t < kW; ++t) rg.s[t] = slot[t];
+    return rg;
+  };
   bool converged = false;
   for (int64_t it0 = 0; it0 < maxit; it0 += 8) {
     const int nit = (int)std::min<int64_t>(8, maxit - it0);
@@ -1260,7 +1448,9 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
         POLAR_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
         int rc = 0;
         for (int k = 0; k < 8 && !rc; ++k)
-          rc = run_iter_any<T>(cpl, op, pl, tau0, r, d, v, lpart, P, Xo, sj, si, active, nA, p, s, k & 1, nullptr, cs);
+          rc = run_iter_any<T>(cpl, op, pl, tau0, slot[k % kW], slot[(k + 1) % kW], d, v, lpart, P, Xo, sj, si, active,
+                               nA, p, s, k & 1, k, nullptr, cs);
+        if (!rc && Tw > 1) rc = flush_window<T>(cpl, pl, ring_of(), active, nA, p, P, Xo, sj, si, s, cs);
         hipGraph_t gr = nullptr;
         const hipError_t ec = hipStreamEndCapture(cs, &gr);
         if (rc) {
@@ -1286,8 +1476,18 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
           evp = ev.data() + slot;
           widths.push_back(s.R);
         }
-        int rc = run_iter_any<T>(cpl, op, pl, tau0, r, d, v, lpart, P, Xo, sj, si, active, nA, p, s, (int)(it & 1), evp, st);
+        int rc = run_iter_any<T>(cpl, op, pl, tau0, slot[k % kW], slot[(k + 1) % kW], d, v, lpart, P, Xo, sj, si,
+                                 active, nA, p, s, (int)(it & 1), k, evp, st);
         if (rc) return rc;
+      }
+      if (Tw > 1) {
+        if (prof) {
+          for (auto& e : evf)
+            if (!e) POLAR_HIP(hipEventCreate(&e));
+          POLAR_HIP(hipEventRecord(evf[0], st));
+        }
+        if (int rc = flush_window<T>(cpl, pl, ring_of(), active, nA, p, P, Xo, sj, si, s, st)) return rc;
+        if (prof) POLAR_HIP(hipEventRecord(evf[1], st));
       }
     }
     int flag = 1;
@@ -1301,6 +1501,11 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
           POLAR_HIP(hipEventElapsedTime(&ms, ev[k * kEv + c], ev[k * kEv + c + 1]));
           kms.push_back(ms);
         }
+      if (Tw > 1) {
+        float ms = 0.f;
+        POLAR_HIP(hipEventElapsedTime(&ms, evf[0], evf[1]));
+        flush_ms += ms;
+      }
     }
     if (!flag) {
       converged = true;
@@ -1339,12 +1544,18 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
       POLAR_HIP(hipMemcpyAsync(perm, permh.data(), sizeof(int) * Rn, hipMemcpyHostToDevice, st));
       hipLaunchKernelGGL(k_writeback, dim3((p * s.R + 255) / 256), dim3(256), 0, st, s, (int)R, h_out, te_out, iters_out);
       POLAR_LAUNCH_CHECK();
+      // the current residual is slot[0] (a full window ends with r_8 there; Tw = 1: the residual)
       const int gv = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (n * Rn + kBlock - 1) / kBlock));
-      hipLaunchKernelGGL(k_compact_vec<T>, dim3(gv), dim3(kBlock), 0, st, n, (int64_t)R, Rn, (const int*)perm, (const T*)r, v);
+      hipLaunchKernelGGL(k_compact_vec<T>, dim3(gv), dim3(kBlock), 0, st, n, (int64_t)R, Rn, (const int*)perm, (const T*)slot[0], v);
       POLAR_LAUNCH_CHECK();
       hipLaunchKernelGGL(k_compact_vec<T>, dim3(gv), dim3(kBlock), 0, st, n, (int64_t)R, Rn, (const int*)perm, (const T*)d, t2);
       POLAR_LAUNCH_CHECK();
-      std::swap(r, v);
+      {
+        T* old0 = slot[0];
+        for (int t = 0; t < kW; ++t)
+          if (slot[t] == old0) slot[t] = v;
+        v = old0;
+      }
       std::swap(d, t2);
       if (int e = compact_rows<T>((int64_t)p * nA, R, Rn, perm, P, st)) return e;
       for (double* a : {s.rr, s.stop, s.a, s.be, s.ap, s.bp})
@@ -1368,7 +1579,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   if (prof) {
     // count only the iterations that did work (launches after convergence return at once)
     std::vector<int64_t> ih((size_t)S * s.R);
-    unsigned long long st2[2];
+    unsigned long long st2[3];
     if (S == 1) {
       POLAR_HIP(hipMemcpyAsync(ih.data(), s.iters, sizeof(int64_t) * s.R, hipMemcpyDeviceToHost, st));
     } else {
@@ -1395,11 +1606,22 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
     // segment's rows only, so the row streams are counted at the mean segment length
     const double nrows = (double)n / S, narows = (double)nA / S;
     g_prof[3].bytes += 3.0 * esz * nrows * (double)st2[0];
-    g_prof[5].bytes += esz * (3.0 * nrows + narows) * (double)st2[0] + 4.0 * esz * narows * (double)st2[1];
+    if (Tw > 1) {
+      // direction update per active column-iteration; per window the flush reads the T residuals
+      // of every column slot and reads and writes X, P of the (pole, column) pairs it changed
+      double slots = 0.0;
+      for (int w : widths) slots += (double)w * S;
+      g_prof[5].bytes += esz * 3.0 * nrows * (double)st2[0] + esz * narows * slots + 4.0 * esz * narows * (double)st2[2];
+      g_prof[5].ms += flush_ms;
+    } else {
+      g_prof[5].bytes += esz * (3.0 * nrows + narows) * (double)st2[0] + 4.0 * esz * narows * (double)st2[1];
+    }
     g_prof[6].launches += 1;
     g_prof[6].bytes += (double)st2[1];
     for (auto e : ev) (void)hipEventDestroy(e);
   }
+  for (auto e : evf)
+    if (e) (void)hipEventDestroy(e);
   hipLaunchKernelGGL(k_writeback, dim3((p * S * s.R + 255) / 256), dim3(256), 0, st, s, (int)R, h_out, te_out, iters_out);
   POLAR_LAUNCH_CHECK();
   if (!converged) return set_error(POLAR_E_ITERCAP, "multi-shift CG iteration cap");
@@ -1407,9 +1629,9 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
 }
 
 template <typename T>
-static int64_t msolve_ws(const polar_op* op, int S, int64_t R, int p, int64_t nA) {
+static int64_t msolve_ws(const polar_op* op, int S, int64_t R, int p, int64_t nA, int flags) {
   SolveMem<T> m;
-  return carve_solve<T>(nullptr, op, S, R, p, nA, m);
+  return carve_solve<T>(nullptr, op, S, R, p, nA, window_of(flags, p), m);
 }
 
 // Segment ranges from the caller's table, checked against the operator.
@@ -1498,9 +1720,8 @@ int polar_spmm(const polar_op* op, double shift, const void* X, int64_t ldx, voi
 }
 
 int64_t polar_msolve_ws_bytes(const polar_op* op, int64_t R, int32_t p, int64_t nA, int32_t flags) {
-  (void)flags;
   if (check_op(op)) return -1;
-  return op->dtype == POLAR_F64 ? msolve_ws<double>(op, 1, R, p, nA) : msolve_ws<float>(op, 1, R, p, nA);
+  return op->dtype == POLAR_F64 ? msolve_ws<double>(op, 1, R, p, nA, flags) : msolve_ws<float>(op, 1, R, p, nA, flags);
 }
 
 int polar_msolve(const polar_op* op, const void* W, int64_t ldw, int64_t R, const double* tau,
@@ -1520,12 +1741,11 @@ int polar_msolve(const polar_op* op, const void* W, int64_t ldw, int64_t R, cons
 
 int64_t polar_msolve_batched_ws_bytes(const polar_op* op, const polar_segs* segs, int64_t R, int32_t p,
                                       int32_t flags) {
-  (void)flags;
   if (check_op(op)) return -1;
   Segs sg;
   if (segs_from(op, segs, sg)) return -1;
   const int64_t nA = sg.a[sg.S];
-  return op->dtype == POLAR_F64 ? msolve_ws<double>(op, sg.S, R, p, nA) : msolve_ws<float>(op, sg.S, R, p, nA);
+  return op->dtype == POLAR_F64 ? msolve_ws<double>(op, sg.S, R, p, nA, flags) : msolve_ws<float>(op, sg.S, R, p, nA, flags);
 }
 
 int polar_msolve_batched(const polar_op* op, const polar_segs* segs, const void* W, int64_t ldw, int64_t R,

@@ -241,8 +241,13 @@ def _max_cols(dtype):
     return 128 if dtype == torch.float64 else 256
 
 
+def _blocked(block, dtype):
+    """Blocked shifted update: on by default for FP32, off for FP64 (the pinned path)."""
+    return (dtype == torch.float32) if block is None else bool(block)
+
+
 def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, freeze=True,
-           trace=False, out=None, width=None, graph=None):
+           trace=False, out=None, width=None, graph=None, block=None):
     """Block multi-shift CG for (M + tau_j I) U = W on all poles (polar_gpu.py:88-133).
 
     Returns (X[p, nA, R], h[p, R] or None, iters[R] host int64) and, with trace=True, a 4th value
@@ -250,7 +255,10 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
     into which X is written (the bank).  `width` caps the columns solved together (the workspace
     scales with it); columns are independent, so the results do not depend on it.  graph=True
     replays the CG iterations as hipGraphs of 8 iterations (same results, fewer launch gaps);
-    None takes the operator's tuned choice (op.use_graph, see polar_amd.autotune)."""
+    None takes the operator's tuned choice (op.use_graph, see polar_amd.autotune).  block=True
+    applies the shifted x/p updates once per 8 iterations from the stored residuals (same CG
+    scalars, counts and tracked norms; X within rounding; ~8x fewer passes over the shifted
+    blocks); None = on for FP32, off for FP64 (whose default path is the pinned one)."""
     lib = L.require_gpu()
     dev = _device()
     dtype = op.dtype
@@ -278,7 +286,8 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
     if graph is None:
         graph = getattr(op, 'use_graph', False)
     flags = ((L.POLAR_TRACK if track else 0) | (L.POLAR_TRACE if trace else 0) |
-             (0 if freeze else L.POLAR_NOFREEZE) | (L.POLAR_GRAPH if graph else 0))
+             (0 if freeze else L.POLAR_NOFREEZE) | (L.POLAR_GRAPH if graph else 0) |
+             (L.POLAR_BLOCK8 if _blocked(block, dtype) else 0))
     s = op.c_op(dtype)
     step = min(R, _max_cols(dtype), width or R)
     ws_bytes = lib.polar_msolve_ws_bytes(s, step, p, nA, flags)
@@ -434,20 +443,21 @@ def chunk_width(n, p, nA, budget=10e9):
     return int(max(8, min(cap, budget // per_col)))
 
 
-def bank_width(n, p, nA, q, dtype, budget=None):
+def bank_width(n, p, nA, q, dtype, budget=None, block=None):
     """Columns per bank block solve on one MI355X: as wide as one wave covers (128 FP64 / 256
     FP32 columns) within a memory budget for the shifted P block and the CG vectors."""
-    return fit_width(n, p, nA, 2 * q, dtype, budget)
+    return fit_width(n, p, nA, 2 * q, dtype, budget, block)
 
 
-def fit_width(n, p, nA, R, dtype, budget=None):
+def fit_width(n, p, nA, R, dtype, budget=None, block=None):
     """Columns (at most R) whose msolve workspace fits in `budget` bytes (default: half of the
-    free device memory): 4 n-row CG vectors and the shifted P block per column."""
+    free device memory): the n-row CG vectors (with the residual ring of the blocked update) and
+    the shifted P block per column."""
     esz = 8 if dtype == torch.float64 else 4
     if budget is None:
         free, _ = torch.cuda.mem_get_info()
         budget = 0.5 * free
-    per_col = esz * (4 * n + p * nA) + 64
+    per_col = esz * ((11 if _blocked(block, dtype) else 4) * n + p * nA) + 64
     w = int(min(R, _max_cols(dtype), max(1, budget // per_col)))
     if w >= 8:
         w -= w % 8
@@ -478,14 +488,15 @@ class _Phases:
 # ------------------------------------------------------------------------------------------------
 def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=None, naive=False,
           seed=2026, kind='block', trace=None, time_limit=None, poles=None, width=None,
-          freeze=True, precision='fp64', profile=False, progress=None, graph=None, tune=None):
+          freeze=True, precision='fp64', profile=False, progress=None, graph=None, tune=None, block=None):
     """POLAR greedy, cache backend (polar_gpu.py:180-254), on the GPU.
 
     profile=True adds `phases`: seconds per phase (bank, select, rerank_solve, rerank_pick,
     maintenance, update), measured with a device synchronisation at each phase boundary.
     graph=True replays the CG iterations of every solve as hipGraphs (same results).
     tune: a launch-shape dict from polar_amd.autotune (same results) or True to tune here (before
-    the timed region).
+    the timed region).  block: blocked shifted update in every solve (see msolve; None = FP32 on,
+    FP64 off).
     progress: optional callable(stage, info) called after the bank ('bank', {seconds, spmv}) and
     after each round ('round', {round, edge, marginal, seconds, spmv}); it reads host values only.
 
@@ -527,7 +538,8 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     # are independent, so the split does not change any result.
     esz = 8 if dtype == torch.float64 else 4
     free = torch.cuda.mem_get_info()[0]
-    joint = (p * nA * 2 * q + 2 * n * q + 2 * q * (4 * n + p * nA)) * esz < 0.8 * free
+    nvec = 11 if _blocked(block, dtype) else 4
+    joint = (p * nA * 2 * q + 2 * n * q + 2 * q * (nvec * n + p * nA)) * esz < 0.8 * free
     if joint:
         RHS = torch.empty((n, 2 * q), dtype=dtype, device=dev)
         Zb, Fb = RHS[:, :q], RHS[:, q:]
@@ -575,7 +587,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
 
     XY = torch.empty((p, nA, 2 * q), dtype=dtype, device=dev)
     if width is None:
-        width = bank_width(n, p, nA, q, dtype)          # sized on what is free after the caches
+        width = bank_width(n, p, nA, q, dtype, block=block)      # sized on what is free after the caches
 
     if joint:
         blocks = [(lambda: RHS, 0, 2 * q)]
@@ -587,7 +599,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
         chunks = [(src, c0, lo, min(c1, lo + width)) for src, c0, c1 in blocks for lo in range(c0, c1, width)]
         for src, c0, lo, hi in chunks:
             res = msolve(op, src()[:, lo - c0:hi - c0], tau, active, tol, counter=cnt, freeze=freeze,
-                         trace=want_trace and lo < q, out=XY[:, :, lo:hi], graph=graph)
+                         trace=want_trace and lo < q, out=XY[:, :, lo:hi], graph=graph, block=block)
             if want_trace and lo < q:
                 ncol = min(q, hi) - lo
                 terms.append((om[:, None] * res[3][:, :ncol].cpu().numpy()).ravel())
@@ -635,9 +647,9 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             if Ubuf is None:                # one re-rank block for all rounds
                 Ubuf = torch.empty((p, nA, max(B, 1)), dtype=dtype, device=dev)
             U = Ubuf[:, :, :Br]
-            rw = fit_width(n, p, nA, Br, dtype)
+            rw = fit_width(n, p, nA, Br, dtype, block=block)
             U, hh, _ = msolve(op, Wb, tau, active, tol, track=need_track, counter=cnt, freeze=freeze, out=U,
-                              width=rw, graph=graph)[:3]
+                              width=rw, graph=graph, block=block)[:3]
             ph.mark('rerank_solve')
             sa = torch.from_numpy(es_h[short, 0].astype(np.int32)).to(dev)
             sb = torch.from_numpy(es_h[short, 1].astype(np.int32)).to(dev)

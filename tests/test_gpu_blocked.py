@@ -1,0 +1,92 @@
+"""Blocked shifted update (POLAR_BLOCK8): X_j, P_j updated once per 8 iterations from the stored
+residuals with coefficients built from the same scalars.  The CG itself (alpha, beta, zeta,
+freezing, iteration counts, tracked norms, trace terms) is untouched, so those must be equal bit
+for bit; X must equal the per-iteration update within rounding (1e-12 relative in FP64), also for
+a partial last window, after column compaction, under hipGraph replay and for segments."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip('torch')
+pytestmark = pytest.mark.gpu
+
+from polar_amd import batched, graphs  # noqa: E402
+from polar_amd import polar_gpu as pg  # noqa: E402
+
+
+def _problem(n=3000, R=24, p=12, seed=0):
+    G = pg.Graph(n, graphs.chung_lu(n, 2.2, 10.0, 3 + seed), 0.05)
+    rng = np.random.default_rng(seed)
+    W = rng.standard_normal((n, R)) * rng.uniform(0.1, 10.0, R)
+    W[:, 3] = 0.0
+    W[:, 7] = 0.0
+    W[:, 7][rng.choice(n, 3)] = 1.0                          # a fast column: compaction happens
+    tau = pg.fixed_rule(0.5, p, G.sigma, G.Lam(4))[0]
+    act = np.sort(rng.choice(n, n // 2, replace=False))
+    return G, W, tau, act
+
+
+def _rel(a, b):
+    return float((a - b).abs().max() / b.abs().max())
+
+
+@pytest.mark.parametrize('graph', [False, True])
+def test_blocked_equals_per_iteration_fp64(graph):
+    G, W, tau, act = _problem()
+    op = pg.Op(G)
+    Wd = torch.from_numpy(W).cuda()
+    X0, h0, it0, te0 = pg.msolve(op, Wd, tau, act, 1e-10, track=True, trace=True, block=False)
+    X1, h1, it1, te1 = pg.msolve(op, Wd, tau, act, 1e-10, track=True, trace=True, block=True, graph=graph)
+    assert np.array_equal(it0, it1) and it0[3] == 0
+    assert torch.equal(h0, h1) and torch.equal(te0, te1)
+    assert _rel(X1, X0) <= 1e-12
+
+
+def test_blocked_partial_window_at_iteration_cap():
+    G, W, tau, act = _problem(R=8)
+    op = pg.Op(G)
+    Wd = torch.from_numpy(W).cuda()
+    outs = []
+    for block in (False, True):
+        X = torch.empty((len(tau), len(act), 8), dtype=torch.float64, device='cuda')
+        with pytest.raises(RuntimeError, match='iteration cap'):
+            pg.msolve(op, Wd, tau, act, 1e-14, maxit=13, block=block, out=X)
+        outs.append(X)
+    assert _rel(outs[1], outs[0]) <= 1e-12
+
+
+def test_blocked_fp32_close_to_fp64():
+    G, W, tau, act = _problem(R=16)
+    X64 = pg.msolve(pg.Op(G), torch.from_numpy(W).cuda(), tau, act, 1e-7)[0]
+    X32 = pg.msolve(pg.Op(G, precision='fp32'), torch.from_numpy(W).cuda().float(), tau, act, 1e-7, block=True)[0]
+    assert _rel(X32.double(), X64) <= 1e-4
+
+
+def test_blocked_segments_equal_single():
+    Gs = [_problem(n, R=8, seed=s)[0] for s, n in enumerate((1500, 3000, 700))]
+    U = batched.UnionGraph(Gs)
+    rng = np.random.default_rng(5)
+    Ws = [rng.standard_normal((G.n, 8)) for G in Gs]
+    acts = [np.sort(rng.choice(G.n, G.n // 3, replace=False)) for G in Gs]
+    tau = np.stack([pg.fixed_rule(s, 10, G.sigma, G.Lam(4))[0] for s, G in zip((0.25, 0.5, 0.75), Gs)])
+    a0 = np.r_[0, np.cumsum([len(a) for a in acts])]
+    segs = batched.Segments(*U.segments(a0))
+    active = torch.from_numpy(np.concatenate([a + U.row0[g] for g, a in enumerate(acts)]).astype(np.int32)).cuda()
+    X, h, it = batched.msolve_batched(pg.Op(U), segs, torch.from_numpy(np.concatenate(Ws)).cuda(), tau, active,
+                                      1e-10, track=True, block=True, graph=True)
+    for g, G in enumerate(Gs):
+        Xs, hs, its = pg.msolve(pg.Op(G), torch.from_numpy(Ws[g]).cuda(), tau[g], acts[g], 1e-10, track=True,
+                                block=True)
+        assert np.array_equal(it[g], its) and torch.equal(h[:, g], hs)
+        assert torch.equal(X[:, a0[g]:a0[g + 1]], Xs)
+
+
+def test_polar_blocked_fp64_same_greedy():
+    G = _problem()[0]
+    cand = graphs.candidates(G.n, G.edges, 500, 4)
+    poles = pg.fixed_rule(0.5, 12, G.sigma, G.Lam(5))[:2]
+    sk = pg.sketches(G, cand, 16, 3, kind='rademacher', on_device='lazy')
+    a = pg.polar(G, cand, 5, 0.5, q=16, B=4, sketch=sk, poles=poles)
+    b = pg.polar(G, cand, 5, 0.5, q=16, B=4, sketch=sk, poles=poles, block=True)
+    assert a['sequence'] == b['sequence'] and a['spmv'] == b['spmv']
+    assert np.max(np.abs(np.array(a['marginals']) - b['marginals']) / np.abs(a['marginals'])) <= 1e-10
+    assert np.max(np.abs(np.array(a['trace_estimate']) - b['trace_estimate']) / np.abs(a['trace_estimate'])) <= 1e-10
