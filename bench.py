@@ -4,8 +4,11 @@
 Metric (k-round greedy wall time + SpMM GB/s at n=4M; FP32-mixed vs FP64 speedup):
   * step = one full POLAR greedy workload of the c2 configuration: Chung-Lu n=100,000 (gamma 2.5,
     mean degree 20, seed 1), sigma=0.05, q=64 Rademacher, 24 poles, N=1,000,000 candidates,
-    k=20 rounds, B=8, for each s in {0.25, 0.5, 0.75}; inputs resident in HBM before timing.
-    `value` = wall seconds of one step in the FP32-mixed form (higher_is_better: false).
+    k=20 rounds, B=8, for each s in {0.25, 0.5, 0.75} (the three s values run as segments of one
+    batched solve per phase; --per-s runs them one after another); inputs resident in HBM.
+    `value` = seconds of one step in the FP32-mixed form, timed as the reference times `seconds`
+    (polar_gpu.py:196-253: after the inputs are resident, through the final sync);
+    `step_wall_s` is the wall clock around the whole call including its host-side setup.
   * the plain FP64 form is timed on the same inputs (fp64_seconds; speedup = fp64/fp32) and its
     selections and marginals are compared with the FP32-mixed ones.
   * SpMM (L + sigma I) X at the c5 shape (Chung-Lu n=4,000,000, m=34.7M, R=128) in FP64 and FP32,
@@ -38,12 +41,58 @@ def log(*a):
     print('[bench]', *a, file=sys.stderr, flush=True)
 
 
-def run_step(pg, G, inp, sk, precision):
+def traffic_of(kernel, workload):
+    """Measured HBM bytes per dispatch of `kernel` on `workload` from profiles/hbm_traffic.md (rows
+    `| kernel | workload | FETCH KiB | WRITE KiB | bytes |`, FETCH/WRITE scaled by the 1 GiB copy
+    probe), or None when that pair has not been measured."""
+    path = os.path.join(ROOT, 'profiles', 'hbm_traffic.md')
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip('|').split('|')]
+        if len(cells) >= 5 and cells[0] == kernel and cells[1] == workload:
+            try:
+                return float(cells[4])
+            except ValueError:
+                return None
+    return None
+
+
+def copy_probe(torch):
+    """Calibration probe for the HBM counters: one plain 1 GiB device copy (reads 1 GiB, writes
+    1 GiB); its FETCH_SIZE / WRITE_SIZE readings scale the other kernels' readings."""
+    src = torch.empty(1 << 28, dtype=torch.float32, device='cuda').fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    torch.cuda.empty_cache()
+
+
+GRAPH = True
+
+
+def step_seconds(outs, union):
+    """The step's timed region as the reference defines `seconds` (polar_gpu.py:196, 252-253): from
+    after the inputs are resident (sketch H2D, candidate endpoint table) through the final sync;
+    one batched region, or the sum over the per-s runs."""
+    return outs[0]['seconds'] if union is not None else float(sum(o['seconds'] for o in outs))
+
+
+def run_step(pg, G, inp, sk, precision, union=None):
+    """One step: the greedy for every s value.  With `union` (a UnionGraph of one copy of G per s)
+    the s values run as segments of one batched solve per phase (same results as one run per s)."""
+    poles = [pg.fixed_rule(s, inp['p'], inp['sigma'], G.Lam(inp['k']))[:2] for s in inp['s_all']]
+    if union is not None:
+        from polar_amd import batched
+        S = len(inp['s_all'])
+        return batched.polar_batched([G] * S, [inp['cand']] * S, inp['k'], list(inp['s_all']), q=inp['q'],
+                                     B=inp['B'], tol=inp['tol'], sketch=[sk] * S, poles=poles,
+                                     precision=precision, union=union, graph=GRAPH)
     outs = []
-    for s in inp['s_all']:
-        tau, om, _ = pg.fixed_rule(s, inp['p'], inp['sigma'], G.Lam(inp['k']))
+    for s, po in zip(inp['s_all'], poles):
         outs.append(pg.polar(G, inp['cand'], inp['k'], s, q=inp['q'], B=inp['B'], tol=inp['tol'], sketch=sk,
-                             poles=(tau, om), precision=precision))
+                             poles=po, precision=precision, graph=GRAPH))
     return outs
 
 
@@ -137,7 +186,13 @@ def main():
     ap.add_argument('--skip-cpu', action='store_true')
     ap.add_argument('--skip-spmm', action='store_true')
     ap.add_argument('--spmm-scale', type=float, default=1.0)
+    ap.add_argument('--per-s', action='store_true', help='one greedy run per s value (no batching over s)')
+    ap.add_argument('--graph', choices=['on', 'off'], default='on', help='hipGraph replay of the CG iterations')
+    ap.add_argument('--spmm-only', action='store_true',
+                    help='only the n=4M SpMM leg and the 1 GiB copy probe (for counter runs)')
     a = ap.parse_args()
+    global GRAPH
+    GRAPH = a.graph == 'on'
     if a.gpus != 1:
         print(f'bench.py measures one GPU only (asked for --gpus {a.gpus}); the pole-sharded multi-GPU '
               'layout is a design note, not built.', file=sys.stderr)
@@ -150,6 +205,11 @@ def main():
     lib = L.require_gpu()
     torch.cuda.set_device(0)
 
+    if a.spmm_only:
+        copy_probe(torch)
+        print(json.dumps(dict(spmm_n4m=spmm_bench(pg, torch, lib, L, a.spmm_scale))), flush=True)
+        return
+
     t = time.time()
     inp = configs.make(a.config, a.scale)
     G = pg.Graph(inp['n'], inp['edges'], inp['sigma'])
@@ -158,26 +218,27 @@ def main():
     log(f'{a.config}: n={inp["n"]} m={inp["m"]} N={inp["N"]} nA={len(np.unique(inp["cand"]))} '
         f'dmax={inp["dmax"]} inputs in {time.time() - t:.1f}s')
 
-    # calibration probe for HBM counters: one plain 1 GiB device copy (read 1 GiB, write 1 GiB)
-    src = torch.empty(1 << 28, dtype=torch.float32, device='cuda').fill_(1.0)
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    del src, dst
-    torch.cuda.empty_cache()
+    copy_probe(torch)
+    union = None
+    if not a.per_s and len(inp['s_all']) > 1:
+        from polar_amd import batched
+        union = batched.UnionGraph([G] * len(inp['s_all']))
 
     for w in range(a.warmup):
-        run_step(pg, G, inp, sk, 'fp32')
+        run_step(pg, G, inp, sk, 'fp32', union)
         log(f'warmup {w} done')
     lib.polar_prof_set(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     outs32 = None
+    timed = 0.0
     for i in range(a.steps):
-        outs32 = run_step(pg, G, inp, sk, 'fp32')
+        outs32 = run_step(pg, G, inp, sk, 'fp32', union)
+        timed += step_seconds(outs32, union)
         log(f'step {i} done')
     torch.cuda.synchronize()
-    el32 = (time.perf_counter() - t0) / a.steps
+    wall32 = (time.perf_counter() - t0) / a.steps
+    el32 = timed / a.steps
     prof = {}
     for cls, name in enumerate(KERNELS + ['solves']):
         ms, la, by = (np.zeros(1), np.zeros(1, np.int64), np.zeros(1))
@@ -189,12 +250,11 @@ def main():
     el64 = None
     agree = None
     if not a.skip_fp64:
-        run_step(pg, G, inp, sk, 'fp64') if a.warmup else None
+        run_step(pg, G, inp, sk, 'fp64', union) if a.warmup else None
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        outs64 = run_step(pg, G, inp, sk, 'fp64')
+        outs64 = run_step(pg, G, inp, sk, 'fp64', union)
         torch.cuda.synchronize()
-        el64 = time.perf_counter() - t0
+        el64 = step_seconds(outs64, union)
         same = all(o32['sequence'] == o64['sequence'] for o32, o64 in zip(outs32, outs64))
         mrel = max(float(np.max(np.abs(np.array(o32['marginals']) - o64['marginals']) / np.abs(o64['marginals'])))
                    for o32, o64 in zip(outs32, outs64))
@@ -212,8 +272,9 @@ def main():
     dom = max(KERNELS, key=lambda k: prof[k]['ms'])
     d = prof[dom]
     achieved = d['bytes'] / (d['ms'] * 1e-3) / 1e9 if d['ms'] > 0 else 0.0
+    wl = f'{a.config}-fp32' + ('' if a.scale == 1.0 else f'-x{a.scale}')
     roofline = dict(kernel=dom, bound='hbm', achieved=achieved, peak=HBM_PEAK_GBPS, unit='GB/s',
-                    frac=achieved / HBM_PEAK_GBPS, traffic=None,
+                    frac=achieved / HBM_PEAK_GBPS, traffic=traffic_of(dom, wl),
                     avg_launch_us=d['ms'] * 1e3 / max(d['launches'], 1),
                     algorithmic_bytes_per_launch=d['bytes'] / max(d['launches'], 1),
                     share_of_timed_region=d['ms'] * 1e-3 / (el32 * a.steps))
@@ -222,12 +283,15 @@ def main():
         cpu = cpu_baseline(inp, colits)
         log('cpu', json.dumps(cpu))
     out = dict(metric=METRIC, value=el32, unit='s', n_gpus=1, steps=a.steps, warmup=a.warmup,
-               ms_per_step=el32 * 1e3, higher_is_better=False, scaling='weak', vs_baseline=None,
+               ms_per_step=el32 * 1e3, step_wall_s=wall32, higher_is_better=False, scaling='weak', vs_baseline=None,
                dtype='fp32 vectors / fp64 recurrences', data='synthetic',
                config=dict(workload=f'{a.config}: Chung-Lu n={inp["n"]}, m={inp["m"]}, sigma={inp["sigma"]}, '
                                     f's={list(inp["s_all"])}, q={inp["q"]}, p={inp["p"]}, N={inp["N"]}, '
                                     f'k={inp["k"]}, B={inp["B"]}, tol={inp["tol"]}',
                            precision='fp32-mixed', column_iterations=colits,
+                           hipgraph=GRAPH,
+                           s_schedule='per s' if union is None else
+                           f'{len(inp["s_all"])} s values as segments of one batched solve per phase',
                            rhs=int(sum(o['rhs'] for o in outs32))),
                fp64_seconds=el64, fp32_vs_fp64_speedup=(el64 / el32) if el64 else None,
                fp32_vs_fp64=agree, spmm_n4m=spmm, roofline=roofline, cpu_baseline=cpu,

@@ -135,14 +135,16 @@ def _seq(x, S):
 
 def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac', sketch=None, seed=2026,
                   kind='rademacher', poles=None, precision='fp32', graph=None, width=None, progress=None,
-                  tune=None, block=None):
+                  tune=None, block=None, union=None):
     """POLAR greedy (polar_gpu.py:180-254, cache backend) on S graphs in lockstep.
 
     graphs: list of Graph; cands: list of (N_g, 2) candidate arrays; s: one value or one per graph;
     sketch: list of (Z, F, Gc) per graph (default: Rademacher with seed + g); poles: list of
     (tau, om) per graph (all with the same pole count) or None for `poles(obj, s_g, ...)`.
     graph: hipGraph replay (None: the tuned choice, else True); tune: a launch-shape dict from
-    polar_amd.autotune or True to tune the batched operator here (before the timed region).
+    polar_amd.autotune or True to tune the batched operator here (before the timed region);
+    union: a UnionGraph of these graphs built earlier (reused: its device arrays stay resident).
+    The same graph may appear several times (e.g. one graph under several s values).
     Returns one result dict per graph with the keys of polar_gpu.polar (`seconds` is the whole
     batch's wall time)."""
     lib = L.require_gpu()
@@ -152,7 +154,8 @@ def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac'
     stream = L.stream_ptr()
     S = len(graphs)
     s_all = _seq(s, S)
-    U = UnionGraph(graphs)
+    U = union if union is not None else UnionGraph(graphs)
+    assert len(U.graphs) == S and all(a is b for a, b in zip(U.graphs, graphs))
     row0 = U.row0
     objc = 2 if obj == 'logdet' else 0
     want_trace = obj != 'logdet'

@@ -1072,14 +1072,18 @@ static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int
   const int64_t C = (int64_t)S * R;
   s.rr = cv.take<double>(C); s.stop = cv.take<double>(C); s.a = cv.take<double>(C);
   s.be = cv.take<double>(C); s.ap = cv.take<double>(C); s.bp = cv.take<double>(C);
-  s.act = cv.take<int>(C); s.dmask = cv.take<int>(C); s.iters = cv.take<int64_t>(C);
+  // the two any-active flags and the per-column active mask are adjacent: the host reads them
+  // with one copy at each convergence check
+  int* fa = cv.take<int>(2 + C);
+  s.flag = fa;
+  s.act = fa + 2;
+  s.dmask = cv.take<int>(C); s.iters = cv.take<int64_t>(C);
   const int64_t pr = (int64_t)p * C;
   s.z = cv.take<double>(pr); s.zp = cv.take<double>(pr); s.zn = cv.take<double>(pr);
   s.aj = cv.take<double>(pr); s.bj = cv.take<double>(pr); s.h = cv.take<double>(pr);
   s.ip = cv.take<double>(pr); s.pn = cv.take<double>(pr); s.te = cv.take<double>(pr);
   s.tg = cv.take<double>(pr); s.shift = cv.take<double>((int64_t)p * S);
   s.live = cv.take<int>(pr); s.m = cv.take<int>(pr);
-  s.flag = cv.take<int>(2);
   s.stats = cv.take<unsigned long long>(3);
   s.parts = cv.take<double>(parts_rows(op, S) * 2 * R);
   if (Tw > 1) {
@@ -1287,6 +1291,24 @@ static int cpl_for(int64_t Rc, int64_t ld, int cpl_max) {
   return 1;
 }
 
+// Pinned host buffer of a thread for the convergence checks (a DMA copy, not a staged one).
+static int* status_buffer(int64_t count, int& rc) {
+  static thread_local int* buf = nullptr;
+  static thread_local int64_t cap = 0;
+  rc = 0;
+  if (count > cap) {
+    if (buf) (void)hipHostFree(buf);
+    buf = nullptr;
+    cap = 0;
+    void* p = nullptr;
+    rc = (int)hipHostMalloc(&p, sizeof(int) * count, hipHostMallocDefault);
+    if (rc) return nullptr;
+    buf = (int*)p;
+    cap = count;
+  }
+  return buf;
+}
+
 // Stream of a thread's captures (default priority; captured work runs on the caller's stream).
 static hipStream_t capture_stream(int& rc) {
   static thread_local hipStream_t cs = nullptr;
@@ -1432,7 +1454,14 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   std::vector<int> widths;       // per iteration: processed width
   const int esz = (int)sizeof(T);
   const int align = cpl0;        // compacted widths stay multiples of the initial vector width
-  std::vector<int> acth(S * R), permh(R);
+  std::vector<int> permh(R);
+  int* stat = nullptr;               // pinned: [flag parity 0, flag parity 1, act...]
+  {
+    int rc = 0;
+    stat = status_buffer(2 + (int64_t)S * R, rc);
+    if (rc) return hip_fail((hipError_t)rc, "hipHostMalloc", __LINE__);
+  }
+  const int* acth = stat + 2;
   hipEvent_t evf[2] = {nullptr, nullptr};   // profiling of the window flushes
   float flush_ms = 0.f;
   auto ring_of = [&]() {
@@ -1490,10 +1519,9 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
         if (prof) POLAR_HIP(hipEventRecord(evf[1], st));
       }
     }
-    int flag = 1;
-    POLAR_HIP(hipMemcpyAsync(&flag, s.flag + ((it0 + nit) & 1), sizeof(int), hipMemcpyDeviceToHost, st));
-    POLAR_HIP(hipMemcpyAsync(acth.data(), s.act, sizeof(int) * (S > 1 ? S * R : s.R), hipMemcpyDeviceToHost, st));
+    POLAR_HIP(hipMemcpyAsync(stat, s.flag, sizeof(int) * (2 + (S > 1 ? S * R : s.R)), hipMemcpyDeviceToHost, st));
     POLAR_HIP(hipStreamSynchronize(st));
+    const int flag = stat[(it0 + nit) & 1];
     if (prof) {
       for (int k = 0; k < nit; ++k)
         for (int c = 0; c < kEv - 1; ++c) {

@@ -456,12 +456,15 @@ def fit_width(n, p, nA, R, dtype, budget=None, block=None):
     esz = 8 if dtype == torch.float64 else 4
     if budget is None:
         free, _ = torch.cuda.mem_get_info()
-        budget = 0.5 * free
+        budget = 0.7 * free
     per_col = esz * ((11 if _blocked(block, dtype) else 4) * n + p * nA) + 64
     w = int(min(R, _max_cols(dtype), max(1, budget // per_col)))
-    if w >= 8:
-        w -= w % 8
-    return max(1, w)
+    # equal chunks (e.g. 2 x 64, not 120 + 8), multiples of 8 where possible
+    chunks = -(-R // w)
+    w = -(-R // chunks)
+    if w > 8 and w % 8 and (w + 7) // 8 * 8 * chunks - R < 8 * chunks and (w + 7) // 8 * 8 * per_col <= budget:
+        w = (w + 7) // 8 * 8
+    return max(1, min(w, R))
 
 
 def _sync():
