@@ -157,7 +157,9 @@ int polar_pair_wsum(int32_t dtype, const void* X, int64_t x_sj, int64_t x_si, in
                     int32_t accumulate, void* stream);
 
 /* score[e] = sum_j om_j h/(1 + t + (sigma + tau_j) h)  (obj 0 = frac/er), or
- * log1p(t[0,e] + sigma h[0,e]) (obj 2 = logdet); dead[e] != 0 -> -inf (polar_gpu.py:208-219). */
+ * log1p(t[0,e] + sigma h[0,e]) (obj 2 = logdet); dead[e] != 0 -> -inf (polar_gpu.py:208-219).
+ * obj 3: the first-order gain sum_j om_j h[j,e] = -d(sketched trace)/dw_e at w_e = 0 (the
+ * adjoint form's gradient shortlist; no counterpart in the reference). */
 int polar_proxy(int32_t obj, const double* h, const double* t, const double* tau, const double* om,
                 int32_t p, int64_t N, double sigma, const uint8_t* dead, double* score, void* stream);
 

@@ -103,6 +103,10 @@ __global__ __launch_bounds__(kBlock) void k_proxy(int obj, const double* __restr
     double sc;
     if (obj == 2) {
       sc = log1p(t[e] + sigma * h[e]);
+    } else if (obj == 3) {
+      // first-order gain: minus the derivative of the sketched trace at weight 0
+      sc = 0.0;
+      for (int j = 0; j < p; ++j) sc = sc + om[j] * h[(int64_t)j * N + e];
     } else {
       sc = 0.0;
       for (int j = 0; j < p; ++j) {

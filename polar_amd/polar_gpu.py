@@ -491,7 +491,8 @@ class _Phases:
 # ------------------------------------------------------------------------------------------------
 def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=None, naive=False,
           seed=2026, kind='block', trace=None, time_limit=None, poles=None, width=None,
-          freeze=True, precision='fp64', profile=False, progress=None, graph=None, tune=None, block=None):
+          freeze=True, precision='fp64', profile=False, progress=None, graph=None, tune=None, block=None,
+          shortlist='proxy'):
     """POLAR greedy, cache backend (polar_gpu.py:180-254), on the GPU.
 
     profile=True adds `phases`: seconds per phase (bank, select, rerank_solve, rerank_pick,
@@ -499,7 +500,9 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     graph=True replays the CG iterations of every solve as hipGraphs (same results).
     tune: a launch-shape dict from polar_amd.autotune (same results) or True to tune here (before
     the timed region).  block: blocked shifted update in every solve (see msolve; None = FP32 on,
-    FP64 off).
+    FP64 off).  shortlist='gradient' ranks candidates by the first-order gain sum_j om_j h[j,e]
+    (minus the sketched trace's derivative, see polar_amd.adjoint) instead of the reference's
+    proxy; the re-rank by exact marginals is unchanged.
     progress: optional callable(stage, info) called after the bank ('bank', {seconds, spmv}) and
     after each round ('round', {round, edge, marginal, seconds, spmv}); it reads host values only.
 
@@ -631,8 +634,8 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             X, Y, h, t, tr = bank()
             traces.append(tr if want_trace else None)
             ph.mark('bank')
-        L.check(lib.polar_proxy(objc, L.ptr(h), L.ptr(t), L.ptr(tau_d), L.ptr(om_d), p, N, sigma, L.ptr(dead),
-                                L.ptr(score), stream), 'proxy')
+        L.check(lib.polar_proxy(3 if shortlist == 'gradient' else objc, L.ptr(h), L.ptr(t), L.ptr(tau_d),
+                                L.ptr(om_d), p, N, sigma, L.ptr(dead), L.ptr(score), stream), 'proxy')
         Br = 1 if naive else min(B, int(live.sum()))
         short = np.zeros(Br, dtype=np.int32)
         L.check(lib.polar_select_topb(L.ptr(score), N, Br, L.ptr(sel_out), short.ctypes.data_as(L.vp),

@@ -71,3 +71,33 @@ def test_fp32_gradient_error():
     g64, g32 = g64.cpu().numpy(), g32.cpu().numpy()
     assert abs(T32 - T64) <= 1e-5 * abs(T64)
     assert np.max(np.abs(g32 - g64)) <= 1e-4 * np.max(np.abs(g64))
+
+
+def test_gradient_shortlist_is_first_round_shortlist():
+    """polar(shortlist='gradient') ranks round 1 by sum_j om_j h[j,e], the first-order gain: the
+    same order as -dT/dw over the candidates from the adjoint."""
+    n = 600
+    e = graphs.barabasi_albert(n, 3, 4)
+    cand = graphs.candidates(n, e, 80, 6)
+    sigma = 0.1
+    G = pg.Graph(n, e, sigma)
+    tau, om, _ = pg.fixed_rule(0.5, 10, sigma, G.Lam(3))
+    q = 16
+    sk = pg.sketches(G, cand, q, 7, kind='rademacher', on_device=True)
+    tr = []
+    out = pg.polar(G, cand, 3, 0.5, q=q, B=6, sketch=sk, poles=(tau, om), tol=1e-12, shortlist='gradient', trace=tr)
+    pairs = np.concatenate([e, cand])
+    wall = np.concatenate([np.ones(len(e)), np.zeros(len(cand))])
+    _, g = adjoint.sketched_trace(n, pairs, wall, sigma, tau, om, sk[0], tol=1e-12)
+    assert tr[0]['shortlist'] == adjoint.gradient_shortlist(g[len(e):], 6)
+    assert len(out['sequence']) == 3
+
+
+def test_run_adjoint_cli_small():
+    from polar_amd import run_adjoint
+    res = run_adjoint.main(['--scale', '0.004', '--samples', '24', '--k', '3', '--B', '4', '--precision', 'fp64',
+                            '--tol', '1e-11'])
+    assert res['check']['max_rel_err'] <= 1e-7
+    res32 = run_adjoint.main(['--scale', '0.004', '--samples', '24', '--skip-greedy'])
+    assert res32['check']['max_rel_err'] <= 1e-3
+    assert set(res['greedy']['runs']) == {'gradient', 'proxy'}
