@@ -70,6 +70,7 @@ def copy_probe(torch):
 
 
 GRAPH = True
+REPLACE = False
 
 
 def step_seconds(outs, union):
@@ -88,11 +89,12 @@ def run_step(pg, G, inp, sk, precision, union=None):
         S = len(inp['s_all'])
         return batched.polar_batched([G] * S, [inp['cand']] * S, inp['k'], list(inp['s_all']), q=inp['q'],
                                      B=inp['B'], tol=inp['tol'], sketch=[sk] * S, poles=poles,
-                                     precision=precision, union=union, graph=GRAPH)
+                                     precision=precision, union=union, graph=GRAPH,
+                                     replace=REPLACE and precision == 'fp32')
     outs = []
     for s, po in zip(inp['s_all'], poles):
         outs.append(pg.polar(G, inp['cand'], inp['k'], s, q=inp['q'], B=inp['B'], tol=inp['tol'], sketch=sk,
-                             poles=po, precision=precision, graph=GRAPH))
+                             poles=po, precision=precision, graph=GRAPH, replace=REPLACE and precision == 'fp32'))
     return outs
 
 
@@ -188,11 +190,13 @@ def main():
     ap.add_argument('--spmm-scale', type=float, default=1.0)
     ap.add_argument('--per-s', action='store_true', help='one greedy run per s value (no batching over s)')
     ap.add_argument('--graph', choices=['on', 'off'], default='on', help='hipGraph replay of the CG iterations')
+    ap.add_argument('--replace', action='store_true', help='FP64 residual replacement in the FP32-mixed step')
     ap.add_argument('--spmm-only', action='store_true',
                     help='only the n=4M SpMM leg and the 1 GiB copy probe (for counter runs)')
     a = ap.parse_args()
-    global GRAPH
+    global GRAPH, REPLACE
     GRAPH = a.graph == 'on'
+    REPLACE = a.replace
     if a.gpus != 1:
         print(f'bench.py measures one GPU only (asked for --gpus {a.gpus}); the pole-sharded multi-GPU '
               'layout is a design note, not built.', file=sys.stderr)
@@ -289,7 +293,7 @@ def main():
                                     f's={list(inp["s_all"])}, q={inp["q"]}, p={inp["p"]}, N={inp["N"]}, '
                                     f'k={inp["k"]}, B={inp["B"]}, tol={inp["tol"]}',
                            precision='fp32-mixed', column_iterations=colits,
-                           hipgraph=GRAPH,
+                           hipgraph=GRAPH, residual_replacement=REPLACE,
                            s_schedule='per s' if union is None else
                            f'{len(inp["s_all"])} s values as segments of one batched solve per phase',
                            rhs=int(sum(o['rhs'] for o in outs32))),

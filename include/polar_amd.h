@@ -51,6 +51,8 @@ enum polar_status {
 #define POLAR_BLOCK8 16   /* shifted x/p updates once per 8 iterations from the stored residuals:
                              same CG scalars and iteration counts, X within rounding of the
                              per-iteration updates, about 8x fewer passes over the shifted blocks */
+#define POLAR_REPLACE 32  /* residual replacement every 24 iterations: r = w - (M + tau0) x0 with the
+                             seed iterate x0 kept in FP64 and the product in FP64 (for FP32 vectors) */
 
 /* Operator y = (diag + shift) x - A_w x + patch(x): the graph Laplacian M0 = diag(deg+sigma) - A
  * of realworld/polar_gpu.py:47-55 stored as diagonal + off-diagonal adjacency (value-free when

@@ -72,7 +72,7 @@ class Segments:
 
 
 def msolve_batched(op, segs, W, tau, active, tol, maxit=100000, track=False, trace=False, out=None,
-                   width=None, graph=None, freeze=True, block=None):
+                   width=None, graph=None, freeze=True, block=None, replace=False):
     """Batched block multi-shift CG: segment g solves (M_g + tau[g, j]) U = W[rows of g, :R].
 
     op: Op of a UnionGraph; segs: Segments; W: (n, R) device; tau: (S, p) host or device; active:
@@ -98,7 +98,7 @@ def msolve_batched(op, segs, W, tau, active, tol, maxit=100000, track=False, tra
         graph = getattr(op, 'use_graph', False)
     flags = ((L.POLAR_TRACK if track else 0) | (L.POLAR_TRACE if trace else 0) |
              (0 if freeze else L.POLAR_NOFREEZE) | (L.POLAR_GRAPH if graph else 0) |
-             (L.POLAR_BLOCK8 if pg._blocked(block, dtype) else 0))
+             (L.POLAR_BLOCK8 if pg._blocked(block, dtype) else 0) | (L.POLAR_REPLACE if replace else 0))
     s = op.c_op(dtype)
     step = min(R, pg._max_cols(dtype), width or R)
     ws_bytes = lib.polar_msolve_batched_ws_bytes(s, segs.c, step, p, flags)
@@ -135,7 +135,7 @@ def _seq(x, S):
 
 def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac', sketch=None, seed=2026,
                   kind='rademacher', poles=None, precision='fp32', graph=None, width=None, progress=None,
-                  tune=None, block=None, union=None):
+                  tune=None, block=None, union=None, replace=False):
     """POLAR greedy (polar_gpu.py:180-254, cache backend) on S graphs in lockstep.
 
     graphs: list of Graph; cands: list of (N_g, 2) candidate arrays; s: one value or one per graph;
@@ -241,7 +241,7 @@ def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac'
     for lo in range(0, 2 * q, width):
         hi = min(2 * q, lo + width)
         res = msolve_batched(op, segs, RHS[:, lo:hi], tau_all, active, tol, trace=want_trace and lo < q,
-                             out=XY[:, :, lo:hi], graph=graph, block=block)
+                             out=XY[:, :, lo:hi], graph=graph, block=block, replace=replace)
         spmv += res[2].sum(axis=1)
         rhs += hi - lo
         if want_trace and lo < q:
@@ -294,7 +294,7 @@ def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac'
         Wb.index_put_((ia, ic), torch.ones(len(ic), dtype=dtype, device=dev))
         Wb.index_put_((ib, ic), torch.full((len(ic),), -1.0, dtype=dtype, device=dev))
         Ub, hh, it = msolve_batched(op, segs, Wb, tau_all, active, tol, track=obj != 'logdet', graph=graph,
-                                    block=block)[:3]
+                                    block=block, replace=replace)[:3]
         spmv += it.sum(axis=1)
         for g in range(S):
             Br = len(shorts[g])

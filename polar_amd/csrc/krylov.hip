@@ -230,6 +230,10 @@ struct CGState {
   int T;
   double *cX, *cP;            // [p][T+1][sld]
   int* dirty;                 // [p][sld] (pole, column) changed in the current window
+  // Residual replacement: the seed iterate x0 (n x ld, FP64) accumulated beside the recurrences;
+  // every few windows the residual becomes w - (M + tau0) x0, computed in FP64 from the
+  // operator values the solve uses.  NULL when off.
+  double* x0;
 };
 
 // Segment of block b of a kernel whose blocks are partitioned by the prefix table `pref`.
@@ -558,7 +562,8 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
 // Kernel B: rn = r - a v (active columns) and |rn|^2 partials (finished by k_scalars).
 template <typename T, int CPL>
 __global__ __launch_bounds__(kBlock) void k_resid(int64_t n, int R, int L, const T* rin, T* rout,
-                                                  const T* __restrict__ v, CGState s, int cur) {
+                                                  const T* __restrict__ v, const T* __restrict__ d, CGState s,
+                                                  int cur) {
   // the finisher of this iteration raises flag[cur^1] again if a column is still active; when
   // converged, both parities stay 0 so every later launch is a no-op
   if (blockIdx.x == 0) {
@@ -597,6 +602,14 @@ __global__ __launch_bounds__(kBlock) void k_resid(int64_t n, int R, int L, const
 #pragma unroll
         for (int e = 0; e < CPL; ++e)
           if (act[e]) rv[e] = rv[e] - (a[e] * vv[e]);
+        if (s.x0) {                          // x0 += alpha d (FP64), for residual replacement
+          T dv[CPL];
+          vload<T, CPL>(d + i * s.ld + c0, dv);
+          double* xp = s.x0 + i * s.ld + c0;
+#pragma unroll
+          for (int e = 0; e < CPL; ++e)
+            if (act[e]) xp[e] = xp[e] + s.a[vb + c0 + e] * (double)dv[e];
+        }
       }
       if (any || rin != rout) vstore<T, CPL>(rout + i * s.ld + c0, rv);
 #pragma unroll
@@ -853,6 +866,86 @@ __global__ void k_coef_reset(CGState s) {
   if (nd) atomicAdd(&s.stats[2], nd);
 }
 
+// Residual replacement: r = w - (M + tau0) x0 on the rows of live segments, in FP64 from the
+// operator's own values (the product is the one the recurrences approximate), for active
+// columns; |r|^2 partials for k_replace_fin.  Each row is summed by one lane group.
+template <typename T, int CPL, bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void k_replace(polar_op op, double shift, const T* __restrict__ W,
+                                                    int64_t ldw, T* __restrict__ r, int R, int L, CGState s,
+                                                    int cur) {
+  const SegPart sp = seg_part(s.bR, s.S, blockIdx.x, gridDim.x);
+  if (s.flag[cur] == 0 || !seg_live(s, cur, sp.g)) return;
+  if (s.S > 1) shift = s.tau[sp.g * s.p];
+  int64_t r0, r1;
+  seg_range(s.seg_row, s.S, sp.g, op.n, r0, r1);
+  const int vb = sp.g * s.ld;
+  Geo g(L);
+  const int c0 = g.li * CPL;
+  const bool colok = c0 < R;
+  DD dacc[CPL];
+#pragma unroll
+  for (int e = 0; e < CPL; ++e) dacc[e] = DD{0.0, 0.0};
+  if (colok) {
+    bool act[CPL];
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) act[e] = s.act[vb + c0 + e] != 0;
+    const int gpb = kWaves * g.G;
+    int pc = -1;
+    for (int64_t i = r0 + (int64_t)sp.rank * gpb + g.wave * g.G + g.gi; i < r1; i += (int64_t)sp.nb * gpb) {
+      const int row = (int)i;
+      double acc[CPL];
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) acc[e] = 0.0;
+      const int beg = op.indptr[row], end = op.indptr[row + 1];
+      for (int k = beg; k < end; ++k) {
+        const int64_t j = op.indices[k];
+        const double wk = WEIGHTED ? (double)((const T*)op.weights)[k] : 1.0;
+        const double* xj = s.x0 + j * s.ld + c0;
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) acc[e] = acc[e] + wk * xj[e];
+      }
+      const double* xr = s.x0 + i * s.ld + c0;
+      const double dg = (double)((const T*)op.diag)[row] + shift;
+      double y[CPL];
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) y[e] = dg * xr[e] - acc[e];
+      if (op.n_patch > 0) {
+        if (pc < 0) pc = lower_bound_i32(op.patch_rows, op.n_patch, row);
+        while (pc < op.n_patch && op.patch_rows[pc] < row) ++pc;
+        if (pc < op.n_patch && op.patch_rows[pc] == row) {
+          for (int q = op.patch_ptr[pc]; q < op.patch_ptr[pc + 1]; ++q) {
+            const double* xq = s.x0 + (int64_t)op.patch_cols[q] * s.ld + c0;
+#pragma unroll
+            for (int e = 0; e < CPL; ++e) y[e] = y[e] + (xr[e] - xq[e]);
+          }
+        }
+      }
+      T wv[CPL], rv[CPL];
+      vload<T, CPL>(W + i * ldw + c0, wv);
+      vload<T, CPL>(r + i * s.ld + c0, rv);
+#pragma unroll
+      for (int e = 0; e < CPL; ++e)
+        if (act[e]) rv[e] = (T)((double)wv[e] - y[e]);
+      vstore<T, CPL>(r + i * s.ld + c0, rv);
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) dd_add(dacc[e], (double)rv[e] * (double)rv[e]);
+    }
+  }
+  block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)blockIdx.x * 2 * R);
+}
+
+// After a replacement: rr of every active (segment, column) is the replaced residual's norm.
+__global__ __launch_bounds__(kBlock) void k_replace_fin(CGState s, int nparts, int cur) {
+  if (s.flag[cur] == 0) return;
+  const int g = blockIdx.x / s.R, c = blockIdx.x - g * s.R;
+  if (!seg_live(s, cur, g)) return;
+  const int vc = g * s.ld + c;
+  int b0, b1;
+  rows_parts(s, g, nparts, b0, b1);
+  const double rr = col_total(s.parts, b0, b1, 0, 0, s.R, c);
+  if (threadIdx.x == 0 && s.act[vc]) s.rr[vc] = rr;
+}
+
 // Init: r = d = W[:, :R]; |w|^2 partials; last block initialises every scalar.
 template <typename T, int CPL>
 __global__ __launch_bounds__(kBlock) void k_init(int64_t n, int R, int L, const T* __restrict__ W,
@@ -1066,7 +1159,7 @@ struct SolveMem {
 
 template <typename T>
 static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int p, int64_t nA, int Tw,
-                           SolveMem<T>& m) {
+                           bool repl, SolveMem<T>& m) {
   Carver cv(base);
   CGState& s = m.s;
   const int64_t C = (int64_t)S * R;
@@ -1091,6 +1184,7 @@ static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int
     s.cP = cv.take<double>(pr * (Tw + 1));
     s.dirty = cv.take<int>(pr);
   }
+  if (repl) s.x0 = cv.take<double>(op->n * R);
   s.colmap = cv.take<int>(R);
   s.sflag = cv.take<int>(2 * S);
   m.dtab = cv.take<int>(5 * (S + 1));
@@ -1226,7 +1320,8 @@ static int run_iter(const polar_op* op, const Plan& pl, double tau0, const T* ri
   hipLaunchKernelGGL(k_alpha, dim3(pl.fin), dim3(kBlock), 0, st, s, nparts_a, cur);
   POLAR_LAUNCH_CHECK();
   if (ev) POLAR_HIP(hipEventRecord(ev[3], st));
-  hipLaunchKernelGGL((k_resid<T, CPL>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, R, pl.L, rin, rout, (const T*)v, s, cur);
+  hipLaunchKernelGGL((k_resid<T, CPL>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, R, pl.L, rin, rout, (const T*)v,
+                     (const T*)d, s, cur);
   POLAR_LAUNCH_CHECK();
   if (ev) POLAR_HIP(hipEventRecord(ev[4], st));
   hipLaunchKernelGGL(k_scalars, dim3(pl.fin), dim3(kBlock), 0, st, s, pl.grid_rows, cur, wt);
@@ -1253,6 +1348,32 @@ static int run_iter_any(int cpl, const polar_op* op, const Plan& pl, double tau0
     if (cpl == 4) return run_iter<T, 4>(op, pl, tau0, rin, rout, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, wt, ev, st);
   }
   return run_iter<T, 1>(op, pl, tau0, rin, rout, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, wt, ev, st);
+}
+
+// Residual replacement on the current residual r (after a window's flush).
+template <typename T>
+static int replace_residual(int cpl, const polar_op* op, const Plan& pl, double tau0, const T* W, int64_t ldw, T* r,
+                            const CGState& s, int cur, hipStream_t st) {
+  const bool wt = op->weighted != 0;
+#define POLAR_REPL(C)                                                                                              \
+  if (wt)                                                                                                        \
+    hipLaunchKernelGGL((k_replace<T, C, true>), dim3(pl.grid_rows), dim3(kBlock), 0, st, *op, tau0, W, ldw, r, s.R, \
+                       pl.L, s, cur);                                                                            \
+  else                                                                                                           \
+    hipLaunchKernelGGL((k_replace<T, C, false>), dim3(pl.grid_rows), dim3(kBlock), 0, st, *op, tau0, W, ldw, r, s.R, \
+                       pl.L, s, cur);
+  if (cpl == 2) {
+    POLAR_REPL(2)
+  } else if (cpl == 4) {
+    if constexpr (sizeof(T) == 4) { POLAR_REPL(4) }
+  } else {
+    POLAR_REPL(1)
+  }
+#undef POLAR_REPL
+  POLAR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_replace_fin, dim3(pl.fin), dim3(kBlock), 0, st, s, pl.grid_rows, cur);
+  POLAR_LAUNCH_CHECK();
+  return 0;
 }
 
 // End of a window of the blocked update: apply the coefficients to X, P, then start a new window.
@@ -1344,6 +1465,8 @@ struct IterGraph {
 // kernel's LDS), else 1 (per-iteration updates).
 static int window_of(int flags, int p) { return ((flags & POLAR_BLOCK8) && p <= kMaxPolesFlush) ? kW : 1; }
 
+constexpr int kReplaceEvery = 24;   // residual replacement period (iterations; whole windows)
+
 template <typename T>
 static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W, int64_t ldw, int64_t R,
                        const double* tau, int p, const int32_t* active, double tol, int64_t maxit, int flags,
@@ -1354,7 +1477,8 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   const int64_t nA = sg.a[S];
   SolveMem<T> m;
   const int Tw = window_of(flags, p);
-  const int64_t need = carve_solve<T>(ws, op, S, R, p, nA, Tw, m);
+  const bool repl = (flags & POLAR_REPLACE) != 0;
+  const int64_t need = carve_solve<T>(ws, op, S, R, p, nA, Tw, repl, m);
   if (need > ws_bytes) return set_error(POLAR_E_ARG, "msolve workspace too small (%lld < %lld)", (long long)ws_bytes, (long long)need);
   CGState s = m.s;
   T *d = m.d, *v = m.v, *t2 = m.t2, *P = m.P, *lpart = m.lpart;
@@ -1402,6 +1526,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
 
   POLAR_HIP(hipMemsetAsync(s.flag, 0, 2 * sizeof(int), st));
   POLAR_HIP(hipMemsetAsync(s.stats, 0, 3 * sizeof(unsigned long long), st));
+  if (repl) POLAR_HIP(hipMemsetAsync(s.x0, 0, sizeof(double) * (size_t)n * R, st));
   if (Tw > 1) {
     // every ring slot holds finite values before a flush reads it (with zero weight or not)
     POLAR_HIP(hipMemsetAsync(m.ring, 0, sizeof(T) * (size_t)n * R * Tw, st));
@@ -1539,6 +1664,10 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
       converged = true;
       break;
     }
+    if (repl && nit == 8 && (it0 + 8) % kReplaceEvery == 0) {
+      // the current residual is slot[0] after a full window (Tw = 8) or the only vector (Tw = 1)
+      if (int rc = replace_residual<T>(cpl, op, pl, tau0, W, ldw, slot[0], s, (int)((it0 + 8) & 1), st)) return rc;
+    }
     if (S > 1) {
       // Re-plan over the segments still iterating: finished segments get no blocks, the others
       // share the whole budget (their kernels skip finished segments either way).
@@ -1585,6 +1714,8 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
         v = old0;
       }
       std::swap(d, t2);
+      if (repl)
+        if (int e = compact_rows<double>(n, R, Rn, perm, s.x0, st)) return e;
       if (int e = compact_rows<T>((int64_t)p * nA, R, Rn, perm, P, st)) return e;
       for (double* a : {s.rr, s.stop, s.a, s.be, s.ap, s.bp})
         if (int e = compact_rows<double>(1, R, Rn, perm, a, st)) return e;
@@ -1659,7 +1790,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
 template <typename T>
 static int64_t msolve_ws(const polar_op* op, int S, int64_t R, int p, int64_t nA, int flags) {
   SolveMem<T> m;
-  return carve_solve<T>(nullptr, op, S, R, p, nA, window_of(flags, p), m);
+  return carve_solve<T>(nullptr, op, S, R, p, nA, window_of(flags, p), (flags & POLAR_REPLACE) != 0, m);
 }
 
 // Segment ranges from the caller's table, checked against the operator.

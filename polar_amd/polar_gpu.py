@@ -247,7 +247,7 @@ def _blocked(block, dtype):
 
 
 def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, freeze=True,
-           trace=False, out=None, width=None, graph=None, block=None):
+           trace=False, out=None, width=None, graph=None, block=None, replace=False):
     """Block multi-shift CG for (M + tau_j I) U = W on all poles (polar_gpu.py:88-133).
 
     Returns (X[p, nA, R], h[p, R] or None, iters[R] host int64) and, with trace=True, a 4th value
@@ -258,7 +258,9 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
     None takes the operator's tuned choice (op.use_graph, see polar_amd.autotune).  block=True
     applies the shifted x/p updates once per 8 iterations from the stored residuals (same CG
     scalars, counts and tracked norms; X within rounding; ~8x fewer passes over the shifted
-    blocks); None = on for FP32, off for FP64 (whose default path is the pinned one)."""
+    blocks); None = on for FP32, off for FP64 (whose default path is the pinned one).
+    replace=True: residual replacement every 24 iterations (r = w - (M + tau0) x0 with the seed
+    iterate x0 accumulated in FP64 and the product in FP64)."""
     lib = L.require_gpu()
     dev = _device()
     dtype = op.dtype
@@ -287,7 +289,7 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
         graph = getattr(op, 'use_graph', False)
     flags = ((L.POLAR_TRACK if track else 0) | (L.POLAR_TRACE if trace else 0) |
              (0 if freeze else L.POLAR_NOFREEZE) | (L.POLAR_GRAPH if graph else 0) |
-             (L.POLAR_BLOCK8 if _blocked(block, dtype) else 0))
+             (L.POLAR_BLOCK8 if _blocked(block, dtype) else 0) | (L.POLAR_REPLACE if replace else 0))
     s = op.c_op(dtype)
     step = min(R, _max_cols(dtype), width or R)
     ws_bytes = lib.polar_msolve_ws_bytes(s, step, p, nA, flags)
@@ -492,7 +494,7 @@ class _Phases:
 def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=None, naive=False,
           seed=2026, kind='block', trace=None, time_limit=None, poles=None, width=None,
           freeze=True, precision='fp64', profile=False, progress=None, graph=None, tune=None, block=None,
-          shortlist='proxy'):
+          shortlist='proxy', replace=False):
     """POLAR greedy, cache backend (polar_gpu.py:180-254), on the GPU.
 
     profile=True adds `phases`: seconds per phase (bank, select, rerank_solve, rerank_pick,
@@ -502,7 +504,8 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     the timed region).  block: blocked shifted update in every solve (see msolve; None = FP32 on,
     FP64 off).  shortlist='gradient' ranks candidates by the first-order gain sum_j om_j h[j,e]
     (minus the sketched trace's derivative, see polar_amd.adjoint) instead of the reference's
-    proxy; the re-rank by exact marginals is unchanged.
+    proxy; the re-rank by exact marginals is unchanged.  replace=True: FP64 residual replacement
+    every 24 iterations in every solve (see msolve).
     progress: optional callable(stage, info) called after the bank ('bank', {seconds, spmv}) and
     after each round ('round', {round, edge, marginal, seconds, spmv}); it reads host values only.
 
@@ -605,7 +608,8 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
         chunks = [(src, c0, lo, min(c1, lo + width)) for src, c0, c1 in blocks for lo in range(c0, c1, width)]
         for src, c0, lo, hi in chunks:
             res = msolve(op, src()[:, lo - c0:hi - c0], tau, active, tol, counter=cnt, freeze=freeze,
-                         trace=want_trace and lo < q, out=XY[:, :, lo:hi], graph=graph, block=block)
+                         trace=want_trace and lo < q, out=XY[:, :, lo:hi], graph=graph, block=block,
+                         replace=replace)
             if want_trace and lo < q:
                 ncol = min(q, hi) - lo
                 terms.append((om[:, None] * res[3][:, :ncol].cpu().numpy()).ravel())
@@ -655,7 +659,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             U = Ubuf[:, :, :Br]
             rw = fit_width(n, p, nA, Br, dtype, block=block)
             U, hh, _ = msolve(op, Wb, tau, active, tol, track=need_track, counter=cnt, freeze=freeze, out=U,
-                              width=rw, graph=graph, block=block)[:3]
+                              width=rw, graph=graph, block=block, replace=replace)[:3]
             ph.mark('rerank_solve')
             sa = torch.from_numpy(es_h[short, 0].astype(np.int32)).to(dev)
             sb = torch.from_numpy(es_h[short, 1].astype(np.int32)).to(dev)

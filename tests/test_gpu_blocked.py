@@ -90,3 +90,27 @@ def test_polar_blocked_fp64_same_greedy():
     assert a['sequence'] == b['sequence'] and a['spmv'] == b['spmv']
     assert np.max(np.abs(np.array(a['marginals']) - b['marginals']) / np.abs(a['marginals'])) <= 1e-10
     assert np.max(np.abs(np.array(a['trace_estimate']) - b['trace_estimate']) / np.abs(a['trace_estimate'])) <= 1e-10
+
+
+@pytest.mark.parametrize('block', [False, True])
+def test_residual_replacement_fp32(block):
+    """FP32 vectors with FP64 residual replacement every 24 iterations: the solve still converges,
+    and its X is at least as close to the FP64 solution as without replacement (1e-4 bound)."""
+    G, W, tau, act = _problem(R=16)
+    X64 = pg.msolve(pg.Op(G), torch.from_numpy(W).cuda(), tau, act, 1e-9)[0]
+    op32 = pg.Op(G, precision='fp32')
+    W32 = torch.from_numpy(W).cuda().float()
+    Xp, _, itp = pg.msolve(op32, W32, tau, act, 1e-7, block=block)
+    Xr, _, itr = pg.msolve(op32, W32, tau, act, 1e-7, block=block, replace=True)
+    ep, er = _rel(Xp.double(), X64), _rel(Xr.double(), X64)
+    assert er <= 1e-4 and er <= 2 * ep, (ep, er)
+    assert np.all(itr >= 0) and itr.max() < 5 * max(itp.max(), 1)
+
+
+def test_residual_replacement_fp64_matches_pinned_within_tolerance():
+    G, W, tau, act = _problem(R=8)
+    op = pg.Op(G)
+    Wd = torch.from_numpy(W).cuda()
+    X0 = pg.msolve(op, Wd, tau, act, 1e-10)[0]
+    X1 = pg.msolve(op, Wd, tau, act, 1e-10, replace=True)[0]
+    assert _rel(X1, X0) <= 1e-8
