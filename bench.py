@@ -4,8 +4,9 @@
 Metric (k-round greedy wall time + SpMM GB/s at n=4M; FP32-mixed vs FP64 speedup):
   * step = one full POLAR greedy workload of the c2 configuration: Chung-Lu n=100,000 (gamma 2.5,
     mean degree 20, seed 1), sigma=0.05, q=64 Rademacher, 24 poles, N=1,000,000 candidates,
-    k=20 rounds, B=8, for each s in {0.25, 0.5, 0.75} (the three s values run as segments of one
-    batched solve per phase; --per-s runs them one after another); inputs resident in HBM.
+    k=20 rounds, B=8, for each s in {0.25, 0.5, 0.75} (the three s values run as column groups of
+    one operator, so every phase is one solve over one adjacency pass; --schedule segments|per-s
+    for the alternatives); inputs resident in HBM.
     `value` = seconds of one step in the FP32-mixed form, timed as the reference times `seconds`
     (polar_gpu.py:196-253: after the inputs are resident, through the final sync);
     `step_wall_s` is the wall clock around the whole call including its host-side setup.
@@ -77,13 +78,24 @@ def step_seconds(outs, union):
     """The step's timed region as the reference defines `seconds` (polar_gpu.py:196, 252-253): from
     after the inputs are resident (sketch H2D, candidate endpoint table) through the final sync;
     one batched region, or the sum over the per-s runs."""
-    return outs[0]['seconds'] if union is not None else float(sum(o['seconds'] for o in outs))
+    batched = union is not None or (SCHEDULE == 'columns' and len(outs) > 1)
+    return outs[0]['seconds'] if batched else float(sum(o['seconds'] for o in outs))
+
+
+SCHEDULE = 'columns'
 
 
 def run_step(pg, G, inp, sk, precision, union=None):
-    """One step: the greedy for every s value.  With `union` (a UnionGraph of one copy of G per s)
-    the s values run as segments of one batched solve per phase (same results as one run per s)."""
+    """One step: the greedy for every s value.  Schedules (same results):
+    'columns'  -- the s values are column groups of one operator (one adjacency pass per phase);
+    'segments' -- `union` (a UnionGraph of one copy of G per s) runs them as row segments;
+    'per-s'    -- one greedy run after another."""
     poles = [pg.fixed_rule(s, inp['p'], inp['sigma'], G.Lam(inp['k']))[:2] for s in inp['s_all']]
+    if SCHEDULE == 'columns' and len(inp['s_all']) > 1:
+        from polar_amd import batched
+        return batched.polar_columns(G, inp['cand'], inp['k'], list(inp['s_all']), q=inp['q'], B=inp['B'],
+                                     tol=inp['tol'], sketch=sk, poles=poles, precision=precision, graph=GRAPH,
+                                     replace=REPLACE and precision == 'fp32')
     if union is not None:
         from polar_amd import batched
         S = len(inp['s_all'])
@@ -188,13 +200,15 @@ def main():
     ap.add_argument('--skip-cpu', action='store_true')
     ap.add_argument('--skip-spmm', action='store_true')
     ap.add_argument('--spmm-scale', type=float, default=1.0)
-    ap.add_argument('--per-s', action='store_true', help='one greedy run per s value (no batching over s)')
+    ap.add_argument('--schedule', choices=['columns', 'segments', 'per-s'], default='columns',
+                    help='how the s values of a step share the GPU (same results)')
     ap.add_argument('--graph', choices=['on', 'off'], default='on', help='hipGraph replay of the CG iterations')
     ap.add_argument('--replace', action='store_true', help='FP64 residual replacement in the FP32-mixed step')
     ap.add_argument('--spmm-only', action='store_true',
                     help='only the n=4M SpMM leg and the 1 GiB copy probe (for counter runs)')
     a = ap.parse_args()
-    global GRAPH, REPLACE
+    global GRAPH, REPLACE, SCHEDULE
+    SCHEDULE = a.schedule
     GRAPH = a.graph == 'on'
     REPLACE = a.replace
     if a.gpus != 1:
@@ -224,7 +238,7 @@ def main():
 
     copy_probe(torch)
     union = None
-    if not a.per_s and len(inp['s_all']) > 1:
+    if a.schedule == 'segments' and len(inp['s_all']) > 1:
         from polar_amd import batched
         union = batched.UnionGraph([G] * len(inp['s_all']))
 
@@ -294,8 +308,9 @@ def main():
                                     f'k={inp["k"]}, B={inp["B"]}, tol={inp["tol"]}',
                            precision='fp32-mixed', column_iterations=colits,
                            hipgraph=GRAPH, residual_replacement=REPLACE,
-                           s_schedule='per s' if union is None else
-                           f'{len(inp["s_all"])} s values as segments of one batched solve per phase',
+                           s_schedule={'columns': f'{len(inp["s_all"])} s values as column groups of one operator',
+                                       'segments': f'{len(inp["s_all"])} s values as segments of one batched solve',
+                                       'per-s': 'one greedy run per s value'}[a.schedule],
                            rhs=int(sum(o['rhs'] for o in outs32))),
                fp64_seconds=el64, fp32_vs_fp64_speedup=(el64 / el32) if el64 else None,
                fp32_vs_fp64=agree, spmm_n4m=spmm, roofline=roofline, cpu_baseline=cpu,

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define POLAR_ABI_VERSION 2
+#define POLAR_ABI_VERSION 3
 
 enum polar_dtype { POLAR_F64 = 0, POLAR_F32 = 1 };
 
@@ -83,6 +83,8 @@ typedef struct polar_op {
   const int32_t* patch_cols;  /* [patch_ptr[n_patch]] */
   int32_t grid_max;           /* tuning: block budget of the kernels with partial sums (0 = default) */
   int32_t pad2;
+  const int32_t* patch_grp;   /* [patch_ptr[n_patch]] column group of each patch entry, or NULL
+                                 (every entry applies to every column) */
 } polar_op;
 
 /* Segments of a block-diagonal operator (the batched form: many graphs in one operator, each on
@@ -91,11 +93,14 @@ typedef struct polar_op {
  * owns rows [rows[g], rows[g+1]) and its long rows, chunks and active rows lie in its ranges. */
 typedef struct polar_segs {
   int32_t S;
-  int32_t pad;
+  int32_t G;                  /* column groups (S == 1): each group of columns is its own system on
+                                 the one graph, with poles tau[g*p..], seed shift tau[g*p] and the
+                                 patch entries tagged g; 0 or 1 = none */
   const int64_t* rows;
   const int64_t* extra;
   const int64_t* longs;
   const int64_t* active;
+  const int32_t* col_group;   /* host int32[R]: group of each column of this call (G > 1) */
 } polar_segs;
 
 int polar_abi_version(void);
@@ -132,7 +137,10 @@ int polar_msolve(const polar_op* op, const void* W, int64_t ldw, int64_t R, cons
  * on the segments of one block-diagonal operator, advanced together -- one launch per kernel for
  * all segments, each (segment, column) with its own CG scalars, finished segments skipped and
  * their blocks given to the others.  Segment g's results equal those of polar_msolve on graph g
- * alone.  The reference runs graphs one at a time (realworld/run_real.py:114-133 loops over
+ * alone.  With segs->G > 1 (one segment), the columns are G groups instead: column c solves with
+ * the poles of group col_group[c] and the patch entries of that group (several s values, each
+ * with its own selected edges, on one graph: one pass over the adjacency for all of them); tau is
+ * then double[G][p] and h_out, te_out are [p][R].  The reference runs graphs one at a time (realworld/run_real.py:114-133 loops over
  * graphs calling polar_gpu.msolve, polar_gpu.py:88-133); this entry point has no counterpart
  * there.  tau: device double[S][p].  active: device int32[nA], nA = segs->active[S], union row
  * ids.  Xout (j, i, c) at Xout + j*x_sj + i*x_si + c over the nA active rows.  h_out, te_out:

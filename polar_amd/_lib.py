@@ -13,7 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / 'libpolar_amd.so'
 
 POLAR_F64, POLAR_F32 = 0, 1
 POLAR_TRACK, POLAR_TRACE, POLAR_NOFREEZE, POLAR_GRAPH, POLAR_BLOCK8, POLAR_REPLACE = 1, 2, 4, 8, 16, 32
-ABI_VERSION = 2
+ABI_VERSION = 3
 POLAR_E_ITERCAP, POLAR_E_ARG = -2, -3
 CHUNK = 64  # POLAR_CHUNK: default long-row chunk length
 
@@ -41,12 +41,13 @@ class PolarOp(ctypes.Structure):
         ('long_rows', vp), ('long_base', vp), ('extra', vp),
         ('n_patch', i32), ('cpl_max', i32),
         ('patch_rows', vp), ('patch_ptr', vp), ('patch_cols', vp),
-        ('grid_max', i32), ('pad2', i32),
+        ('grid_max', i32), ('pad2', i32), ('patch_grp', vp),
     ]
 
 
 class PolarSegs(ctypes.Structure):
-    _fields_ = [('S', i32), ('pad', i32), ('rows', vp), ('extra', vp), ('longs', vp), ('active', vp)]
+    _fields_ = [('S', i32), ('G', i32), ('rows', vp), ('extra', vp), ('longs', vp), ('active', vp),
+                ('col_group', vp)]
 
 
 OPP = ctypes.POINTER(PolarOp)

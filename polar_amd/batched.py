@@ -72,12 +72,16 @@ class Segments:
 
 
 def msolve_batched(op, segs, W, tau, active, tol, maxit=100000, track=False, trace=False, out=None,
-                   width=None, graph=None, freeze=True, block=None, replace=False):
+                   width=None, graph=None, freeze=True, block=None, replace=False, col_group=None):
     """Batched block multi-shift CG: segment g solves (M_g + tau[g, j]) U = W[rows of g, :R].
 
     op: Op of a UnionGraph; segs: Segments; W: (n, R) device; tau: (S, p) host or device; active:
     device int32 union row ids (segment g's at segs.a0[g]:segs.a0[g+1]).  Returns X (p, nA, R),
-    h (p, S, R) or None, iters (S, R) host int64, and te (p, S, R) with trace=True."""
+    h (p, S, R) or None, iters (S, R) host int64, and te (p, S, R) with trace=True.
+
+    Column groups (one segment): col_group (R,) assigns column c to group col_group[c], which
+    solves with the poles tau[col_group[c]] (tau is (G, p)) and the op's added edges tagged with
+    that group; h, te are then (p, 1, R) and iters (1, R)."""
     lib = L.require_gpu()
     dev = pg._device()
     dtype = op.dtype
@@ -86,7 +90,14 @@ def msolve_batched(op, segs, W, tau, active, tol, maxit=100000, track=False, tra
         W = W.contiguous()
     n, R = W.shape
     S = segs.S
-    tau_d = torch.as_tensor(tau, dtype=torch.float64, device=dev).reshape(S, -1).contiguous()
+    if col_group is not None:
+        assert S == 1
+        col_group = np.ascontiguousarray(col_group, dtype=np.int32)
+        assert col_group.shape == (R,)
+        tau_d = torch.as_tensor(tau, dtype=torch.float64, device=dev)
+        tau_d = tau_d.reshape(-1, tau_d.shape[-1]).contiguous()
+    else:
+        tau_d = torch.as_tensor(tau, dtype=torch.float64, device=dev).reshape(S, -1).contiguous()
     p = tau_d.shape[1]
     nA = int(segs.a0[-1])
     X = out if out is not None else torch.empty((p, nA, R), dtype=dtype, device=dev)
@@ -110,9 +121,14 @@ def msolve_batched(op, segs, W, tau, active, tol, maxit=100000, track=False, tra
         hb = (h if r == R else torch.empty((p, S, r), dtype=torch.float64, device=dev)) if track else None
         tb = (te if r == R else torch.empty((p, S, r), dtype=torch.float64, device=dev)) if trace else None
         ib = iters if r == R else torch.empty((S, r), dtype=torch.int64, device=dev)
+        if col_group is not None:
+            segs.c.G = int(tau_d.shape[0])
+            segs.c.col_group = col_group[lo:lo + r].ctypes.data_as(L.vp)
         rc = lib.polar_msolve_batched(s, segs.c, L.ptr(W[:, lo:]), W.stride(0), r, L.ptr(tau_d), p, L.ptr(act),
                                       float(tol), int(maxit), flags, L.ptr(X[:, :, lo:]), X.stride(0), X.stride(1),
                                       L.ptr(hb), L.ptr(tb), L.ptr(ib), L.ptr(ws), ws_bytes, L.stream_ptr())
+        if col_group is not None:
+            segs.c.G, segs.c.col_group = 0, None
         L.check(rc, 'polar_msolve_batched')
         if r != R:
             iters[:, lo:lo + r] = ib
@@ -351,6 +367,209 @@ def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac'
     seconds = time.perf_counter() - t0
     out = []
     for g in range(S):
+        tr = traces[g]
+        if want_trace:
+            for dn in tr_hist[g]:
+                dn_h = dn.cpu().numpy()
+                tr.append(tr[-1] - math.fsum(oms[g] * dn_h[:, 1] / dn_h[:, 0]))
+        out.append(dict(sequence=seqs[g], seconds=seconds, spmv=int(spmv[g]), rhs=int(rhs[g]), p=p,
+                        rounds=len(seqs[g]), truncated=len(seqs[g]) < k, marginals=margs[g],
+                        trace_estimate=tr[:max(len(seqs[g]), 1)],
+                        precision='fp64' if dtype == torch.float64 else 'fp32'))
+    return out
+
+
+def single_segment(G, nA):
+    """Segments of a plain graph (one segment) with nA active rows."""
+    sc = G.schedule()
+    return Segments(np.array([0, G.n]), np.array([0, len(sc['extra'])]), np.array([0, len(sc['long_rows'])]),
+                    np.array([0, nA]))
+
+
+def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac', sketch=None, seed=2026,
+                  kind='rademacher', poles=None, precision='fp32', graph=None, width=None, block=None,
+                  replace=False, progress=None):
+    """POLAR greedy (polar_gpu.py:180-254) for several s values on ONE graph, in lockstep, as column
+    groups of one operator: group g is the run for s_list[g], with its own poles, its own bank
+    columns [Z | F_g], its own caches and its own selected edges (patch entries tagged g).  Every
+    solve of a phase is one launch sequence that reads the adjacency once for all groups.  Results
+    equal polar_gpu.polar run once per s (same sketch): one dict per s value."""
+    lib = L.require_gpu()
+    dev = pg._device()
+    dtype = pg._dt(precision)
+    dcode = L.dtype_code(dtype)
+    stream = L.stream_ptr()
+    Gn = len(s_list)
+    objc = 2 if obj == 'logdet' else 0
+    want_trace = obj != 'logdet'
+    if poles is None:
+        poles = [pg.poles(obj, s, G.sigma, G.Lam(k), eps)[:2] for s in s_list]
+    taus = [np.asarray(t, float) for t, _ in poles]
+    oms = [np.asarray(o, float) for _, o in poles]
+    p = len(taus[0])
+    if any(len(t) != p for t in taus):
+        raise ValueError('column groups need the same pole count for every s')
+    tau_all = torch.from_numpy(np.stack(taus)).to(dev)
+    om_d = [torch.from_numpy(o).to(dev) for o in oms]
+    cand = np.asarray(cand)
+    N = len(cand)
+    if sketch is None:
+        sketch = pg.sketches(G, cand, q, seed, kind=kind, on_device='lazy' if kind == 'rademacher' else False,
+                             precision=dtype)
+    n = G.n
+    op = pg.Op(G, precision=dtype)
+    W2 = 2 * q if not isinstance(sketch, pg.DeviceSketch) else 2 * sketch.q
+    q = W2 // 2
+    RHS = torch.empty((n, W2 * Gn), dtype=dtype, device=dev)        # [Z | F_g] per group
+    if isinstance(sketch, pg.DeviceSketch):
+        sketch.fill(RHS[:, :q], RHS[:, q:W2], op)
+        Gc = sketch.rows
+    else:
+        Z, F, Gc = sketch
+        RHS[:, :q].copy_(torch.as_tensor(Z, device=dev).to(dtype))
+        RHS[:, q:W2].copy_(torch.as_tensor(F, device=dev).to(dtype))
+    for g in range(1, Gn):
+        RHS[:, g * W2:(g + 1) * W2].copy_(RHS[:, :W2])
+    if isinstance(Gc, torch.Tensor):
+        Gc = Gc.to(dev, torch.float64)
+    elif not isinstance(Gc, pg.RademacherRows):
+        Gc = np.asarray(Gc, dtype=np.float64)
+    active_h = np.unique(cand)
+    es_h = np.searchsorted(active_h, cand)
+    nA = len(active_h)
+    active = torch.from_numpy(active_h.astype(np.int32)).to(dev)
+    ea = torch.from_numpy(es_h[:, 0].astype(np.int32)).to(dev)
+    eb = torch.from_numpy(es_h[:, 1].astype(np.int32)).to(dev)
+    segs = single_segment(G, nA)
+    spmv = np.zeros(Gn, dtype=np.int64)
+    rhs = np.zeros(Gn, dtype=np.int64)
+    live = [np.ones(N, bool) for _ in range(Gn)]
+    dead = [torch.zeros(N, dtype=torch.uint8, device=dev) for _ in range(Gn)]
+    score = torch.empty(N, dtype=torch.float64, device=dev)
+    sel_ws = torch.empty(lib.polar_select_ws_bytes(N), dtype=torch.uint8, device=dev)
+    sel_out = torch.empty(64, dtype=torch.int32, device=dev)
+    coef = torch.empty((p, 2 * q + 3), dtype=torch.float64, device=dev)
+    wX = torch.empty((p, nA), dtype=torch.float64, device=dev)
+    wY = torch.empty_like(wX)
+    gvec = torch.empty(max(B, 1) + 2, dtype=torch.float64, device=dev)
+    gc_row = torch.empty(q, dtype=torch.float64, device=dev)
+    seqs = [[] for _ in range(Gn)]
+    margs = [[] for _ in range(Gn)]
+    traces = [[] for _ in range(Gn)]
+    tr_hist = [[] for _ in range(Gn)]
+    pg._sync()
+    t0 = time.perf_counter()
+
+    C = W2 * Gn
+    XY = torch.empty((p, nA, C), dtype=dtype, device=dev)
+    if width is None:
+        width = pg.fit_width(n, p, nA, C, dtype, block=block)
+    col_of = np.arange(C) // W2                                      # group of every bank column
+    terms = [[] for _ in range(Gn)]
+    for lo in range(0, C, width):
+        hi = min(C, lo + width)
+        res = msolve_batched(op, segs, RHS[:, lo:hi], tau_all, active, tol, trace=want_trace, out=XY[:, :, lo:hi],
+                             graph=graph, block=block, replace=replace, col_group=col_of[lo:hi])
+        it = res[2][0]
+        te = res[3][:, 0, :].cpu().numpy() if want_trace else None
+        for c in range(lo, hi):
+            g = col_of[c]
+            spmv[g] += it[c - lo]
+            rhs[g] += 1
+            if want_trace and c - g * W2 < q:
+                terms[g].append(oms[g] * te[:, c - lo])
+    Xs = [XY[:, :, g * W2:g * W2 + q] for g in range(Gn)]
+    Ys = [XY[:, :, g * W2 + q:(g + 1) * W2] for g in range(Gn)]
+    hs, ts = [], []
+    for g in range(Gn):
+        h = torch.empty((p, N), dtype=torch.float64, device=dev)
+        t = torch.empty((p, N), dtype=torch.float64, device=dev)
+        L.check(lib.polar_pair_sqnorm(dcode, L.ptr(Xs[g]), Xs[g].stride(0), Xs[g].stride(1), p, q, L.ptr(ea), L.ptr(eb),
+                                      N, L.ptr(h), stream), 'pair_sqnorm')
+        L.check(lib.polar_pair_sqnorm(dcode, L.ptr(Ys[g]), Ys[g].stride(0), Ys[g].stride(1), p, q, L.ptr(ea), L.ptr(eb),
+                                      N, L.ptr(t), stream), 'pair_sqnorm')
+        hs.append(h)
+        ts.append(t)
+        traces[g].append(math.fsum(np.concatenate(terms[g])) if want_trace else None)
+    if progress is not None:
+        progress('bank', dict(seconds=time.perf_counter() - t0, spmv=int(spmv.sum())))
+
+    for rnd in range(k):
+        shorts = []
+        for g in range(Gn):
+            Br = min(B, int(live[g].sum()))
+            short = np.zeros(Br, dtype=np.int32)
+            if Br:
+                L.check(lib.polar_proxy(objc, L.ptr(hs[g]), L.ptr(ts[g]), L.ptr(tau_all[g]), L.ptr(om_d[g]), p, N,
+                                        G.sigma, L.ptr(dead[g]), L.ptr(score), stream), 'proxy')
+                L.check(lib.polar_select_topb(L.ptr(score), N, Br, L.ptr(sel_out), short.ctypes.data_as(L.vp),
+                                              L.ptr(sel_ws), sel_ws.numel(), stream), 'select_topb')
+            shorts.append(short)
+        off = np.r_[0, np.cumsum([len(sh) for sh in shorts])]
+        Rt = int(off[-1])
+        if Rt == 0:
+            break
+        ia = torch.from_numpy(np.concatenate([cand[sh, 0] for sh in shorts])).to(dev)
+        ib = torch.from_numpy(np.concatenate([cand[sh, 1] for sh in shorts])).to(dev)
+        ic = torch.arange(Rt, device=dev)
+        Wb = torch.zeros((n, Rt), dtype=dtype, device=dev)
+        Wb.index_put_((ia, ic), torch.ones(Rt, dtype=dtype, device=dev))
+        Wb.index_put_((ib, ic), torch.full((Rt,), -1.0, dtype=dtype, device=dev))
+        cg = np.concatenate([np.full(len(sh), g, np.int32) for g, sh in enumerate(shorts)])
+        Ub, hh, it = msolve_batched(op, segs, Wb, tau_all, active, tol, track=obj != 'logdet', graph=graph,
+                                    block=block, replace=replace, col_group=cg)[:3]
+        for g in range(Gn):
+            Br = len(shorts[g])
+            if Br == 0:
+                continue
+            spmv[g] += int(it[0, off[g]:off[g + 1]].sum())
+            rhs[g] += Br
+            short = shorts[g]
+            Ug = Ub[:, :, off[g]:]
+            sa = torch.from_numpy(es_h[short, 0].astype(np.int32)).to(dev)
+            sb = torch.from_numpy(es_h[short, 1].astype(np.int32)).to(dev)
+            hg = hh[:, 0, off[g]:off[g + 1]].contiguous() if hh is not None else None
+            win = np.zeros(1, dtype=np.int32)
+            val = np.zeros(1, dtype=np.float64)
+            L.check(lib.polar_rerank_pick(objc, dcode, L.ptr(Ug), Ug.stride(0), Ug.stride(1), p, Br, L.ptr(sa),
+                                          L.ptr(sb), L.ptr(hg), L.ptr(om_d[g]), L.ptr(gvec),
+                                          win.ctypes.data_as(L.vp), val.ctypes.data_as(L.vp), stream), 'rerank_pick')
+            j = 0 if Br == 1 else int(win[0])
+            e = int(short[j])
+            u = Ug[:, :, j]
+            margs[g].append(float(val[0]))
+            seqs[g].append(e)
+            live[g][e] = False
+            dead[g][e] = 1
+            aa, bb = (int(x) for x in es_h[e])
+            a, b = (int(x) for x in cand[e])
+            if isinstance(Gc, torch.Tensor):
+                gc_row.copy_(Gc[e])
+            else:
+                gc_row.copy_(torch.from_numpy(np.ascontiguousarray(Gc[e], dtype=np.float64)))
+            if rnd + 1 < k:
+                Xg, Yg = Xs[g], Ys[g]
+                L.check(lib.polar_maint_prep(dcode, L.ptr(Xg), L.ptr(Yg), Xg.stride(0), Xg.stride(1), p, q, L.ptr(u),
+                                             u.stride(0), u.stride(1), aa, bb, L.ptr(gc_row), L.ptr(coef), stream),
+                        'maint_prep')
+                L.check(lib.polar_maint_rank1(dcode, L.ptr(Xg), L.ptr(Yg), Xg.stride(0), Xg.stride(1), p, nA, q,
+                                              L.ptr(u), u.stride(0), u.stride(1), L.ptr(coef), L.ptr(wX), L.ptr(wY),
+                                              stream), 'maint_rank1')
+                L.check(lib.polar_maint_cache(dcode, L.ptr(u), u.stride(0), u.stride(1), p, nA, N, L.ptr(ea), L.ptr(eb),
+                                              L.ptr(coef), q, L.ptr(wX), L.ptr(wY), L.ptr(hs[g]), L.ptr(ts[g]), stream),
+                        'maint_cache')
+                if want_trace:
+                    tr_hist[g].append(coef[:, 2 * q:2 * q + 2].clone())
+            Fv = RHS[:, g * W2 + q:(g + 1) * W2]
+            L.check(lib.polar_rows_axpy(dcode, L.ptr(Fv), Fv.stride(0), q, a, b, L.ptr(gc_row), stream), 'rows_axpy')
+            op.add(a, b, group=g)
+        if progress is not None:
+            progress('round', dict(round=rnd, edges=[sq[-1] if sq else None for sq in seqs],
+                                   seconds=time.perf_counter() - t0, spmv=int(spmv.sum())))
+    pg._sync()
+    seconds = time.perf_counter() - t0
+    out = []
+    for g in range(Gn):
         tr = traces[g]
         if want_trace:
             for dn in tr_hist[g]:

@@ -166,22 +166,47 @@ __device__ __forceinline__ void gather_acc(const int32_t* __restrict__ idx, cons
   }
 }
 
-// y = (diag + shift) x - acc + patch(x) for a complete row; returns y in yv.
+// Column groups: several systems on one graph, each a group of columns with its own poles, its
+// own seed shift and its own added edges (patch entries tagged with the group).
+struct ColGroups {
+  const int* grp;        // [columns] column -> group, or NULL (one group: `shift` applies)
+  const double* tau;     // [G][p]: tau[g*p] is group g's seed shift
+  int p;
+};
+
+// y = (diag + shift) x - acc + patch(x) for a complete row; returns y in yv.  With column groups
+// each column uses its group's shift and its group's patch entries (same arithmetic per column).
 template <typename T, int CPL>
 __device__ __forceinline__ void finish_row(const polar_op& op, double shift, const T* __restrict__ X,
                                            int64_t ldx, int row, int c0, int pidx, const T (&acc)[CPL],
-                                           T (&xv)[CPL], T (&yv)[CPL]) {
+                                           T (&xv)[CPL], T (&yv)[CPL], const ColGroups& cg) {
   vload<T, CPL>(X + (int64_t)row * ldx + c0, xv);
-  const T dg = (T)((double)((const T*)op.diag)[row] + shift);
+  const double dgd = (double)((const T*)op.diag)[row];
+  int gcol[CPL];
+  if (cg.grp) {
 #pragma unroll
-  for (int e = 0; e < CPL; ++e) yv[e] = (dg * xv[e]) - acc[e];
+    for (int e = 0; e < CPL; ++e) {
+      gcol[e] = cg.grp[c0 + e];
+      const T dg = (T)(dgd + cg.tau[gcol[e] * cg.p]);
+      yv[e] = (dg * xv[e]) - acc[e];
+    }
+  } else {
+    const T dg = (T)(dgd + shift);
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) {
+      gcol[e] = 0;
+      yv[e] = (dg * xv[e]) - acc[e];
+    }
+  }
   if (pidx >= 0) {
     const int pb = op.patch_ptr[pidx], pe = op.patch_ptr[pidx + 1];
     for (int q = pb; q < pe; ++q) {
       T xj[CPL];
       vload<T, CPL>(X + (int64_t)op.patch_cols[q] * ldx + c0, xj);
+      const int pg = op.patch_grp ? op.patch_grp[q] : -1;
 #pragma unroll
-      for (int e = 0; e < CPL; ++e) yv[e] = yv[e] + (xv[e] - xj[e]);
+      for (int e = 0; e < CPL; ++e)
+        if (pg < 0 || pg == gcol[e]) yv[e] = yv[e] + (xv[e] - xj[e]);
     }
   }
 }
@@ -234,6 +259,9 @@ struct CGState {
   // every few windows the residual becomes w - (M + tau0) x0, computed in FP64 from the
   // operator values the solve uses.  NULL when off.
   double* x0;
+  // Column groups (one segment): G groups, cgrp[slot] = group of the column in that slot.
+  int G;
+  int* cgrp;
 };
 
 // Segment of block b of a kernel whose blocks are partitioned by the prefix table `pref`.
@@ -312,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_scalars(CGState s, int nparts, int c
   const double cc = act ? a * s.bp[vc] / s.ap[vc] : 0.0;
   const double nr = sqrt(rrn);
   const double stop = s.stop[vc];
-  const double* shift = s.shift + g * s.p;
+  const double* shift = s.shift + (s.G > 1 ? s.cgrp[c] : g) * s.p;
   __syncthreads();
   int nlive = 0;
   for (int j = threadIdx.x; j < s.p; j += kBlock) {
@@ -483,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
       }
       if (colok) {
         T xv[CPL], yv[CPL];
-        finish_row<T, CPL>(op, shift, X, ldx, row, c0, pidx, acc, xv, yv);
+        finish_row<T, CPL>(op, shift, X, ldx, row, c0, pidx, acc, xv, yv, ColGroups{s.G > 1 ? s.cgrp : nullptr, s.tau, s.p});
         vstore<T, CPL>(Y + (int64_t)row * ldy + c0, yv);
         if (DOT) {
 #pragma unroll
@@ -546,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
           }
       }
       T xv[CPL], yv[CPL];
-      finish_row<T, CPL>(op, shift, X, ldx, row, c0, pidx, acc, xv, yv);
+      finish_row<T, CPL>(op, shift, X, ldx, row, c0, pidx, acc, xv, yv, ColGroups{s.G > 1 ? s.cgrp : nullptr, s.tau, s.p});
       vstore<T, CPL>(Y + (int64_t)row * ldy + c0, yv);
       if (DOT) {
 #pragma unroll
@@ -771,6 +799,7 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
   __shared__ int s_jl[kMaxPolesFlush];
   __shared__ int s_nj;
   __shared__ T s_cx[(kW + 1) * 256], s_cp[(kW + 1) * 256];
+  __shared__ int s_dirty[256];
   if (threadIdx.x == 0) s_nj = 0;
   __syncthreads();
   for (int j = threadIdx.x; j < p; j += kBlock) {
@@ -809,10 +838,17 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
         s_cx[q] = (T)s.cX[src];
         s_cp[q] = (T)s.cP[src];
       }
+      for (int c = threadIdx.x; c < R; c += kBlock) s_dirty[c] = s.dirty[(int64_t)j * s.sld + vb + c];
       __syncthreads();
+      // columns unchanged in the window have identity coefficients: nothing to read or write
+      bool mine = false;
+      if (colok) {
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) mine |= s_dirty[c0 + e] != 0;
+      }
 #pragma unroll
       for (int k = 0; k < kFlushRows; ++k) {
-        if (!ok[k]) continue;
+        if (!ok[k] || !mine) continue;
         T* pp = P + ((int64_t)j * nA + row[k]) * s.ld + c0;
         T* xrow = Xo + (int64_t)j * sj + row[k] * si;
         T pv[CPL], xv[CPL];
@@ -905,18 +941,24 @@ __global__ __launch_bounds__(kBlock) void k_replace(polar_op op, double shift, c
         for (int e = 0; e < CPL; ++e) acc[e] = acc[e] + wk * xj[e];
       }
       const double* xr = s.x0 + i * s.ld + c0;
-      const double dg = (double)((const T*)op.diag)[row] + shift;
       double y[CPL];
+      int gcol[CPL];
 #pragma unroll
-      for (int e = 0; e < CPL; ++e) y[e] = dg * xr[e] - acc[e];
+      for (int e = 0; e < CPL; ++e) {
+        gcol[e] = s.G > 1 ? s.cgrp[c0 + e] : 0;
+        const double dg = (double)((const T*)op.diag)[row] + (s.G > 1 ? s.tau[gcol[e] * s.p] : shift);
+        y[e] = dg * xr[e] - acc[e];
+      }
       if (op.n_patch > 0) {
         if (pc < 0) pc = lower_bound_i32(op.patch_rows, op.n_patch, row);
         while (pc < op.n_patch && op.patch_rows[pc] < row) ++pc;
         if (pc < op.n_patch && op.patch_rows[pc] == row) {
           for (int q = op.patch_ptr[pc]; q < op.patch_ptr[pc + 1]; ++q) {
             const double* xq = s.x0 + (int64_t)op.patch_cols[q] * s.ld + c0;
+            const int pg = op.patch_grp ? op.patch_grp[q] : -1;
 #pragma unroll
-            for (int e = 0; e < CPL; ++e) y[e] = y[e] + (xr[e] - xq[e]);
+            for (int e = 0; e < CPL; ++e)
+              if (pg < 0 || pg == gcol[e]) y[e] = y[e] + (xr[e] - xq[e]);
           }
         }
       }
@@ -1029,6 +1071,8 @@ struct Plan {
 struct Segs {
   int S = 1;
   std::vector<int64_t> row, x, l, a;   // [S+1]: rows, long-row chunk units, long rows, active rows
+  int G = 1;                           // column groups (S == 1 only)
+  std::vector<int> col_group;          // [R] when G > 1
 };
 
 constexpr int kBatchGrid = 4096;   // default block budget of a batched kernel with partial sums
@@ -1175,7 +1219,7 @@ static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int
   s.z = cv.take<double>(pr); s.zp = cv.take<double>(pr); s.zn = cv.take<double>(pr);
   s.aj = cv.take<double>(pr); s.bj = cv.take<double>(pr); s.h = cv.take<double>(pr);
   s.ip = cv.take<double>(pr); s.pn = cv.take<double>(pr); s.te = cv.take<double>(pr);
-  s.tg = cv.take<double>(pr); s.shift = cv.take<double>((int64_t)p * S);
+  s.tg = cv.take<double>(pr); s.shift = cv.take<double>((int64_t)p * std::max<int64_t>(S, R));
   s.live = cv.take<int>(pr); s.m = cv.take<int>(pr);
   s.stats = cv.take<unsigned long long>(3);
   s.parts = cv.take<double>(parts_rows(op, S) * 2 * R);
@@ -1185,6 +1229,7 @@ static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int
     s.dirty = cv.take<int>(pr);
   }
   if (repl) s.x0 = cv.take<double>(op->n * R);
+  s.cgrp = cv.take<int>(R);
   s.colmap = cv.take<int>(R);
   s.sflag = cv.take<int>(2 * S);
   m.dtab = cv.take<int>(5 * (S + 1));
@@ -1497,6 +1542,10 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   s.S = S;
   s.sld = (int)(S * R);
   s.tau = tau;
+  s.G = sg.G;
+  if (sg.G > 1)
+    POLAR_HIP(hipMemcpyAsync(s.cgrp, sg.col_group.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
+  const int ngroups = std::max(S, sg.G);
   int cpl = cpl0;
   std::vector<char> live(S, 1);
   Plan pl = make_plan(op, R, cpl, &sg, &live);
@@ -1535,7 +1584,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   }
   hipLaunchKernelGGL(k_iota, dim3(1), dim3(256), 0, st, s.colmap, (int)R);
   POLAR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_shift, dim3((p * S + 255) / 256), dim3(256), 0, st, tau, p, S, s.shift);
+  hipLaunchKernelGGL(k_shift, dim3((p * ngroups + 255) / 256), dim3(256), 0, st, tau, p, ngroups, s.shift);
   POLAR_LAUNCH_CHECK();
   if (cpl == 2) {
     hipLaunchKernelGGL((k_init<T, 2>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, (int)R, pl.L, W, ldw, slot[0], d, s);
@@ -1559,8 +1608,8 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
     POLAR_LAUNCH_CHECK();
   }
 
-  double tau0 = 0.0;          // S == 1: the seed shift is a kernel argument (segments read tau)
-  if (S == 1) {
+  double tau0 = 0.0;          // one system: the seed shift is a kernel argument (else read from tau)
+  if (S == 1 && sg.G == 1) {
     POLAR_HIP(hipMemcpyAsync(&tau0, tau, sizeof(double), hipMemcpyDeviceToHost, st));
     POLAR_HIP(hipStreamSynchronize(st));
   }
@@ -1727,6 +1776,8 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
       for (int* a : {s.live, s.m})
         if (int e = compact_rows<int>(p, R, Rn, perm, a, st)) return e;
       if (int e = compact_rows<int>(1, R, Rn, perm, s.colmap, st)) return e;
+      if (sg.G > 1)
+        if (int e = compact_rows<int>(1, R, Rn, perm, s.cgrp, st)) return e;
       s.R = Rn;
       s.ident = 0;
       cpl = cpl_for<T>(Rn, R, cpl0);
@@ -1794,7 +1845,7 @@ static int64_t msolve_ws(const polar_op* op, int S, int64_t R, int p, int64_t nA
 }
 
 // Segment ranges from the caller's table, checked against the operator.
-static int segs_from(const polar_op* op, const polar_segs* in, Segs& sg) {
+static int segs_from(const polar_op* op, const polar_segs* in, int64_t R, Segs& sg) {
   if (!in || in->S < 1 || !in->rows || !in->extra || !in->longs || !in->active)
     return set_error(POLAR_E_ARG, "invalid polar_segs");
   const int S = in->S;
@@ -1809,6 +1860,14 @@ static int segs_from(const polar_op* op, const polar_segs* in, Segs& sg) {
   for (int g = 0; g < S; ++g)
     if (sg.row[g + 1] <= sg.row[g] || sg.x[g + 1] < sg.x[g] || sg.l[g + 1] < sg.l[g] || sg.a[g + 1] < sg.a[g])
       return set_error(POLAR_E_ARG, "segment %d: empty or decreasing range", g);
+  sg.G = in->G > 1 ? in->G : 1;
+  if (sg.G > 1) {
+    if (S != 1) return set_error(POLAR_E_ARG, "column groups need a single segment");
+    if (!in->col_group) return set_error(POLAR_E_ARG, "column groups without col_group");
+    sg.col_group.assign(in->col_group, in->col_group + R);
+    for (int v : sg.col_group)
+      if (v < 0 || v >= sg.G) return set_error(POLAR_E_ARG, "column group %d out of range", v);
+  }
   return 0;
 }
 
@@ -1902,7 +1961,7 @@ int64_t polar_msolve_batched_ws_bytes(const polar_op* op, const polar_segs* segs
                                       int32_t flags) {
   if (check_op(op)) return -1;
   Segs sg;
-  if (segs_from(op, segs, sg)) return -1;
+  if (segs_from(op, segs, R, sg)) return -1;
   const int64_t nA = sg.a[sg.S];
   return op->dtype == POLAR_F64 ? msolve_ws<double>(op, sg.S, R, p, nA, flags) : msolve_ws<float>(op, sg.S, R, p, nA, flags);
 }
@@ -1913,7 +1972,7 @@ int polar_msolve_batched(const polar_op* op, const polar_segs* segs, const void*
                          double* te_out, int64_t* iters_out, void* ws, int64_t ws_bytes, void* stream) {
   if (int rc = check_op(op)) return rc;
   Segs sg;
-  if (int rc = segs_from(op, segs, sg)) return rc;
+  if (int rc = segs_from(op, segs, R, sg)) return rc;
   const int64_t nA = sg.a[sg.S];
   if (R <= 0 || ldw < R || p <= 0 || !W || !tau || (nA > 0 && !active) || !Xout || !ws)
     return set_error(POLAR_E_ARG, "bad msolve arguments");

@@ -138,6 +138,7 @@ class Op:
         self.M0 = G.M0
         self.A = []
         self.B = []
+        self.grp = []             # column group of each added edge (None: all columns)
         self.dtype = _dt(precision)
         self._patch = None
         self._struct = None
@@ -145,9 +146,12 @@ class Op:
         self.grid_max = 0
         self.use_graph = False
 
-    def add(self, a, b):
+    def add(self, a, b, group=None):
+        """Add the unit-weight edge (a, b) (polar_gpu.py:68-69); with `group`, only for the columns of
+        that column group (several systems on one graph, see polar_amd.batched.polar_columns)."""
         self.A.append(int(a))
         self.B.append(int(b))
+        self.grp.append(None if group is None else int(group))
         self._patch = None
         self._struct = None
 
@@ -156,15 +160,18 @@ class Op:
             dev = _device()
             if self.A:
                 nbr = {}
-                for a, b in zip(self.A, self.B):
-                    nbr.setdefault(a, []).append(b)
-                    nbr.setdefault(b, []).append(a)
+                for a, b, g in zip(self.A, self.B, self.grp):
+                    nbr.setdefault(a, []).append((b, g))
+                    nbr.setdefault(b, []).append((a, g))
                 rows = np.array(sorted(nbr), dtype=np.int32)
                 ptr = np.zeros(len(rows) + 1, dtype=np.int32)
                 ptr[1:] = np.cumsum([len(nbr[r]) for r in rows])
-                cols = np.array([c for r in rows for c in nbr[r]], dtype=np.int32)
-                self._patch = (torch.from_numpy(rows).to(dev), torch.from_numpy(ptr).to(dev),
-                               torch.from_numpy(cols).to(dev))
+                cols = np.array([c for r in rows for c, _ in nbr[r]], dtype=np.int32)
+                arrs = [torch.from_numpy(rows).to(dev), torch.from_numpy(ptr).to(dev), torch.from_numpy(cols).to(dev)]
+                if any(g is not None for g in self.grp):
+                    grp = np.array([-1 if g is None else g for r in rows for _, g in nbr[r]], dtype=np.int32)
+                    arrs.append(torch.from_numpy(grp).to(dev))
+                self._patch = tuple(arrs)
             else:
                 self._patch = ()
         return self._patch
@@ -196,7 +203,8 @@ class Op:
         s.extra = L.ptr(idx['extra']) if s.n_extra else None
         if patch:
             s.n_patch = len(patch[0])
-            s.patch_rows, s.patch_ptr, s.patch_cols = (L.ptr(t) for t in patch)
+            s.patch_rows, s.patch_ptr, s.patch_cols = (L.ptr(t) for t in patch[:3])
+            s.patch_grp = L.ptr(patch[3]) if len(patch) > 3 else None
         self._struct = (dtype, s)
         return s
 

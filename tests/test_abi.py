@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, s), s
     assert set(declared_symbols()) == set(_lib.EXPORTS)
     loaded = _lib.load()
-    assert loaded.polar_abi_version() == _lib.ABI_VERSION == 2
+    assert loaded.polar_abi_version() == _lib.ABI_VERSION == 3
 
 
 def test_polar_op_struct_layout():
@@ -36,10 +36,11 @@ def test_polar_op_struct_layout():
     off = {name: getattr(PolarOp, name).offset for name, _ in PolarOp._fields_}
     assert off['indptr'] == 16 and off['diag'] == 40 and off['n_long'] == 48 and off['chunk'] == 60
     assert off['long_rows'] == 64 and off['n_patch'] == 88 and off['cpl_max'] == 92
-    assert off['patch_cols'] == 112 and off['grid_max'] == 120
-    assert ctypes.sizeof(PolarOp) == 128
+    assert off['patch_cols'] == 112 and off['grid_max'] == 120 and off['patch_grp'] == 128
+    assert ctypes.sizeof(PolarOp) == 136
     from polar_amd._lib import PolarSegs
-    assert PolarSegs.rows.offset == 8 and PolarSegs.active.offset == 32 and ctypes.sizeof(PolarSegs) == 40
+    assert PolarSegs.rows.offset == 8 and PolarSegs.active.offset == 32 and PolarSegs.col_group.offset == 40
+    assert ctypes.sizeof(PolarSegs) == 48
 
 
 def test_product_path_has_no_cpu_fallback(monkeypatch):

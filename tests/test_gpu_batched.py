@@ -141,3 +141,52 @@ def test_autotune_keeps_results_and_run_batched_cli(tmp_path, monkeypatch):
         assert out['spmv'] == ref['spmv']
     res = run_batched.main(['--graphs', '3', '--scale', '0.03', '--k', '3', '--tune'])
     assert res['selections_identical'] and set(res['modes']) == {'batched', 'nograph', 'sequential'}
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_msolve_column_groups_equal_single(precision):
+    """Column groups on one graph: each group has its own poles and its own added edges; every
+    group's X, tracked norms and iteration counts equal the single solve bit for bit."""
+    G = _graphs()[1]
+    dt = pg._dt(precision)
+    rng = np.random.default_rng(4)
+    adds = [[(3, 1700), (10, 2999)], [(3, 5), (400, 401), (10, 11)], []]
+    taus = np.stack([pg.fixed_rule(s, 6, G.sigma, G.Lam(4))[0] for s in (0.25, 0.5, 0.75)])
+    widths = [8, 4, 4]
+    Ws = [rng.standard_normal((G.n, w)) for w in widths]
+    act = np.sort(rng.choice(G.n, 900, replace=False))
+    opg = pg.Op(G, precision=dt)
+    for g, es in enumerate(adds):
+        for a, b in es:
+            opg.add(a, b, group=g)
+    cg = np.concatenate([np.full(w, g) for g, w in enumerate(widths)])
+    W = torch.from_numpy(np.concatenate(Ws, axis=1)).cuda().to(dt)
+    segs = batched.single_segment(G, len(act))
+    X, h, it = batched.msolve_batched(opg, segs, W, taus, torch.from_numpy(act.astype(np.int32)).cuda(), 1e-9,
+                                      track=True, col_group=cg, graph=True)
+    off = np.r_[0, np.cumsum(widths)]
+    for g, es in enumerate(adds):
+        op = pg.Op(G, precision=dt)
+        for a, b in es:
+            op.add(a, b)
+        Xs, hs, its = pg.msolve(op, torch.from_numpy(Ws[g]).cuda().to(dt), taus[g], act, 1e-9, track=True)
+        assert np.array_equal(it[0, off[g]:off[g + 1]], its), g
+        assert torch.equal(X[:, :, off[g]:off[g + 1]], Xs), g
+        assert torch.equal(h[:, 0, off[g]:off[g + 1]], hs), g
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_polar_columns_equals_per_s(precision):
+    G = _graphs()[1]
+    cand = graphs.candidates(G.n, G.edges, 400, 12)
+    s_list = [0.25, 0.5, 0.75]
+    poles = [pg.fixed_rule(s, 8, G.sigma, G.Lam(5))[:2] for s in s_list]
+    sk = pg.sketches(G, cand, 16, 21, kind='rademacher', on_device='lazy')
+    out = batched.polar_columns(G, cand, 5, s_list, q=16, B=3, sketch=sk, poles=poles, precision=precision)
+    for g, s in enumerate(s_list):
+        ref = pg.polar(G, cand, 5, s, q=16, B=3, sketch=sk, poles=poles[g], precision=precision)
+        got = out[g]
+        assert got['sequence'] == ref['sequence'], g
+        assert got['marginals'] == ref['marginals'], g
+        assert got['trace_estimate'] == ref['trace_estimate'], g
+        assert (got['spmv'], got['rhs']) == (ref['spmv'], ref['rhs']), g
