@@ -30,6 +30,9 @@ Parity pins (what checks each function; tests/test_oracle.py):
 * polar          -- the reference's own fixture synthetic/scalability/results.json (sequences of
                     Naive and Inc-Full, spmv counts) and B=1 == Naive; dense closed-form greedy.
 * true_marginals -- eigenvalue prefix gains.
+* lanczos_quad   -- with m >= n steps the Gauss rule is exact: z^T f(A) z from the dense
+                    eigendecomposition; slq_greedy -- its per-candidate estimates vs exact
+                    z^T f(M + b b^T) z over the same probes (m = n).
 """
 import math
 import time
@@ -599,3 +602,76 @@ def eval_prefix(M0, cand, seq, s, obj='frac'):
         M[b, a] -= 1
         out.append(base - frac_value(np.linalg.eigvalsh(M), s, obj))
     return np.array(out)
+
+
+# ------------------------------------------------------------------------------------------------
+# SLQ-Greedy baseline (realworld/slq.py:10-58): Hutchinson probes shared by the base matrix and
+# every candidate (common random numbers), m-step Lanczos quadrature for z^T f(A) z.
+def lanczos_quad(apply, Z, m, f):
+    """z_c^T f(A_c) z_c for every column c of Z (n x C) by m-step Lanczos (slq.py:10-21): plain
+    three-term recurrence, no reorthogonalisation; T_c's eigenpairs give the Gauss rule."""
+    n, C = Z.shape
+    nz = np.sqrt(np.sum(Z * Z, 0))
+    V = Z / nz
+    Vp = np.zeros_like(V)
+    al = np.zeros((m, C))
+    be = np.zeros((m, C))
+    b = np.zeros(C)
+    for j in range(m):
+        W = apply(V) - b * Vp
+        a = np.sum(W * V, 0)
+        W = W - a * V
+        b = np.sqrt(np.sum(W * W, 0))
+        al[j] = a
+        be[j] = b
+        Vp = V
+        V = W / np.where(b > 0, b, 1.0)
+    T = np.zeros((C, m, m))
+    idx = np.arange(m)
+    T[:, idx, idx] = al.T
+    T[:, idx[:-1], idx[1:]] = be[:-1].T
+    T[:, idx[1:], idx[:-1]] = be[:-1].T
+    th, U = np.linalg.eigh(T)
+    return nz ** 2 * np.sum(U[:, 0, :] ** 2 * f(np.maximum(th, 1e-300)), axis=1)
+
+
+def slq_greedy(G, cand, k, s, probes=20, m=40, chunk=4000, seed=7):
+    """slq.py:24-58: per round, base = mean_z z^T f(M_S) z and for every live candidate the mean
+    of z^T f(M_S + b b^T) z over the same probes; the argmax of base - that is selected."""
+    f = lambda x: x ** (-s)
+    n = G.n
+    rng = np.random.default_rng(seed)
+    live = np.ones(len(cand), bool)
+    seq = []
+    op = Op(G)
+    spmv = 0
+    ests = []
+    for _ in range(k):
+        Z = rng.choice([-1.0, 1.0], size=(n, probes))
+        base = float(np.mean(lanczos_quad(lambda X: op(X, 0.0), Z, m, f)))
+        spmv += probes * m
+        ids = np.flatnonzero(live)
+        est = np.empty(len(ids))
+        per = max(1, chunk // probes)
+        for lo in range(0, len(ids), per):
+            sub = ids[lo:lo + per]
+            c = len(sub)
+            ea = np.repeat(cand[sub, 0], probes)
+            eb = np.repeat(cand[sub, 1], probes)
+            cols = np.arange(c * probes)
+
+            def apply(X):
+                Y = op(X, 0.0)
+                d = X[ea, cols] - X[eb, cols]
+                Y[ea, cols] += d
+                Y[eb, cols] -= d
+                return Y
+            q = lanczos_quad(apply, np.tile(Z, (1, c)), m, f).reshape(c, probes)
+            est[lo:lo + c] = base - np.mean(q, 1)
+            spmv += c * probes * m
+        e = int(ids[np.argmax(est)])
+        seq.append(e)
+        ests.append(est)
+        live[e] = False
+        op.add(int(cand[e, 0]), int(cand[e, 1]))
+    return dict(sequence=seq, spmv=spmv, probes=probes, lanczos_steps=m, estimates=ests)

@@ -9,8 +9,9 @@ Same CLI flags, JSON schema and file names as the reference (run_real.py:23-133,
 '_evaluator': {...}}}, seconds_total}.  Additions: --sigma (the reference's module global
 SIGMA), --poles P (fixed-count rule instead of `composite`), --kind (sketch kind), --precision,
 --out (results directory) and synthetic graph names (c1, c2, c3, c5, c4_<i>, see configs.py) in
---graphs; SNAP/KONECT names load from $POLAR_DATA like the reference.  `opt` and `param` modes
-are not part of this build.
+--graphs; SNAP/KONECT names load from $POLAR_DATA like the reference.  --slq adds the SLQ-Greedy
+baseline (polar_amd/slq.py) as in the reference.  `opt` and `param` modes are not part of this
+build.
 """
 import argparse
 import json
@@ -136,6 +137,10 @@ def effect(a):
                                     precision=a.precision))
         add('Sketched-Logdet', pg.polar(G, cand, k, 0.5, q=a.q, B=a.B, tol=a.tol, obj='logdet', sketch=sk,
                                         precision=a.precision))
+        if a.slq:                                                               # run_real.py:76-79
+            from . import slq
+            for s in a.s:
+                add(f'SLQ(s={s})', slq.slq_greedy(G, cand, k, s, probes=a.slq_probes, m=a.slq_steps))
         add('Degree', dict(sequence=pg.degree_greedy(G, cand, k)))
         for r in range(a.random):
             add(f'Random#{r}', dict(sequence=pg.random_set(cand, k, 1000 + r)))
@@ -220,6 +225,9 @@ def parser():
     P.add_argument('--skip-naive', action='store_true')
     P.add_argument('--cross-only', action='store_true', help='evaluate Random only under the fractional targets')
     P.add_argument('--pool', choices=['uniform', '2hop'], default='uniform')
+    P.add_argument('--slq', action='store_true')
+    P.add_argument('--slq-probes', type=int, default=20)
+    P.add_argument('--slq-steps', type=int, default=40)
     P.add_argument('--tag', default='')
     P.add_argument('--pool-seed', type=int, default=0)
     P.add_argument('--sigma', type=float, default=1.0, help='the reference module global SIGMA')

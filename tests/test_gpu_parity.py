@@ -259,3 +259,19 @@ def test_msolve_column_compaction(precision):
         assert np.all(itg[0::2] == 0)
         assert np.max(np.abs(Xg - Xo)) <= 1e-4 * np.max(np.abs(Xo))
         assert np.max(np.abs(hg.cpu().numpy() - ho) / np.maximum(ho, 1e-300)) <= 1e-4
+
+
+def test_slq_greedy_matches_oracle():
+    """SLQ-Greedy baseline (realworld/slq.py): same probes, same recurrences; the GPU estimates
+    equal the oracle's to 1e-9 relative and the selections are identical."""
+    from polar_amd import slq
+    n = 300
+    e = graphs.barabasi_albert(n, 3, 8)
+    cand = graphs.candidates(n, e, 40, 9)
+    Gh = pg.Graph(n, e, 0.2)
+    Go = O.Graph(n, e, 0.2)
+    got = slq.slq_greedy(Gh, cand, 3, 0.5, probes=6, m=14, chunk=60, seed=11, return_estimates=True)
+    ref = O.slq_greedy(Go, cand, 3, 0.5, probes=6, m=14, chunk=60, seed=11)
+    assert got['sequence'] == ref['sequence'] and got['spmv'] == ref['spmv']
+    for a, b in zip(got['estimates'], ref['estimates']):
+        assert np.max(np.abs(a - b) / np.abs(b)) <= 1e-9

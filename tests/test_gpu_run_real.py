@@ -20,9 +20,11 @@ def test_time_and_effect_modes(tmp_path):
     for k in ('sequence', 'seconds', 'spmv', 'rhs', 'p', 'rounds', 'truncated'):
         assert k in r['POLAR(B=3)'][0]
     assert d['graph']['N'] == 800 and d['graph']['sigma'] == 0.1
-    run_real.main(['effect', *common, '--random', '2'])
+    run_real.main(['effect', *common, '--random', '2', '--slq', '--slq-probes', '4', '--slq-steps', '10'])
     e = json.loads((tmp_path / 'effect_c1.json').read_text())
-    assert {'POLAR(s=0.5)', 'Sketched-ER', 'Sketched-Logdet', 'Degree', 'Random#0', 'Random#1'} <= set(e['selections'])
+    assert {'POLAR(s=0.5)', 'Sketched-ER', 'Sketched-Logdet', 'Degree', 'Random#0', 'Random#1',
+            'SLQ(s=0.5)'} <= set(e['selections'])
+    assert len(e['gains']['frac(s=0.5)']['SLQ(s=0.5)']) == 4
     g = e['gains']['frac(s=0.5)']
     assert g['_evaluator'] == {'evaluator': 'eigenvalues'} and len(g['POLAR(s=0.5)']) == 4
     assert all(x > 0 for x in g['POLAR(s=0.5)'])

@@ -322,3 +322,45 @@ def test_cr_mode_matches_dense_closed_form():
     assert a['sequence'] == b['sequence']
     assert np.allclose(a['marginals'], b['marginals'], rtol=1e-9)
     assert np.allclose(a['trace_estimate'], b['trace_estimate'], rtol=1e-9)
+
+
+def test_lanczos_quadrature_exact_with_full_krylov_space():
+    """m-step Lanczos quadrature is the Gauss rule of the spectral measure of z: with m >= n steps
+    it is exact, z^T A^-s z from the eigendecomposition (pins the oracle's slq.py restatement)."""
+    n = 14
+    rng = np.random.default_rng(3)
+    e = graphs.barabasi_albert(n, 2, 1)
+    G = O.Graph(n, e, 0.3)
+    A = G.M0.toarray()
+    lam, V = np.linalg.eigh(A)
+    Z = rng.choice([-1.0, 1.0], size=(n, 5))
+    f = lambda x: x ** -0.5
+    exact = np.einsum('ic,ij,j,kj,kc->c', Z, V, f(lam), V, Z)
+    got = O.lanczos_quad(lambda X: A @ X, Z, n, f)
+    assert np.allclose(got, exact, rtol=1e-9, atol=0)
+    # fewer steps: the Gauss rule of degree 2m-1 converges from below for this completely
+    # monotone f (a dropped term or a wrong sign would not), not exact
+    err = [np.max(np.abs(O.lanczos_quad(lambda X: A @ X, Z, m, f) / exact - 1)) for m in (6, 10)]
+    assert 1e-5 < err[0] < 1e-2 and err[1] < 1e-5
+    assert np.all(O.lanczos_quad(lambda X: A @ X, Z, 6, f) <= exact)
+
+
+def test_slq_greedy_estimates_are_trace_differences():
+    """With m = n the SLQ estimate of candidate e is exactly mean_z z^T (f(M) - f(M + b b^T)) z."""
+    n = 16
+    e = graphs.barabasi_albert(n, 2, 4)
+    cand = graphs.candidates(n, e, 6, 2)
+    G = O.Graph(n, e, 0.2)
+    out = O.slq_greedy(G, cand, 2, 0.5, probes=4, m=n, chunk=8, seed=5)
+    A = G.M0.toarray()
+    Z = np.random.default_rng(5).choice([-1.0, 1.0], size=(n, 4))
+    def fq(M):
+        lam, V = np.linalg.eigh(M)
+        return np.mean(np.einsum('ic,ij,j,kj,kc->c', Z, V, lam ** -0.5, V, Z))
+    base = fq(A)
+    want = []
+    for a, b in cand:
+        Mb = A.copy(); Mb[a, a] += 1; Mb[b, b] += 1; Mb[a, b] -= 1; Mb[b, a] -= 1
+        want.append(base - fq(Mb))
+    assert np.allclose(out['estimates'][0], want, rtol=1e-8, atol=1e-12)
+    assert out['sequence'][0] == int(np.argmax(want))
