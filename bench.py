@@ -176,7 +176,37 @@ def cpu_baseline(inp, step_colits, seconds_budget=20.0):
                        f'(n={inp["n"]}, p={inp["p"]}, nA={len(active)}): {per_colit * 1e3:.3f} ms per '
                        f'column-iteration, scaled by the step\'s {step_colits} column-iterations',
                 cpu_model=_cpu_model(), os_cpu_count=os.cpu_count(),
-                affinity=len(os.sched_getaffinity(0)))
+                affinity=len(os.sched_getaffinity(0)), blas_threads=_blas_threads(),
+                note='scipy.sparse products are single-threaded; the oracle runs on one core')
+
+
+def cpu_c1(pg):
+    """The CPU-runnable c1 workload run to completion by the oracle (NumPy/SciPy FP64) and by the
+    FP64 HIP path on the same inputs: BA n=2,000 (m0=5, seed 0), sigma=0.1, s=0.5, q=32
+    Rademacher, 16 poles, N=5,000, k=10, B=1."""
+    from oracle import polar_oracle as O
+    from polar_amd import configs
+    c = configs.make('c1')
+    Go = O.Graph(c['n'], c['edges'], c['sigma'])
+    sk = O.sketches(Go, c['cand'], c['q'], c['seed'], kind='rademacher')
+    kw = dict(q=c['q'], B=c['B'], tol=c['tol'], sketch=sk, poles=(c['tau'], c['om']))
+    t = time.perf_counter()
+    ro = O.polar(Go, c['cand'], c['k'], c['s'], **kw)
+    cpu_s = time.perf_counter() - t
+    G = pg.Graph(c['n'], c['edges'], c['sigma'])
+    pg.polar(G, c['cand'], c['k'], c['s'], **kw)                    # warm
+    rg = pg.polar(G, c['cand'], c['k'], c['s'], **kw)
+    return dict(workload='c1: BA n=2000 m0=5, sigma=0.1, s=0.5, q=32, p=16, N=5000, k=10, B=1, FP64',
+                cpu_seconds=cpu_s, gpu_seconds=rg['seconds'], cpu_over_gpu=cpu_s / rg['seconds'],
+                selections_identical=ro['sequence'] == rg['sequence'], cores=1)
+
+
+def _blas_threads():
+    try:
+        from threadpoolctl import threadpool_info
+        return {i.get('internal_api', i.get('user_api', '?')): i.get('num_threads') for i in threadpool_info()}
+    except Exception:
+        return None
 
 
 def _cpu_model():
@@ -299,6 +329,7 @@ def main():
     cpu = None
     if not a.skip_cpu:
         cpu = cpu_baseline(inp, colits)
+        cpu['c1_to_completion'] = cpu_c1(pg)
         log('cpu', json.dumps(cpu))
     out = dict(metric=METRIC, value=el32, unit='s', n_gpus=1, steps=a.steps, warmup=a.warmup,
                ms_per_step=el32 * 1e3, step_wall_s=wall32, higher_is_better=False, scaling='weak', vs_baseline=None,
