@@ -4,9 +4,10 @@
 Metric (k-round greedy wall time + SpMM GB/s at n=4M; FP32-mixed vs FP64 speedup):
   * step = one full POLAR greedy workload of the c2 configuration: Chung-Lu n=100,000 (gamma 2.5,
     mean degree 20, seed 1), sigma=0.05, q=64 Rademacher, 24 poles, N=1,000,000 candidates,
-    k=20 rounds, B=8, for each s in {0.25, 0.5, 0.75} (the three s values run as column groups of
-    one operator, so every phase is one solve over one adjacency pass; --schedule segments|per-s
-    for the alternatives); inputs resident in HBM.
+    k=20 rounds, B=8, for each s in {0.25, 0.5, 0.75}: the three greedy runs advance together -- the
+    bank as one solve over three row segments, every re-rank solve over the three runs' shortlists
+    as column groups of one operator (one adjacency pass); --schedule columns|segments|per-s for
+    the alternatives, all with the same results; inputs resident in HBM.
     `value` = seconds of one step in the FP32-mixed form, timed as the reference times `seconds`
     (polar_gpu.py:196-253: after the inputs are resident, through the final sync);
     `step_wall_s` is the wall clock around the whole call including its host-side setup.
@@ -78,11 +79,11 @@ def step_seconds(outs, union):
     """The step's timed region as the reference defines `seconds` (polar_gpu.py:196, 252-253): from
     after the inputs are resident (sketch H2D, candidate endpoint table) through the final sync;
     one batched region, or the sum over the per-s runs."""
-    batched = union is not None or (SCHEDULE == 'columns' and len(outs) > 1)
+    batched = union is not None or (SCHEDULE in ('columns', 'hybrid') and len(outs) > 1)
     return outs[0]['seconds'] if batched else float(sum(o['seconds'] for o in outs))
 
 
-SCHEDULE = 'columns'
+SCHEDULE = 'hybrid'
 
 
 def run_step(pg, G, inp, sk, precision, union=None):
@@ -91,11 +92,12 @@ def run_step(pg, G, inp, sk, precision, union=None):
     'segments' -- `union` (a UnionGraph of one copy of G per s) runs them as row segments;
     'per-s'    -- one greedy run after another."""
     poles = [pg.fixed_rule(s, inp['p'], inp['sigma'], G.Lam(inp['k']))[:2] for s in inp['s_all']]
-    if SCHEDULE == 'columns' and len(inp['s_all']) > 1:
+    if SCHEDULE in ('columns', 'hybrid') and len(inp['s_all']) > 1:
         from polar_amd import batched
         return batched.polar_columns(G, inp['cand'], inp['k'], list(inp['s_all']), q=inp['q'], B=inp['B'],
                                      tol=inp['tol'], sketch=sk, poles=poles, precision=precision, graph=GRAPH,
-                                     replace=REPLACE and precision == 'fp32')
+                                     replace=REPLACE and precision == 'fp32',
+                                     bank_union=union if SCHEDULE == 'hybrid' else None)
     if union is not None:
         from polar_amd import batched
         S = len(inp['s_all'])
@@ -230,7 +232,7 @@ def main():
     ap.add_argument('--skip-cpu', action='store_true')
     ap.add_argument('--skip-spmm', action='store_true')
     ap.add_argument('--spmm-scale', type=float, default=1.0)
-    ap.add_argument('--schedule', choices=['columns', 'segments', 'per-s'], default='columns',
+    ap.add_argument('--schedule', choices=['hybrid', 'columns', 'segments', 'per-s'], default='hybrid',
                     help='how the s values of a step share the GPU (same results)')
     ap.add_argument('--graph', choices=['on', 'off'], default='on', help='hipGraph replay of the CG iterations')
     ap.add_argument('--replace', action='store_true', help='FP64 residual replacement in the FP32-mixed step')
@@ -268,7 +270,7 @@ def main():
 
     copy_probe(torch)
     union = None
-    if a.schedule == 'segments' and len(inp['s_all']) > 1:
+    if a.schedule in ('segments', 'hybrid') and len(inp['s_all']) > 1:
         from polar_amd import batched
         union = batched.UnionGraph([G] * len(inp['s_all']))
 
@@ -339,7 +341,8 @@ def main():
                                     f'k={inp["k"]}, B={inp["B"]}, tol={inp["tol"]}',
                            precision='fp32-mixed', column_iterations=colits,
                            hipgraph=GRAPH, residual_replacement=REPLACE,
-                           s_schedule={'columns': f'{len(inp["s_all"])} s values as column groups of one operator',
+                           s_schedule={'hybrid': f'{len(inp["s_all"])} s values: bank as row segments, re-rank as column groups',
+                                       'columns': f'{len(inp["s_all"])} s values as column groups of one operator',
                                        'segments': f'{len(inp["s_all"])} s values as segments of one batched solve',
                                        'per-s': 'one greedy run per s value'}[a.schedule],
                            rhs=int(sum(o['rhs'] for o in outs32))),

@@ -388,12 +388,16 @@ def single_segment(G, nA):
 
 def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac', sketch=None, seed=2026,
                   kind='rademacher', poles=None, precision='fp32', graph=None, width=None, block=None,
-                  replace=False, progress=None):
+                  replace=False, progress=None, bank_union=None):
     """POLAR greedy (polar_gpu.py:180-254) for several s values on ONE graph, in lockstep, as column
     groups of one operator: group g is the run for s_list[g], with its own poles, its own bank
     columns [Z | F_g], its own caches and its own selected edges (patch entries tagged g).  Every
     solve of a phase is one launch sequence that reads the adjacency once for all groups.  Results
-    equal polar_gpu.polar run once per s (same sketch): one dict per s value."""
+    equal polar_gpu.polar run once per s (same sketch): one dict per s value.
+
+    bank_union: a UnionGraph of len(s_list) copies of G; the bank then runs as one segmented solve
+    on it (the groups' 2q columns side by side in rows, finished groups leave the grid) and the
+    re-rank solves as column groups (same results either way)."""
     lib = L.require_gpu()
     dev = pg._device()
     dtype = pg._dt(precision)
@@ -460,26 +464,53 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
     pg._sync()
     t0 = time.perf_counter()
 
-    C = W2 * Gn
-    XY = torch.empty((p, nA, C), dtype=dtype, device=dev)
-    if width is None:
-        width = pg.fit_width(n, p, nA, C, dtype, block=block)
-    col_of = np.arange(C) // W2                                      # group of every bank column
     terms = [[] for _ in range(Gn)]
-    for lo in range(0, C, width):
-        hi = min(C, lo + width)
-        res = msolve_batched(op, segs, RHS[:, lo:hi], tau_all, active, tol, trace=want_trace, out=XY[:, :, lo:hi],
-                             graph=graph, block=block, replace=replace, col_group=col_of[lo:hi])
-        it = res[2][0]
-        te = res[3][:, 0, :].cpu().numpy() if want_trace else None
-        for c in range(lo, hi):
-            g = col_of[c]
-            spmv[g] += it[c - lo]
-            rhs[g] += 1
-            if want_trace and c - g * W2 < q:
-                terms[g].append(oms[g] * te[:, c - lo])
-    Xs = [XY[:, :, g * W2:g * W2 + q] for g in range(Gn)]
-    Ys = [XY[:, :, g * W2 + q:(g + 1) * W2] for g in range(Gn)]
+    if bank_union is not None:
+        U = bank_union
+        assert len(U.graphs) == Gn and all(x is G for x in U.graphs)
+        a0 = np.arange(Gn + 1, dtype=np.int64) * nA
+        segs_u = Segments(*U.segments(a0))
+        act_u = torch.cat([active + int(U.row0[g]) for g in range(Gn)])
+        op_u = pg.Op(U, precision=dtype)
+        RHS_u = torch.empty((U.n, W2), dtype=dtype, device=dev)
+        for g in range(Gn):
+            RHS_u[U.row0[g]:U.row0[g + 1]].copy_(RHS[:, g * W2:(g + 1) * W2])
+        XYu = torch.empty((p, Gn * nA, W2), dtype=dtype, device=dev)
+        wu = width or pg.fit_width(U.n, p, Gn * nA, W2, dtype, block=block)
+        for lo in range(0, W2, wu):
+            hi = min(W2, lo + wu)
+            res = msolve_batched(op_u, segs_u, RHS_u[:, lo:hi], tau_all, act_u, tol, trace=want_trace and lo < q,
+                                 out=XYu[:, :, lo:hi], graph=graph, block=block, replace=replace)
+            for g in range(Gn):
+                spmv[g] += int(res[2][g].sum())
+                rhs[g] += hi - lo
+                if want_trace and lo < q:
+                    te = res[3][:, g, :min(q, hi) - lo].cpu().numpy()
+                    terms[g].append((oms[g][:, None] * te).ravel())
+        del RHS_u
+        Xs = [XYu[:, g * nA:(g + 1) * nA, :q] for g in range(Gn)]
+        Ys = [XYu[:, g * nA:(g + 1) * nA, q:] for g in range(Gn)]
+    else:
+        C = W2 * Gn
+        XY = torch.empty((p, nA, C), dtype=dtype, device=dev)
+        if width is None:
+            width = pg.fit_width(n, p, nA, C, dtype, block=block)
+        col_of = np.arange(C) // W2                                  # group of every bank column
+        for lo in range(0, C, width):
+            hi = min(C, lo + width)
+            res = msolve_batched(op, segs, RHS[:, lo:hi], tau_all, active, tol, trace=want_trace,
+                                 out=XY[:, :, lo:hi], graph=graph, block=block, replace=replace,
+                                 col_group=col_of[lo:hi])
+            it = res[2][0]
+            te = res[3][:, 0, :].cpu().numpy() if want_trace else None
+            for c in range(lo, hi):
+                g = col_of[c]
+                spmv[g] += it[c - lo]
+                rhs[g] += 1
+                if want_trace and c - g * W2 < q:
+                    terms[g].append(oms[g] * te[:, c - lo])
+        Xs = [XY[:, :, g * W2:g * W2 + q] for g in range(Gn)]
+        Ys = [XY[:, :, g * W2 + q:(g + 1) * W2] for g in range(Gn)]
     hs, ts = [], []
     for g in range(Gn):
         h = torch.empty((p, N), dtype=torch.float64, device=dev)

@@ -183,6 +183,11 @@ def test_polar_columns_equals_per_s(precision):
     poles = [pg.fixed_rule(s, 8, G.sigma, G.Lam(5))[:2] for s in s_list]
     sk = pg.sketches(G, cand, 16, 21, kind='rademacher', on_device='lazy')
     out = batched.polar_columns(G, cand, 5, s_list, q=16, B=3, sketch=sk, poles=poles, precision=precision)
+    out_u = batched.polar_columns(G, cand, 5, s_list, q=16, B=3, sketch=sk, poles=poles, precision=precision,
+                                  bank_union=batched.UnionGraph([G] * 3))
+    for a, b in zip(out, out_u):
+        for key in ('sequence', 'marginals', 'trace_estimate', 'spmv', 'rhs'):
+            assert a[key] == b[key], key
     for g, s in enumerate(s_list):
         ref = pg.polar(G, cand, 5, s, q=16, B=3, sketch=sk, poles=poles[g], precision=precision)
         got = out[g]
