@@ -781,7 +781,7 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
 // poles changed in the window.  Each thread keeps its rows' window residuals in registers; one
 // pole's coefficients at a time are staged in LDS and shared by the block's rows.
 constexpr int kW = 8;            // window length of the blocked update
-constexpr int kFlushRows = 2;    // rows per thread per tile
+constexpr int kFlushRows = 1;    // rows per thread per tile (measured: 1 < 2 < 4 in step time)
 constexpr int kMaxPolesFlush = 1024;
 
 template <typename T> struct Ring {
