@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define POLAR_ABI_VERSION 3
+#define POLAR_ABI_VERSION 4
 
 enum polar_dtype { POLAR_F64 = 0, POLAR_F32 = 1 };
 
@@ -197,20 +197,20 @@ int polar_rerank_pick(int32_t obj, int32_t dtype, const void* U, int64_t u_sj, i
  * u + j*u_sj + i*u_si):
  *   prep:  xi[j,:] = X[j,aa]-X[j,bb]; eta[j,:] = gc - (Y[j,aa]-Y[j,bb]); D[j] = 1+u[j,aa]-u[j,bb];
  *          nxi[j] = |xi_j|^2; neta[j] = |eta_j|^2   (coef: double[p*(2q+3)] laid out xi|eta|D|nxi|neta)
- *   rank1: wX[j,i] = X[j,i,:].xi_j then X[j,i,:] -= u[j,i] xi_j / D_j   (one pass, old rows)
- *          wY[j,i] = Y[j,i,:].eta_j then Y[j,i,:] += u[j,i] eta_j / D_j
- *   cache: cc = (u[j,ea]-u[j,eb])/D_j; h = max(0, h - 2cc(wX[ea]-wX[eb]) + cc^2 nxi);
- *          t = max(0, t + 2cc(wY[ea]-wY[eb]) + cc^2 neta). */
+ *   rank1: wX[i,j] = X[j,i,:].xi_j then X[j,i,:] -= u[j,i] xi_j / D_j   (one pass, old rows)
+ *          wY[i,j] = Y[j,i,:].eta_j then Y[j,i,:] += u[j,i] eta_j / D_j;  ut[i,j] = u[j,i]
+ *          (wX, wY, ut: double[nA][p], vertex-major so a pair's endpoint values are contiguous)
+ *   cache: cc = (ut[ea,j]-ut[eb,j])/D_j; h = max(0, h - 2cc(wX[ea,j]-wX[eb,j]) + cc^2 nxi);
+ *          t = max(0, t + 2cc(wY[ea,j]-wY[eb,j]) + cc^2 neta), one thread per pair. */
 int polar_maint_prep(int32_t dtype, const void* X, const void* Y, int64_t sj, int64_t si,
                      int32_t p, int64_t q, const void* u, int64_t u_sj, int64_t u_si, int32_t aa,
                      int32_t bb, const double* gc, double* coef, void* stream);
 int polar_maint_rank1(int32_t dtype, void* X, void* Y, int64_t sj, int64_t si, int32_t p,
                       int64_t nA, int64_t q, const void* u, int64_t u_sj, int64_t u_si,
-                      const double* coef, double* wX, double* wY, void* stream);
-int polar_maint_cache(int32_t dtype, const void* u, int64_t u_sj, int64_t u_si, int32_t p,
-                      int64_t nA, int64_t N, const int32_t* ea, const int32_t* eb,
-                      const double* coef, int64_t q, const double* wX, const double* wY,
-                      double* h, double* t, void* stream);
+                      const double* coef, double* wX, double* wY, double* ut, void* stream);
+int polar_maint_cache(int32_t p, int64_t N, const int32_t* ea, const int32_t* eb, const double* coef,
+                      int64_t q, const double* wX, const double* wY, const double* ut, double* h,
+                      double* t, void* stream);
 
 /* F[a, :] += g; F[b, :] -= g for one row-vector g (double[q]); F row-major, leading dim ldf. */
 int polar_rows_axpy(int32_t dtype, void* F, int64_t ldf, int64_t q, int32_t a, int32_t b,

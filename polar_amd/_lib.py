@@ -13,7 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / 'libpolar_amd.so'
 
 POLAR_F64, POLAR_F32 = 0, 1
 POLAR_TRACK, POLAR_TRACE, POLAR_NOFREEZE, POLAR_GRAPH, POLAR_BLOCK8, POLAR_REPLACE = 1, 2, 4, 8, 16, 32
-ABI_VERSION = 3
+ABI_VERSION = 4
 POLAR_E_ITERCAP, POLAR_E_ARG = -2, -3
 CHUNK = 64  # POLAR_CHUNK: default long-row chunk length
 
@@ -68,8 +68,8 @@ _SIGS = {
     'polar_pair_rhs': (i32, [i32, i64, i32, vp, vp, vp, vp]),
     'polar_rerank_pick': (i32, [i32, i32, vp, i64, i64, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
     'polar_maint_prep': (i32, [i32, vp, vp, i64, i64, i32, i64, vp, i64, i64, i32, i32, vp, vp, vp]),
-    'polar_maint_rank1': (i32, [i32, vp, vp, i64, i64, i32, i64, i64, vp, i64, i64, vp, vp, vp, vp]),
-    'polar_maint_cache': (i32, [i32, vp, i64, i64, i32, i64, i64, vp, vp, vp, i64, vp, vp, vp, vp, vp]),
+    'polar_maint_rank1': (i32, [i32, vp, vp, i64, i64, i32, i64, i64, vp, i64, i64, vp, vp, vp, vp, vp]),
+    'polar_maint_cache': (i32, [i32, i64, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp]),
     'polar_rows_axpy': (i32, [i32, vp, i64, i64, i32, i32, vp, vp]),
     'polar_rademacher': (i32, [i32, u64, i32, vp, i64, i64, f64, vp, i64, vp]),
     'polar_edge_sketch': (i32, [OPP, u64, i64, f64, vp, i64, vp]),

@@ -241,6 +241,7 @@ def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac'
     coef = torch.empty((p, 2 * q + 3), dtype=torch.float64, device=dev)
     wX = torch.empty((p, max(len(a) for a in actives)), dtype=torch.float64, device=dev)
     wY = torch.empty_like(wX)
+    wU = torch.empty_like(wX)
     gvec = torch.empty(max(B, 1) + 2, dtype=torch.float64, device=dev)
     gc_row = torch.empty(q, dtype=torch.float64, device=dev)
     seqs = [[] for _ in range(S)]
@@ -350,10 +351,10 @@ def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac'
                         'maint_prep')
                 L.check(lib.polar_maint_rank1(dcode, L.ptr(Xg), L.ptr(Yg), Xg.stride(0), Xg.stride(1), p, nAg, q,
                                               L.ptr(u), u.stride(0), u.stride(1), L.ptr(coef), L.ptr(wX), L.ptr(wY),
-                                              stream), 'maint_rank1')
-                L.check(lib.polar_maint_cache(dcode, L.ptr(u), u.stride(0), u.stride(1), p, nAg, Ns[g], L.ptr(ea[g]),
-                                              L.ptr(eb[g]), L.ptr(coef), q, L.ptr(wX), L.ptr(wY), L.ptr(hs[g]),
-                                              L.ptr(ts[g]), stream), 'maint_cache')
+                                              L.ptr(wU), stream), 'maint_rank1')
+                L.check(lib.polar_maint_cache(p, Ns[g], L.ptr(ea[g]), L.ptr(eb[g]), L.ptr(coef), q, L.ptr(wX),
+                                              L.ptr(wY), L.ptr(wU), L.ptr(hs[g]), L.ptr(ts[g]), stream),
+                        'maint_cache')
                 if want_trace:
                     tr_hist[g].append(coef[:, 2 * q:2 * q + 2].clone())
             Fv = RHS[:, q:]
@@ -455,6 +456,7 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
     coef = torch.empty((p, 2 * q + 3), dtype=torch.float64, device=dev)
     wX = torch.empty((p, nA), dtype=torch.float64, device=dev)
     wY = torch.empty_like(wX)
+    wU = torch.empty_like(wX)
     gvec = torch.empty(max(B, 1) + 2, dtype=torch.float64, device=dev)
     gc_row = torch.empty(q, dtype=torch.float64, device=dev)
     seqs = [[] for _ in range(Gn)]
@@ -585,10 +587,9 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
                         'maint_prep')
                 L.check(lib.polar_maint_rank1(dcode, L.ptr(Xg), L.ptr(Yg), Xg.stride(0), Xg.stride(1), p, nA, q,
                                               L.ptr(u), u.stride(0), u.stride(1), L.ptr(coef), L.ptr(wX), L.ptr(wY),
-                                              stream), 'maint_rank1')
-                L.check(lib.polar_maint_cache(dcode, L.ptr(u), u.stride(0), u.stride(1), p, nA, N, L.ptr(ea), L.ptr(eb),
-                                              L.ptr(coef), q, L.ptr(wX), L.ptr(wY), L.ptr(hs[g]), L.ptr(ts[g]), stream),
-                        'maint_cache')
+                                              L.ptr(wU), stream), 'maint_rank1')
+                L.check(lib.polar_maint_cache(p, N, L.ptr(ea), L.ptr(eb), L.ptr(coef), q, L.ptr(wX), L.ptr(wY),
+                                              L.ptr(wU), L.ptr(hs[g]), L.ptr(ts[g]), stream), 'maint_cache')
                 if want_trace:
                     tr_hist[g].append(coef[:, 2 * q:2 * q + 2].clone())
             Fv = RHS[:, g * W2 + q:(g + 1) * W2]

@@ -270,7 +270,8 @@ template <typename T, int CPL>
 __global__ __launch_bounds__(kBlock) void k_maint_rank1(T* __restrict__ X, T* __restrict__ Y, int64_t sj, int64_t si,
                                                         int p, int64_t nA, int64_t q, const T* __restrict__ u,
                                                         int64_t u_sj, int64_t u_si, const double* __restrict__ coef,
-                                                        double* __restrict__ wX, double* __restrict__ wY, int L) {
+                                                        double* __restrict__ wX, double* __restrict__ wY,
+                                                        double* __restrict__ ut, int L) {
   Geo g(L);
   const int c0 = g.li * CPL;
   const int64_t gpb = (int64_t)kWaves * g.G;
@@ -309,9 +310,10 @@ __global__ __launch_bounds__(kBlock) void k_maint_rank1(T* __restrict__ X, T* __
     }
     sx = dd_group_sum(sx, L);
     sy = dd_group_sum(sy, L);
-    if (g.li == 0) {
-      wX[item] = sx.hi + sx.lo;
-      wY[item] = sy.hi + sy.lo;
+    if (g.li == 0) {                 // vertex-major: all poles of active row i adjacent
+      wX[i * p + j] = sx.hi + sx.lo;
+      wY[i * p + j] = sy.hi + sy.lo;
+      ut[i * p + j] = ui;
     }
     if (ok) {
 #pragma unroll
@@ -336,30 +338,52 @@ __global__ __launch_bounds__(kBlock) void k_maint_rank1(T* __restrict__ X, T* __
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_maint_cache(const T* __restrict__ u, int64_t u_sj, int64_t u_si, int p,
-                                                        int64_t nA, int64_t N, const int32_t* __restrict__ ea,
+constexpr int kMaintLds = 48 * 1024;   // LDS bytes of one k_maint_cache tile
+
+// h/t cache update (polar_gpu.py:246-247).  A workgroup owns a tile of P consecutive pairs and
+// every pole.  Phase 1 walks (pair, pole) pole-fastest, so the pair's endpoint runs of ut, wX, wY
+// (vertex-major, p doubles per vertex) are read contiguously, and parks cc, dwx, dwy in LDS.
+// Phase 2 walks (pole, pair) pair-fastest, so h[j*N + e] and t[j*N + e] are read and written in
+// contiguous runs of P.  Same arithmetic per (pole, pair) as the unfused formula.
+__global__ __launch_bounds__(kBlock) void k_maint_cache(int p, int64_t N, int P, const int32_t* __restrict__ ea,
                                                         const int32_t* __restrict__ eb, const double* __restrict__ coef,
                                                         int64_t q, const double* __restrict__ wX,
-                                                        const double* __restrict__ wY, double* __restrict__ h,
-                                                        double* __restrict__ t) {
-  const int64_t total = (int64_t)p * N;
+                                                        const double* __restrict__ wY, const double* __restrict__ ut,
+                                                        double* __restrict__ h, double* __restrict__ t) {
+  extern __shared__ double lds[];
+  const int ld = P + 1;                       // padded pole stride: phase-1 stores spread over banks
+  double* s_cc = lds;
+  double* s_dx = lds + (int64_t)p * ld;
+  double* s_dy = lds + 2 * (int64_t)p * ld;
   const int64_t stride = 2 * q + 3;
-  for (int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; item < total;
-       item += (int64_t)gridDim.x * blockDim.x) {
-    const int j = (int)(item / N);
-    const int64_t e = item - (int64_t)j * N;
-    const double* cj = coef + (int64_t)j * stride;
-    const double D = cj[2 * q], nxi = cj[2 * q + 1], neta = cj[2 * q + 2];
-    const int ia = ea[e], ib = eb[e];
-    const T* uj = u + (int64_t)j * u_sj;
-    const double cc = ((double)uj[(int64_t)ia * u_si] - (double)uj[(int64_t)ib * u_si]) / D;
-    const double dwx = wX[(int64_t)j * nA + ia] - wX[(int64_t)j * nA + ib];
-    const double dwy = wY[(int64_t)j * nA + ia] - wY[(int64_t)j * nA + ib];
-    const double hn = (h[item] - ((2.0 * cc) * dwx)) + ((cc * cc) * nxi);
-    const double tn = (t[item] + ((2.0 * cc) * dwy)) + ((cc * cc) * neta);
-    h[item] = hn > 0.0 ? hn : 0.0;
-    t[item] = tn > 0.0 ? tn : 0.0;
+  const int items = P * p;
+  for (int64_t e0 = (int64_t)blockIdx.x * P; e0 < N; e0 += (int64_t)gridDim.x * P) {
+    const int np = (int)((N - e0) < P ? (N - e0) : P);
+    for (int it = threadIdx.x; it < np * p; it += blockDim.x) {
+      const int k = it / p;
+      const int j = it - k * p;
+      const int64_t ia = (int64_t)ea[e0 + k] * p + j, ib = (int64_t)eb[e0 + k] * p + j;
+      const double D = coef[(int64_t)j * stride + 2 * q];
+      s_cc[j * ld + k] = (ut[ia] - ut[ib]) / D;
+      s_dx[j * ld + k] = wX[ia] - wX[ib];
+      s_dy[j * ld + k] = wY[ia] - wY[ib];
+    }
+    __syncthreads();
+    for (int it = threadIdx.x; it < items; it += blockDim.x) {
+      const int j = it / P;
+      const int k = it - j * P;
+      if (k < np) {
+        const double* cj = coef + (int64_t)j * stride;
+        const double nxi = cj[2 * q + 1], neta = cj[2 * q + 2];
+        const double cc = s_cc[j * ld + k], dwx = s_dx[j * ld + k], dwy = s_dy[j * ld + k];
+        const int64_t item = (int64_t)j * N + e0 + k;
+        const double hn = (h[item] - ((2.0 * cc) * dwx)) + ((cc * cc) * nxi);
+        const double tn = (t[item] + ((2.0 * cc) * dwy)) + ((cc * cc) * neta);
+        h[item] = hn > 0.0 ? hn : 0.0;
+        t[item] = tn > 0.0 ? tn : 0.0;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -569,8 +593,8 @@ int polar_maint_prep(int32_t dtype, const void* X, const void* Y, int64_t sj, in
 
 int polar_maint_rank1(int32_t dtype, void* X, void* Y, int64_t sj, int64_t si, int32_t p, int64_t nA, int64_t q,
                       const void* u, int64_t u_sj, int64_t u_si, const double* coef, double* wX, double* wY,
-                      void* stream) {
-  if (!X || !Y || !u || !coef || !wX || !wY || p <= 0 || q <= 0) return set_error(POLAR_E_ARG, "bad maint_rank1 args");
+                      double* ut, void* stream) {
+  if (!X || !Y || !u || !coef || !wX || !wY || !ut || p <= 0 || q <= 0) return set_error(POLAR_E_ARG, "bad maint_rank1 args");
   hipStream_t st = (hipStream_t)stream;
   int cpl = std::min(pick_cpl_rows(dtype, q, X, sj, si), pick_cpl_rows(dtype, q, Y, sj, si));
   int L = lanes_for(q, cpl);
@@ -578,28 +602,28 @@ int polar_maint_rank1(int32_t dtype, void* X, void* Y, int64_t sj, int64_t si, i
   const int gpb = kWaves * (64 / L);
   const int grid = grid_for((int64_t)p * nA, gpb);
   if (dtype == POLAR_F64) {
-    if (cpl == 2) hipLaunchKernelGGL((k_maint_rank1<double, 2>), dim3(grid), dim3(kBlock), 0, st, (double*)X, (double*)Y, sj, si, p, nA, q, (const double*)u, u_sj, u_si, coef, wX, wY, L);
-    else hipLaunchKernelGGL((k_maint_rank1<double, 1>), dim3(grid), dim3(kBlock), 0, st, (double*)X, (double*)Y, sj, si, p, nA, q, (const double*)u, u_sj, u_si, coef, wX, wY, L);
+    if (cpl == 2) hipLaunchKernelGGL((k_maint_rank1<double, 2>), dim3(grid), dim3(kBlock), 0, st, (double*)X, (double*)Y, sj, si, p, nA, q, (const double*)u, u_sj, u_si, coef, wX, wY, ut, L);
+    else hipLaunchKernelGGL((k_maint_rank1<double, 1>), dim3(grid), dim3(kBlock), 0, st, (double*)X, (double*)Y, sj, si, p, nA, q, (const double*)u, u_sj, u_si, coef, wX, wY, ut, L);
   } else {
-    if (cpl == 4) hipLaunchKernelGGL((k_maint_rank1<float, 4>), dim3(grid), dim3(kBlock), 0, st, (float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, L);
-    else if (cpl == 2) hipLaunchKernelGGL((k_maint_rank1<float, 2>), dim3(grid), dim3(kBlock), 0, st, (float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, L);
-    else hipLaunchKernelGGL((k_maint_rank1<float, 1>), dim3(grid), dim3(kBlock), 0, st, (float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, L);
+    if (cpl == 4) hipLaunchKernelGGL((k_maint_rank1<float, 4>), dim3(grid), dim3(kBlock), 0, st, (float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, ut, L);
+    else if (cpl == 2) hipLaunchKernelGGL((k_maint_rank1<float, 2>), dim3(grid), dim3(kBlock), 0, st, (float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, ut, L);
+    else hipLaunchKernelGGL((k_maint_rank1<float, 1>), dim3(grid), dim3(kBlock), 0, st, (float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, ut, L);
   }
   POLAR_LAUNCH_CHECK();
   return 0;
 }
 
-int polar_maint_cache(int32_t dtype, const void* u, int64_t u_sj, int64_t u_si, int32_t p, int64_t nA, int64_t N,
-                      const int32_t* ea, const int32_t* eb, const double* coef, int64_t q, const double* wX,
-                      const double* wY, double* h, double* t, void* stream) {
-  if (!u || !ea || !eb || !coef || !wX || !wY || !h || !t || p <= 0) return set_error(POLAR_E_ARG, "bad maint_cache args");
+int polar_maint_cache(int32_t p, int64_t N, const int32_t* ea, const int32_t* eb, const double* coef, int64_t q,
+                      const double* wX, const double* wY, const double* ut, double* h, double* t, void* stream) {
+  if (!ea || !eb || !coef || !wX || !wY || !ut || !h || !t || p <= 0) return set_error(POLAR_E_ARG, "bad maint_cache args");
   if (N == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
-  const int grid = grid_for((int64_t)p * N, kBlock);
-  if (dtype == POLAR_F64)
-    hipLaunchKernelGGL(k_maint_cache<double>, dim3(grid), dim3(kBlock), 0, st, (const double*)u, u_sj, u_si, p, nA, N, ea, eb, coef, q, wX, wY, h, t);
-  else
-    hipLaunchKernelGGL(k_maint_cache<float>, dim3(grid), dim3(kBlock), 0, st, (const float*)u, u_sj, u_si, p, nA, N, ea, eb, coef, q, wX, wY, h, t);
+  // Pairs per tile: 3 LDS doubles per (pole, pair) within 48 KiB, padded stride P + 1.
+  const int P = std::min<int64_t>(64, (int64_t)kMaintLds / (24 * (int64_t)p) - 1);
+  if (P < 1) return set_error(POLAR_E_ARG, "maint_cache: p=%d poles exceed the LDS tile", p);
+  const size_t lds = (size_t)24 * p * (P + 1);
+  const int grid = grid_for((N + P - 1) / P, 1);
+  hipLaunchKernelGGL(k_maint_cache, dim3(grid), dim3(kBlock), lds, st, p, N, P, ea, eb, coef, q, wX, wY, ut, h, t);
   POLAR_LAUNCH_CHECK();
   return 0;
 }

@@ -595,6 +595,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     coef = torch.empty((p, 2 * q + 3), dtype=torch.float64, device=dev)
     wX = torch.empty((p, nA), dtype=torch.float64, device=dev)
     wY = torch.empty((p, nA), dtype=torch.float64, device=dev)
+    wU = torch.empty((nA, p), dtype=torch.float64, device=dev)
     gvec = torch.empty(max(B, 1) + 2, dtype=torch.float64, device=dev)
     gc_row = torch.empty(q, dtype=torch.float64, device=dev)
     tr_hist = []
@@ -697,11 +698,10 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
                                          u.stride(0), u.stride(1), aa, bb, L.ptr(gc_row), L.ptr(coef), stream),
                     'maint_prep')
             L.check(lib.polar_maint_rank1(dcode, L.ptr(X), L.ptr(Y), X.stride(0), X.stride(1), p, nA, q, L.ptr(u),
-                                          u.stride(0), u.stride(1), L.ptr(coef), L.ptr(wX), L.ptr(wY), stream),
-                    'maint_rank1')
-            L.check(lib.polar_maint_cache(dcode, L.ptr(u), u.stride(0), u.stride(1), p, nA, N, L.ptr(ea), L.ptr(eb),
-                                          L.ptr(coef), q, L.ptr(wX), L.ptr(wY), L.ptr(h), L.ptr(t), stream),
-                    'maint_cache')
+                                          u.stride(0), u.stride(1), L.ptr(coef), L.ptr(wX), L.ptr(wY), L.ptr(wU),
+                                          stream), 'maint_rank1')
+            L.check(lib.polar_maint_cache(p, N, L.ptr(ea), L.ptr(eb), L.ptr(coef), q, L.ptr(wX), L.ptr(wY),
+                                          L.ptr(wU), L.ptr(h), L.ptr(t), stream), 'maint_cache')
             if want_trace:
                 tr_hist.append(coef[:, 2 * q:2 * q + 2].clone())
         Fv = Fb

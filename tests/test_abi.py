@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, s), s
     assert set(declared_symbols()) == set(_lib.EXPORTS)
     loaded = _lib.load()
-    assert loaded.polar_abi_version() == _lib.ABI_VERSION == 3
+    assert loaded.polar_abi_version() == _lib.ABI_VERSION == 4
 
 
 def test_polar_op_struct_layout():
