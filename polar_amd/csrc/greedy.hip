@@ -266,6 +266,37 @@ __global__ __launch_bounds__(kBlock) void k_maint_prep(const T* __restrict__ X, 
 }
 
 // One pass over the old rows: w = row . coef, then row -/+= u * coef / D (X with xi, Y with eta).
+// Each row group loads its next (pole, row) item before it reduces and updates the current one, so
+// a wave keeps two rows of X and Y in flight.  FP64 rows apply u * coef / D exactly as written
+// (the pinned arithmetic); FP32 rows multiply by a reciprocal of D computed once per item (the
+// double-precision difference is far below FP32 rounding).
+template <typename T, int CPL>
+struct RankRow {
+  T xv[CPL], yv[CPL];
+  double ui;
+};
+
+template <typename T, int CPL>
+__device__ __forceinline__ void rank_load(const T* __restrict__ X, const T* __restrict__ Y, int64_t sj, int64_t si,
+                                          int64_t nA, int64_t q, const T* __restrict__ u, int64_t u_sj,
+                                          int64_t u_si, int64_t item, int c0, RankRow<T, CPL>& r) {
+  const int j = (int)(item / nA);
+  const int64_t i = item - (int64_t)j * nA;
+  r.ui = (double)u[(int64_t)j * u_sj + i * u_si];
+  const T* xr = X + (int64_t)j * sj + i * si;
+  const T* yr = Y + (int64_t)j * sj + i * si;
+  if (c0 + CPL <= q) {
+    vload<T, CPL>(xr + c0, r.xv);
+    vload<T, CPL>(yr + c0, r.yv);
+  } else {
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) {
+      r.xv[e] = (c0 + e < q) ? xr[c0 + e] : T(0);
+      r.yv[e] = (c0 + e < q) ? yr[c0 + e] : T(0);
+    }
+  }
+}
+
 template <typename T, int CPL>
 __global__ __launch_bounds__(kBlock) void k_maint_rank1(T* __restrict__ X, T* __restrict__ Y, int64_t sj, int64_t si,
                                                         int p, int64_t nA, int64_t q, const T* __restrict__ u,
@@ -277,34 +308,27 @@ __global__ __launch_bounds__(kBlock) void k_maint_rank1(T* __restrict__ X, T* __
   const int64_t gpb = (int64_t)kWaves * g.G;
   const int64_t total = (int64_t)p * nA;
   const int64_t stride = 2 * q + 3;
-  for (int64_t item = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi; item < total;
-       item += (int64_t)gridDim.x * gpb) {
+  const int64_t step = (int64_t)gridDim.x * gpb;
+  const bool ok = c0 < q;
+  int64_t item = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi;
+  RankRow<T, CPL> cur;
+  if (item < total && ok) rank_load<T, CPL>(X, Y, sj, si, nA, q, u, u_sj, u_si, item, c0, cur);
+  for (; item < total; item += step) {
+    RankRow<T, CPL> nxt;
+    const int64_t nitem = item + step;
+    if (nitem < total && ok) rank_load<T, CPL>(X, Y, sj, si, nA, q, u, u_sj, u_si, nitem, c0, nxt);
     const int j = (int)(item / nA);
     const int64_t i = item - (int64_t)j * nA;
     const double* cj = coef + (int64_t)j * stride;
     const double D = cj[2 * q];
-    const double ui = (double)u[(int64_t)j * u_sj + i * u_si];
-    T* xr = X + (int64_t)j * sj + i * si;
-    T* yr = Y + (int64_t)j * sj + i * si;
+    const double ui = ok ? cur.ui : (double)u[(int64_t)j * u_sj + i * u_si];
     DD sx{0.0, 0.0}, sy{0.0, 0.0};
-    const bool ok = c0 < q;
-    T xv[CPL], yv[CPL];
     if (ok) {
-      if (c0 + CPL <= q) {
-        vload<T, CPL>(xr + c0, xv);
-        vload<T, CPL>(yr + c0, yv);
-      } else {
-#pragma unroll
-        for (int e = 0; e < CPL; ++e) {
-          xv[e] = (c0 + e < q) ? xr[c0 + e] : T(0);
-          yv[e] = (c0 + e < q) ? yr[c0 + e] : T(0);
-        }
-      }
 #pragma unroll
       for (int e = 0; e < CPL; ++e) {
         if (c0 + e < q) {
-          dd_add(sx, (double)xv[e] * cj[c0 + e]);
-          dd_add(sy, (double)yv[e] * cj[q + c0 + e]);
+          dd_add(sx, (double)cur.xv[e] * cj[c0 + e]);
+          dd_add(sy, (double)cur.yv[e] * cj[q + c0 + e]);
         }
       }
     }
@@ -316,11 +340,21 @@ __global__ __launch_bounds__(kBlock) void k_maint_rank1(T* __restrict__ X, T* __
       ut[i * p + j] = ui;
     }
     if (ok) {
+      T* xr = X + (int64_t)j * sj + i * si;
+      T* yr = Y + (int64_t)j * sj + i * si;
+      T xv[CPL], yv[CPL];
+      if constexpr (sizeof(T) == 8) {
 #pragma unroll
-      for (int e = 0; e < CPL; ++e) {
-        if (c0 + e < q) {
-          xv[e] = (T)((double)xv[e] - ((ui * cj[c0 + e]) / D));
-          yv[e] = (T)((double)yv[e] + ((ui * cj[q + c0 + e]) / D));
+        for (int e = 0; e < CPL; ++e) {
+          xv[e] = (T)((double)cur.xv[e] - ((ui * cj[c0 + e]) / D));
+          yv[e] = (T)((double)cur.yv[e] + ((ui * cj[q + c0 + e]) / D));
+        }
+      } else {
+        const double f = ui / D;
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) {
+          xv[e] = (T)((double)cur.xv[e] - f * cj[c0 + e]);
+          yv[e] = (T)((double)cur.yv[e] + f * cj[q + c0 + e]);
         }
       }
       if (c0 + CPL <= q) {
@@ -335,6 +369,7 @@ __global__ __launch_bounds__(kBlock) void k_maint_rank1(T* __restrict__ X, T* __
           }
       }
     }
+    cur = nxt;
   }
 }
 
