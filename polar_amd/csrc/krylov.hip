@@ -255,6 +255,7 @@ struct CGState {
   int T;
   double *cX, *cP;            // [p][T+1][sld]
   int* dirty;                 // [p][sld] (pole, column) changed in the current window
+  int* pdirty;                // [p][S] pole j has a changed column of segment g
   // Residual replacement: the seed iterate x0 (n x ld, FP64) accumulated beside the recurrences;
   // every few windows the residual becomes w - (M + tau0) x0, computed in FP64 from the
   // operator values the solve uses.  NULL when off.
@@ -385,6 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_scalars(CGState s, int nparts, int c
       }
       cp[(int64_t)t * s.sld] = zn;
       s.dirty[idx] = 1;
+      s.pdirty[j * s.S + g] = 1;
     }
     if (s.freeze && m && fabs(zn) * nr <= stop) s.live[idx] = 0;
   }
@@ -778,8 +780,10 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
 
 // Blocked shifted update, once per window of kW iterations (after the window's last iteration):
 // X_j += cX0 P_j + sum_t cXt r_t;  P_j = cP0 P_j + sum_t cPt r_t  on the active rows, for the
-// poles changed in the window.  Each thread keeps its rows' window residuals in registers; one
-// pole's coefficients at a time are staged in LDS and shared by the block's rows.
+// poles changed in the window.  Each thread keeps its rows' window residuals in registers; the
+// poles' coefficients are staged in LDS and shared by the block's rows, in two buffers: the next
+// pole's are staged right after the current pole is applied, so one barrier per pole suffices.
+// The changed poles come from one flag per pole (pdirty).
 constexpr int kW = 8;            // window length of the blocked update
 constexpr int kFlushRows = 1;    // rows per thread per tile (measured: 1 < 2 < 4 in step time)
 constexpr int kMaxPolesFlush = 1024;
@@ -798,18 +802,27 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
   const int vb = sp.g * s.ld;
   __shared__ int s_jl[kMaxPolesFlush];
   __shared__ int s_nj;
-  __shared__ T s_cx[(kW + 1) * 256], s_cp[(kW + 1) * 256];
-  __shared__ int s_dirty[256];
+  // two coefficient buffers: pole jj reads buffer jj & 1, pole jj + 1 is staged into the other
+  constexpr int kMaxR = 64 * (16 / (int)sizeof(T));   // widest solve: 64 lanes of 16-byte vectors
+  __shared__ T s_cx[2][(kW + 1) * kMaxR], s_cp[2][(kW + 1) * kMaxR];
+  __shared__ int s_dirty[2][kMaxR];
   if (threadIdx.x == 0) s_nj = 0;
   __syncthreads();
-  for (int j = threadIdx.x; j < p; j += kBlock) {
-    bool any = false;
-    for (int c = 0; c < R; ++c) any |= s.dirty[(int64_t)j * s.sld + vb + c] != 0;
-    if (any) s_jl[atomicAdd(&s_nj, 1)] = j;     // pole order is irrelevant: poles are independent
-  }
+  for (int j = threadIdx.x; j < p; j += kBlock)
+    if (s.pdirty[j * s.S + sp.g]) s_jl[atomicAdd(&s_nj, 1)] = j;   // poles are independent: any order
   __syncthreads();
   const int nj = s_nj;
   if (nj == 0) return;                           // uniform over the block
+  const int nq = (kW + 1) * R;                   // coefficients per pole (R <= kMaxR)
+  auto stage = [&](int j, int b) {               // pole j's coefficients and column flags -> buffer b
+    for (int q = threadIdx.x; q < nq; q += kBlock) {
+      const int t = q / R, c = q - t * R;
+      const int64_t src = ((int64_t)j * (kW + 1) + t) * s.sld + vb + c;
+      s_cx[b][q] = (T)s.cX[src];
+      s_cp[b][q] = (T)s.cP[src];
+    }
+    for (int c = threadIdx.x; c < R; c += kBlock) s_dirty[b][c] = s.dirty[(int64_t)j * s.sld + vb + c];
+  };
   Geo g(L);
   const int c0 = g.li * CPL;
   const bool colok = c0 < R;
@@ -829,22 +842,17 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
         for (int t = 1; t <= kW; ++t) vload<T, CPL>(ring.s[t % kW] + ar * s.ld + c0, rr[k][t - 1]);
       }
     }
+    __syncthreads();                             // the previous tile's last buffer is consumed
+    stage(s_jl[0], 0);
+    __syncthreads();
     for (int jj = 0; jj < nj; ++jj) {
       const int j = s_jl[jj];
-      __syncthreads();                           // the previous pole's coefficients are consumed
-      for (int q = threadIdx.x; q < (kW + 1) * R; q += kBlock) {
-        const int t = q / R, c = q - t * R;
-        const int64_t src = ((int64_t)j * (kW + 1) + t) * s.sld + vb + c;
-        s_cx[q] = (T)s.cX[src];
-        s_cp[q] = (T)s.cP[src];
-      }
-      for (int c = threadIdx.x; c < R; c += kBlock) s_dirty[c] = s.dirty[(int64_t)j * s.sld + vb + c];
-      __syncthreads();
+      const int b = jj & 1;
       // columns unchanged in the window have identity coefficients: nothing to read or write
       bool mine = false;
       if (colok) {
 #pragma unroll
-        for (int e = 0; e < CPL; ++e) mine |= s_dirty[c0 + e] != 0;
+        for (int e = 0; e < CPL; ++e) mine |= s_dirty[b][c0 + e] != 0;
       }
 #pragma unroll
       for (int k = 0; k < kFlushRows; ++k) {
@@ -861,12 +869,12 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
         }
 #pragma unroll
         for (int e = 0; e < CPL; ++e) {
-          T ax = s_cx[c0 + e] * pv[e];
-          T ap = s_cp[c0 + e] * pv[e];
+          T ax = s_cx[b][c0 + e] * pv[e];
+          T ap = s_cp[b][c0 + e] * pv[e];
 #pragma unroll
           for (int t = 1; t <= kW; ++t) {
-            ax = ax + s_cx[t * R + c0 + e] * rr[k][t - 1][e];
-            ap = ap + s_cp[t * R + c0 + e] * rr[k][t - 1][e];
+            ax = ax + s_cx[b][t * R + c0 + e] * rr[k][t - 1][e];
+            ap = ap + s_cp[b][t * R + c0 + e] * rr[k][t - 1][e];
           }
           xv[e] = xv[e] + ax;
           pv[e] = ap;
@@ -879,6 +887,8 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
         }
         vstore<T, CPL>(pp, pv);
       }
+      if (jj + 1 < nj) stage(s_jl[jj + 1], b ^ 1);   // b ^ 1 was last read before the previous barrier
+      __syncthreads();
     }
   }
 }
@@ -899,6 +909,9 @@ __global__ void k_coef_reset(CGState s) {
     nd += s.dirty[idx] != 0;
     s.dirty[idx] = 0;
   }
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)s.p * s.S;
+       idx += (int64_t)gridDim.x * blockDim.x)
+    s.pdirty[idx] = 0;
   if (nd) atomicAdd(&s.stats[2], nd);
 }
 
@@ -1227,6 +1240,7 @@ static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int
     s.cX = cv.take<double>(pr * (Tw + 1));
     s.cP = cv.take<double>(pr * (Tw + 1));
     s.dirty = cv.take<int>(pr);
+    s.pdirty = cv.take<int>((int64_t)p * S);
   }
   if (repl) s.x0 = cv.take<double>(op->n * R);
   s.cgrp = cv.take<int>(R);
