@@ -787,15 +787,19 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
 constexpr int kW = 8;            // window length of the blocked update
 constexpr int kFlushRows = 1;    // rows per thread per tile (measured: 1 < 2 < 4 in step time)
 constexpr int kMaxPolesFlush = 1024;
+constexpr int kFlushLanes = 8;   // lanes per row of a one-segment flush tile (column slices)
 
 template <typename T> struct Ring {
   const T* s[kW];                // r_t is in s[t % kW]
 };
 
 template <typename T, int CPL>
-__global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int L, Ring<T> ring,
+__global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int L, int cw, Ring<T> ring,
                                                   const int32_t* __restrict__ active, T* __restrict__ P,
                                                   T* __restrict__ Xo, int64_t sj, int64_t si, CGState s) {
+  // column slice blockIdx.y: columns [cbase, cbase + Rs) (cw = L * CPL columns per slice)
+  const int cbase = blockIdx.y * cw;
+  const int Rs = min(cw, R - cbase);
   const SegPart sp = seg_part(s.bS, s.S, blockIdx.x, gridDim.x);
   int64_t a0, a1;
   seg_range(s.seg_a, s.S, sp.g, nA, a0, a1);
@@ -813,19 +817,20 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
   __syncthreads();
   const int nj = s_nj;
   if (nj == 0) return;                           // uniform over the block
-  const int nq = (kW + 1) * R;                   // coefficients per pole (R <= kMaxR)
+  const int nq = (kW + 1) * Rs;                  // coefficients per pole and slice (Rs <= kMaxR)
   auto stage = [&](int j, int b) {               // pole j's coefficients and column flags -> buffer b
     for (int q = threadIdx.x; q < nq; q += kBlock) {
-      const int t = q / R, c = q - t * R;
-      const int64_t src = ((int64_t)j * (kW + 1) + t) * s.sld + vb + c;
+      const int t = q / Rs, c = q - t * Rs;
+      const int64_t src = ((int64_t)j * (kW + 1) + t) * s.sld + vb + cbase + c;
       s_cx[b][q] = (T)s.cX[src];
       s_cp[b][q] = (T)s.cP[src];
     }
-    for (int c = threadIdx.x; c < R; c += kBlock) s_dirty[b][c] = s.dirty[(int64_t)j * s.sld + vb + c];
+    for (int c = threadIdx.x; c < Rs; c += kBlock) s_dirty[b][c] = s.dirty[(int64_t)j * s.sld + vb + cbase + c];
   };
   Geo g(L);
-  const int c0 = g.li * CPL;
-  const bool colok = c0 < R;
+  const int cl = g.li * CPL;                     // column within the slice
+  const int c0 = cbase + cl;
+  const bool colok = cl < Rs;
   const int gpb = kWaves * g.G;
   const int64_t tile = (int64_t)gpb * kFlushRows;
   for (int64_t base = a0 + (int64_t)sp.rank * tile; base < a1; base += (int64_t)sp.nb * tile) {
@@ -852,7 +857,7 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
       bool mine = false;
       if (colok) {
 #pragma unroll
-        for (int e = 0; e < CPL; ++e) mine |= s_dirty[b][c0 + e] != 0;
+        for (int e = 0; e < CPL; ++e) mine |= s_dirty[b][cl + e] != 0;
       }
 #pragma unroll
       for (int k = 0; k < kFlushRows; ++k) {
@@ -869,12 +874,12 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
         }
 #pragma unroll
         for (int e = 0; e < CPL; ++e) {
-          T ax = s_cx[b][c0 + e] * pv[e];
-          T ap = s_cp[b][c0 + e] * pv[e];
+          T ax = s_cx[b][cl + e] * pv[e];
+          T ap = s_cp[b][cl + e] * pv[e];
 #pragma unroll
           for (int t = 1; t <= kW; ++t) {
-            ax = ax + s_cx[b][t * R + c0 + e] * rr[k][t - 1][e];
-            ap = ap + s_cp[b][t * R + c0 + e] * rr[k][t - 1][e];
+            ax = ax + s_cx[b][t * Rs + cl + e] * rr[k][t - 1][e];
+            ap = ap + s_cp[b][t * Rs + cl + e] * rr[k][t - 1][e];
           }
           xv[e] = xv[e] + ax;
           pv[e] = ap;
@@ -1440,12 +1445,23 @@ template <typename T>
 static int flush_window(int cpl, const Plan& pl, const Ring<T>& ring, const int32_t* active, int64_t nA, int p,
                         T* P, T* Xo, int64_t sj, int64_t si, const CGState& s, hipStream_t st) {
   if (nA > 0 && pl.nb_s > 0) {
+    // One segment: column slices of 8 lanes (128-byte row pieces), so a block's tile has 32 rows
+    // and each staged pole serves 4x more rows than a full-width tile at 32 lanes.  Segments keep
+    // full-width tiles (their block partition is the plan's).
+    int Lf = pl.L, nbx = pl.nb_s;
+    if (s.S == 1 && pl.L > kFlushLanes) {
+      Lf = kFlushLanes;
+      const int64_t gpb = (int64_t)kWaves * (64 / Lf);
+      nbx = (int)std::min<int64_t>(kUpdGrid, std::max<int64_t>(1, (nA + gpb - 1) / gpb));
+    }
+    const int cw = Lf * cpl;
+    const dim3 grid(nbx, (s.R + cw - 1) / cw);
     if (cpl == 2) {
-      hipLaunchKernelGGL((k_flush<T, 2>), dim3(pl.nb_s), dim3(kBlock), 0, st, nA, p, s.R, pl.L, ring, active, P, Xo, sj, si, s);
+      hipLaunchKernelGGL((k_flush<T, 2>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s);
     } else if (cpl == 4) {
-      if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_flush<T, 4>), dim3(pl.nb_s), dim3(kBlock), 0, st, nA, p, s.R, pl.L, ring, active, P, Xo, sj, si, s);
+      if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_flush<T, 4>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s);
     } else {
-      hipLaunchKernelGGL((k_flush<T, 1>), dim3(pl.nb_s), dim3(kBlock), 0, st, nA, p, s.R, pl.L, ring, active, P, Xo, sj, si, s);
+      hipLaunchKernelGGL((k_flush<T, 1>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s);
     }
     POLAR_LAUNCH_CHECK();
   }
