@@ -787,7 +787,7 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
 constexpr int kW = 8;            // window length of the blocked update
 constexpr int kFlushRows = 1;    // rows per thread per tile (measured: 1 < 2 < 4 in step time)
 constexpr int kMaxPolesFlush = 1024;
-constexpr int kFlushLanes = 8;   // lanes per row of a one-segment flush tile (column slices)
+constexpr int kFlushLanes = 8;   // lanes per row of a flush tile (column slices of 128 bytes)
 
 template <typename T> struct Ring {
   const T* s[kW];                // r_t is in s[t % kW]
@@ -1080,7 +1080,8 @@ static int g_prof_on = 0;
 struct Plan {
   int cpl, L, G;
   int grid_a, grid_l, grid_rows;   // blocks of the kernels that write partial sums
-  int nb_d, nb_s;                  // k_update: direction part, shifted part
+  int nb_d, nb_s;                  // k_update: direction part, shifted part (or the window flush)
+  int Lf;                          // lanes per row of the shifted part (flush: column slices)
   int fin;                         // finisher grid: one workgroup per (segment, column)
   std::vector<int> tab;            // S > 1: prefix tables bA, bL, bR, bD, bS (S+1 entries each)
 };
@@ -1115,12 +1116,16 @@ static void partition(const std::vector<int64_t>& units, int64_t gpb, int64_t ca
 static int batch_cap(const polar_op* op) { return op->grid_max > 0 ? op->grid_max : kBatchGrid; }
 
 static Plan make_plan(const polar_op* op, int64_t R, int cpl, const Segs* sg = nullptr,
-                      const std::vector<char>* live = nullptr) {
+                      const std::vector<char>* live = nullptr, int Tw = 1) {
   Plan pl;
   pl.cpl = cpl;
   pl.L = lanes_for(R, cpl);
   pl.G = 64 / pl.L;
   const int64_t gpb = (int64_t)kWaves * pl.G;
+  // blocked update: the shifted part is the window flush, whose tiles are column slices of at
+  // most kFlushLanes lanes per row (more rows per tile); otherwise k_update's full-width rows
+  pl.Lf = Tw > 1 ? std::min(pl.L, kFlushLanes) : pl.L;
+  const int64_t gpb_s = (int64_t)kWaves * (64 / pl.Lf);
   const int S = sg ? sg->S : 1;
   pl.fin = (int)(S * R);
   if (S == 1) {
@@ -1131,7 +1136,7 @@ static Plan make_plan(const polar_op* op, int64_t R, int cpl, const Segs* sg = n
     pl.grid_l = (int)std::min<int64_t>(cap, std::max<int64_t>(1, ((int64_t)op->n_long + gpb - 1) / gpb));
     pl.grid_rows = (int)std::min<int64_t>(cap, std::max<int64_t>(1, (op->n + gpb - 1) / gpb));
     pl.nb_d = (int)std::min<int64_t>(kUpdGrid, std::max<int64_t>(1, (op->n + gpb - 1) / gpb));
-    pl.nb_s = (int)std::min<int64_t>(kUpdGrid, std::max<int64_t>(1, (nA + gpb - 1) / gpb));
+    pl.nb_s = (int)std::min<int64_t>(kUpdGrid, std::max<int64_t>(1, (nA + gpb_s - 1) / gpb_s));
     return pl;
   }
   const int64_t cap = batch_cap(op);
@@ -1149,7 +1154,7 @@ static Plan make_plan(const polar_op* op, int64_t R, int cpl, const Segs* sg = n
   partition(ul, gpb, cap, t + (S + 1));
   partition(ur, gpb, cap, t + 2 * (S + 1));
   partition(ur, gpb, kUpdGrid, t + 3 * (S + 1));
-  partition(us, gpb, kUpdGrid, t + 4 * (S + 1));
+  partition(us, gpb_s, kUpdGrid, t + 4 * (S + 1));
   pl.grid_a = t[S];
   pl.grid_l = t[2 * (S + 1) - 1];
   pl.grid_rows = t[3 * (S + 1) - 1];
@@ -1445,17 +1450,11 @@ template <typename T>
 static int flush_window(int cpl, const Plan& pl, const Ring<T>& ring, const int32_t* active, int64_t nA, int p,
                         T* P, T* Xo, int64_t sj, int64_t si, const CGState& s, hipStream_t st) {
   if (nA > 0 && pl.nb_s > 0) {
-    // One segment: column slices of 8 lanes (128-byte row pieces), so a block's tile has 32 rows
-    // and each staged pole serves 4x more rows than a full-width tile at 32 lanes.  Segments keep
-    // full-width tiles (their block partition is the plan's).
-    int Lf = pl.L, nbx = pl.nb_s;
-    if (s.S == 1 && pl.L > kFlushLanes) {
-      Lf = kFlushLanes;
-      const int64_t gpb = (int64_t)kWaves * (64 / Lf);
-      nbx = (int)std::min<int64_t>(kUpdGrid, std::max<int64_t>(1, (nA + gpb - 1) / gpb));
-    }
+    // column slices of pl.Lf <= 8 lanes (128-byte row pieces): a block's tile has 32 rows, and
+    // each staged pole serves 4x more rows than a full-width tile at 32 lanes
+    const int Lf = pl.Lf;
     const int cw = Lf * cpl;
-    const dim3 grid(nbx, (s.R + cw - 1) / cw);
+    const dim3 grid(pl.nb_s, (s.R + cw - 1) / cw);
     if (cpl == 2) {
       hipLaunchKernelGGL((k_flush<T, 2>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s);
     } else if (cpl == 4) {
@@ -1578,7 +1577,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   const int ngroups = std::max(S, sg.G);
   int cpl = cpl0;
   std::vector<char> live(S, 1);
-  Plan pl = make_plan(op, R, cpl, &sg, &live);
+  Plan pl = make_plan(op, R, cpl, &sg, &live, Tw);
   s.nb_a = pl.grid_a;
   if (S > 1) {
     std::vector<int64_t> segh(4 * (S + 1));
@@ -1758,7 +1757,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
         live[g] = on;
       }
       if (changed) {
-        pl = make_plan(op, R, cpl, &sg, &live);
+        pl = make_plan(op, R, cpl, &sg, &live, Tw);
         s.nb_a = pl.grid_a;
         POLAR_HIP(hipMemcpyAsync(m.dtab, pl.tab.data(), sizeof(int) * pl.tab.size(), hipMemcpyHostToDevice, st));
         graph.drop();
@@ -1811,7 +1810,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
       s.R = Rn;
       s.ident = 0;
       cpl = cpl_for<T>(Rn, R, cpl0);
-      pl = make_plan(op, Rn, cpl, &sg, &live);
+      pl = make_plan(op, Rn, cpl, &sg, &live, Tw);
       s.nb_a = pl.grid_a;
       graph.drop();
     }
