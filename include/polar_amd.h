@@ -107,7 +107,9 @@ int polar_abi_version(void);
 const char* polar_last_error(void);
 
 /* Live profiling of the CG iteration kernels with HIP events (no counterpart in the reference;
- * used by the benchmark).  polar_prof_set(1) resets and enables it.  Classes 0..5: k_opmul
+ * used by the benchmark).  polar_prof_set(1) resets and enables it for every class;
+ * polar_prof_set(2) times class 0 only (two events per iteration instead of seven, so the
+ * timed work is barely perturbed); 0 disables it.  Classes 0..5: k_opmul
  * (operator product + d.v partials), k_longfin (long-row chunk combine), k_alpha, k_resid
  * (residual update + |r|^2 partials), k_scalars (per-pole recurrences), k_update (direction and
  * shifted x/p updates); per class the summed event time (ms), launches that did work, and

@@ -1384,18 +1384,18 @@ static int run_iter(const polar_op* op, const Plan& pl, double tau0, const T* ri
     hipLaunchKernelGGL((k_longfin<T, CPL, true>), dim3(pl.grid_l), dim3(kBlock), 0, st, *op, tau0, (const T*)d, (int64_t)s.ld, v, (int64_t)s.ld, R, pl.L, (const T*)lpart, s, cur, pl.grid_a);
     POLAR_LAUNCH_CHECK();
   }
-  if (ev) POLAR_HIP(hipEventRecord(ev[2], st));
+  if (ev && g_prof_on == 1) POLAR_HIP(hipEventRecord(ev[2], st));
   const int nparts_a = pl.grid_a + (longs ? pl.grid_l : 0);
   hipLaunchKernelGGL(k_alpha, dim3(pl.fin), dim3(kBlock), 0, st, s, nparts_a, cur);
   POLAR_LAUNCH_CHECK();
-  if (ev) POLAR_HIP(hipEventRecord(ev[3], st));
+  if (ev && g_prof_on == 1) POLAR_HIP(hipEventRecord(ev[3], st));
   hipLaunchKernelGGL((k_resid<T, CPL>), dim3(pl.grid_rows), dim3(kBlock), 0, st, n, R, pl.L, rin, rout, (const T*)v,
                      (const T*)d, s, cur);
   POLAR_LAUNCH_CHECK();
-  if (ev) POLAR_HIP(hipEventRecord(ev[4], st));
+  if (ev && g_prof_on == 1) POLAR_HIP(hipEventRecord(ev[4], st));
   hipLaunchKernelGGL(k_scalars, dim3(pl.fin), dim3(kBlock), 0, st, s, pl.grid_rows, cur, wt);
   POLAR_LAUNCH_CHECK();
-  if (ev) POLAR_HIP(hipEventRecord(ev[5], st));
+  if (ev && g_prof_on == 1) POLAR_HIP(hipEventRecord(ev[5], st));
   const int nb_d = pl.nb_d, nb_s = s.T > 1 ? 0 : pl.nb_s;    // blocked: the window flush updates X, P
   if (p * R <= 1024)
     hipLaunchKernelGGL((k_update<T, CPL, true>), dim3(nb_d + nb_s), dim3(kBlock), 0, st, n, nA, p, R, pl.L,
@@ -1404,7 +1404,7 @@ static int run_iter(const polar_op* op, const Plan& pl, double tau0, const T* ri
     hipLaunchKernelGGL((k_update<T, CPL, false>), dim3(nb_d + nb_s), dim3(kBlock), 0, st, n, nA, p, R, pl.L,
                        (const T*)rout, d, active, P, Xo, sj, si, nb_d, s, cur);
   POLAR_LAUNCH_CHECK();
-  if (ev) POLAR_HIP(hipEventRecord(ev[6], st));
+  if (ev && g_prof_on == 1) POLAR_HIP(hipEventRecord(ev[6], st));
   return 0;
 }
 
@@ -1713,13 +1713,14 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
         if (rc) return rc;
       }
       if (Tw > 1) {
-        if (prof) {
+        const bool fprof = prof && g_prof_on == 1;
+        if (fprof) {
           for (auto& e : evf)
             if (!e) POLAR_HIP(hipEventCreate(&e));
           POLAR_HIP(hipEventRecord(evf[0], st));
         }
         if (int rc = flush_window<T>(cpl, pl, ring_of(), active, nA, p, P, Xo, sj, si, s, st)) return rc;
-        if (prof) POLAR_HIP(hipEventRecord(evf[1], st));
+        if (fprof) POLAR_HIP(hipEventRecord(evf[1], st));
       }
     }
     POLAR_HIP(hipMemcpyAsync(stat, s.flag, sizeof(int) * (2 + (S > 1 ? S * R : s.R)), hipMemcpyDeviceToHost, st));
@@ -1729,10 +1730,10 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
       for (int k = 0; k < nit; ++k)
         for (int c = 0; c < kEv - 1; ++c) {
           float ms = 0.f;
-          POLAR_HIP(hipEventElapsedTime(&ms, ev[k * kEv + c], ev[k * kEv + c + 1]));
+          if (c == 0 || g_prof_on == 1) POLAR_HIP(hipEventElapsedTime(&ms, ev[k * kEv + c], ev[k * kEv + c + 1]));
           kms.push_back(ms);
         }
-      if (Tw > 1) {
+      if (Tw > 1 && g_prof_on == 1) {
         float ms = 0.f;
         POLAR_HIP(hipEventElapsedTime(&ms, evf[0], evf[1]));
         flush_ms += ms;
