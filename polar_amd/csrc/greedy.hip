@@ -322,21 +322,42 @@ __global__ __launch_bounds__(kBlock) void k_maint_rank1(T* __restrict__ X, T* __
     const double* cj = coef + (int64_t)j * stride;
     const double D = cj[2 * q];
     const double ui = ok ? cur.ui : (double)u[(int64_t)j * u_sj + i * u_si];
-    DD sx{0.0, 0.0}, sy{0.0, 0.0};
-    if (ok) {
+    double wx, wy;
+    if constexpr (sizeof(T) == 8) {  // FP64 rows: double-double dots, rounded once (pinned)
+      DD sx{0.0, 0.0}, sy{0.0, 0.0};
+      if (ok) {
 #pragma unroll
-      for (int e = 0; e < CPL; ++e) {
-        if (c0 + e < q) {
-          dd_add(sx, (double)cur.xv[e] * cj[c0 + e]);
-          dd_add(sy, (double)cur.yv[e] * cj[q + c0 + e]);
+        for (int e = 0; e < CPL; ++e) {
+          if (c0 + e < q) {
+            dd_add(sx, (double)cur.xv[e] * cj[c0 + e]);
+            dd_add(sy, (double)cur.yv[e] * cj[q + c0 + e]);
+          }
         }
       }
+      sx = dd_group_sum(sx, L);
+      sy = dd_group_sum(sy, L);
+      wx = sx.hi + sx.lo;
+      wy = sy.hi + sy.lo;
+    } else {                         // FP32 rows: FP64 dots (the rows carry FP32 rounding anyway)
+      wx = 0.0;
+      wy = 0.0;
+      if (ok) {
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) {
+          if (c0 + e < q) {
+            wx += (double)cur.xv[e] * cj[c0 + e];
+            wy += (double)cur.yv[e] * cj[q + c0 + e];
+          }
+        }
+      }
+      for (int o = L >> 1; o > 0; o >>= 1) {
+        wx += __shfl_xor(wx, o, 64);
+        wy += __shfl_xor(wy, o, 64);
+      }
     }
-    sx = dd_group_sum(sx, L);
-    sy = dd_group_sum(sy, L);
     if (g.li == 0) {                 // vertex-major: all poles of active row i adjacent
-      wX[i * p + j] = sx.hi + sx.lo;
-      wY[i * p + j] = sy.hi + sy.lo;
+      wX[i * p + j] = wx;
+      wY[i * p + j] = wy;
       ut[i * p + j] = ui;
     }
     if (ok) {

@@ -176,12 +176,11 @@ struct ColGroups {
 
 // y = (diag + shift) x - acc + patch(x) for a complete row; returns y in yv.  With column groups
 // each column uses its group's shift and its group's patch entries (same arithmetic per column).
+// finish_row_with: the row's own x (xv) and diagonal (dgd) already loaded.
 template <typename T, int CPL>
-__device__ __forceinline__ void finish_row(const polar_op& op, double shift, const T* __restrict__ X,
-                                           int64_t ldx, int row, int c0, int pidx, const T (&acc)[CPL],
-                                           T (&xv)[CPL], T (&yv)[CPL], const ColGroups& cg) {
-  vload<T, CPL>(X + (int64_t)row * ldx + c0, xv);
-  const double dgd = (double)((const T*)op.diag)[row];
+__device__ __forceinline__ void finish_row_with(const polar_op& op, double shift, const T* __restrict__ X,
+                                                int64_t ldx, int c0, int pidx, const T (&acc)[CPL],
+                                                const T (&xv)[CPL], double dgd, T (&yv)[CPL], const ColGroups& cg) {
   int gcol[CPL];
   if (cg.grp) {
 #pragma unroll
@@ -209,6 +208,15 @@ __device__ __forceinline__ void finish_row(const polar_op& op, double shift, con
         if (pg < 0 || pg == gcol[e]) yv[e] = yv[e] + (xv[e] - xj[e]);
     }
   }
+}
+
+template <typename T, int CPL>
+__device__ __forceinline__ void finish_row(const polar_op& op, double shift, const T* __restrict__ X,
+                                           int64_t ldx, int row, int c0, int pidx, const T (&acc)[CPL],
+                                           T (&xv)[CPL], T (&yv)[CPL], const ColGroups& cg) {
+  vload<T, CPL>(X + (int64_t)row * ldx + c0, xv);
+  const double dgd = (double)((const T*)op.diag)[row];
+  finish_row_with<T, CPL>(op, shift, X, ldx, c0, pidx, acc, xv, dgd, yv, cg);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -503,6 +511,13 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
     T acc[CPL];
 #pragma unroll
     for (int e = 0; e < CPL; ++e) acc[e] = T(0);
+    // a complete row's own x and diagonal are loaded before its gathers (their latency overlaps)
+    T xv[CPL];
+    double dgd = 0.0;
+    if (lidx < 0 && colok) {
+      vload<T, CPL>(X + (int64_t)row * ldx + c0, xv);
+      dgd = (double)((const T*)op.diag)[row];
+    }
     gather_acc<T, CPL, WEIGHTED>(op.indices, (const T*)op.weights, X, ldx, beg, end, g, colok, c0, acc);
     if (lidx < 0) {
       int pidx = -1;
@@ -512,8 +527,8 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
         if (pc < op.n_patch && op.patch_rows[pc] == row) pidx = pc;
       }
       if (colok) {
-        T xv[CPL], yv[CPL];
-        finish_row<T, CPL>(op, shift, X, ldx, row, c0, pidx, acc, xv, yv, ColGroups{s.G > 1 ? s.cgrp : nullptr, s.tau, s.p});
+        T yv[CPL];
+        finish_row_with<T, CPL>(op, shift, X, ldx, c0, pidx, acc, xv, dgd, yv, ColGroups{s.G > 1 ? s.cgrp : nullptr, s.tau, s.p});
         vstore<T, CPL>(Y + (int64_t)row * ldy + c0, yv);
         if (DOT) {
 #pragma unroll
