@@ -593,9 +593,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     sel_ws = torch.empty(lib.polar_select_ws_bytes(N), dtype=torch.uint8, device=dev)
     sel_out = torch.empty(64, dtype=torch.int32, device=dev)
     coef = torch.empty((p, 2 * q + 3), dtype=torch.float64, device=dev)
-    wX = torch.empty((p, nA), dtype=torch.float64, device=dev)
-    wY = torch.empty((p, nA), dtype=torch.float64, device=dev)
-    wU = torch.empty((nA, p), dtype=torch.float64, device=dev)
+    wX = wY = wU = None                 # maintenance buffers, allocated after the bank solves
     gvec = torch.empty(max(B, 1) + 2, dtype=torch.float64, device=dev)
     gc_row = torch.empty(q, dtype=torch.float64, device=dev)
     tr_hist = []
@@ -694,6 +692,10 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             gc_row.copy_(torch.from_numpy(np.ascontiguousarray(Gc[e], dtype=np.float64)))
         ph.mark('rerank_pick')
         if not naive and rnd + 1 < k:                                              # :242-249
+            if wX is None:              # 3 p nA doubles, kept out of the bank's memory budget
+                wX = torch.empty((p, nA), dtype=torch.float64, device=dev)
+                wY = torch.empty((p, nA), dtype=torch.float64, device=dev)
+                wU = torch.empty((nA, p), dtype=torch.float64, device=dev)
             L.check(lib.polar_maint_prep(dcode, L.ptr(X), L.ptr(Y), X.stride(0), X.stride(1), p, q, L.ptr(u),
                                          u.stride(0), u.stride(1), aa, bb, L.ptr(gc_row), L.ptr(coef), stream),
                     'maint_prep')
