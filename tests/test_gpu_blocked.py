@@ -114,3 +114,40 @@ def test_residual_replacement_fp64_matches_pinned_within_tolerance():
     X0 = pg.msolve(op, Wd, tau, act, 1e-10)[0]
     X1 = pg.msolve(op, Wd, tau, act, 1e-10, replace=True)[0]
     assert _rel(X1, X0) <= 1e-8
+
+
+@pytest.mark.parametrize('graph', [False, True])
+def test_blocked_column_slices_fp32(graph):
+    """FP32 at 72 columns: the window flush runs as column slices of 32, 32 and 8 columns (tiles
+    of 32 rows).  CG scalars, tracked norms, trace terms and iteration counts equal the
+    per-iteration update bit for bit; X agrees within FP32 rounding."""
+    G, W, tau, act = _problem(n=2500, R=72, p=20)
+    op = pg.Op(G, precision='fp32')
+    Wd = torch.from_numpy(W).cuda().float()
+    X0, h0, it0, te0 = pg.msolve(op, Wd, tau, act, 1e-6, track=True, trace=True, block=False)
+    X1, h1, it1, te1 = pg.msolve(op, Wd, tau, act, 1e-6, track=True, trace=True, block=True, graph=graph)
+    assert np.array_equal(it0, it1)
+    assert torch.equal(h0, h1) and torch.equal(te0, te1)
+    assert _rel(X1.double(), X0.double()) <= 1e-5
+
+
+def test_blocked_segments_column_slices_equal_single():
+    """Segments whose flush tiles are column slices (FP32, 40 columns: slices of 32 and 8 over the
+    plan's segment partition): batched == per-graph bit for bit."""
+    Gs = [_problem(n, R=8, seed=s)[0] for s, n in enumerate((1200, 2600, 500))]
+    U = batched.UnionGraph(Gs)
+    rng = np.random.default_rng(9)
+    Ws = [rng.standard_normal((G.n, 40)) for G in Gs]
+    acts = [np.sort(rng.choice(G.n, G.n // 2, replace=False)) for G in Gs]
+    tau = np.stack([pg.fixed_rule(s, 12, G.sigma, G.Lam(4))[0] for s, G in zip((0.25, 0.5, 0.75), Gs)])
+    a0 = np.r_[0, np.cumsum([len(a) for a in acts])]
+    segs = batched.Segments(*U.segments(a0))
+    active = torch.from_numpy(np.concatenate([a + U.row0[g] for g, a in enumerate(acts)]).astype(np.int32)).cuda()
+    W = torch.from_numpy(np.concatenate(Ws)).cuda().float()
+    X, h, it = batched.msolve_batched(pg.Op(U, precision='fp32'), segs, W, tau, active, 1e-6, track=True,
+                                      block=True)
+    for g, G in enumerate(Gs):
+        Xs, hs, its = pg.msolve(pg.Op(G, precision='fp32'), torch.from_numpy(Ws[g]).cuda().float(), tau[g],
+                                acts[g], 1e-6, track=True, block=True)
+        assert np.array_equal(it[g], its) and torch.equal(h[:, g], hs)
+        assert torch.equal(X[:, a0[g]:a0[g + 1]], Xs)

@@ -275,3 +275,21 @@ def test_slq_greedy_matches_oracle():
     assert got['sequence'] == ref['sequence'] and got['spmv'] == ref['spmv']
     for a, b in zip(got['estimates'], ref['estimates']):
         assert np.max(np.abs(a - b) / np.abs(b)) <= 1e-9
+
+
+def test_polar_many_poles_fp64_parity():
+    """150 poles: the h/t maintenance runs in LDS tiles of 12 pairs (the last one partial) and
+    the rank-1 pass over 150 poles; the greedy equals the pinned oracle (sequence, counters,
+    marginals and traces within 1e-13)."""
+    c = configs.make('c2', 0.015)
+    tau, om = pg.fixed_rule(0.5, 150, c['sigma'], pg.Graph(c['n'], c['edges'], c['sigma']).Lam(5))[:2]
+    Go = O.Graph(c['n'], c['edges'], c['sigma'])
+    cand = c['cand'][:503]
+    sk = O.sketches(Go, cand, 8, c['seed'], kind='rademacher')
+    kw = dict(q=8, tol=1e-7, sketch=sk, poles=(tau, om))
+    ro = O.polar(Go, cand, 4, 0.5, B=3, width=16, arith='cr', **kw)
+    rg = pg.polar(pg.Graph(c['n'], c['edges'], c['sigma']), cand, 4, 0.5, B=3, **kw)
+    assert rg['sequence'] == ro['sequence']
+    assert rg['spmv'] == ro['spmv'] and rg['p'] == 150
+    assert _rel(rg['marginals'], ro['marginals']) < 1e-13
+    assert _rel(rg['trace_estimate'], ro['trace_estimate']) < 1e-13
