@@ -219,6 +219,8 @@ __device__ __forceinline__ void finish_row(const polar_op& op, double shift, con
   finish_row_with<T, CPL>(op, shift, X, ldx, c0, pidx, acc, xv, dgd, yv, cg);
 }
 
+constexpr int kW = 8;            // window length of the blocked update
+
 // ------------------------------------------------------------------------------------------------
 // CG state (device), laid out in the workspace.
 struct CGState {
@@ -322,42 +324,85 @@ __global__ __launch_bounds__(kBlock) void k_alpha(CGState s, int nparts, int cur
   const int g = blockIdx.x / s.R, c = blockIdx.x - g * s.R;
   if (!seg_live(s, cur, g)) return;
   const int vc = g * s.ld + c;
+  const bool act = s.act[vc] != 0;
+  const double rr = s.rr[vc];
   double dv;
   if (s.S <= 1)
     dv = col_total(s.parts, 0, nparts, 0, 0, s.R, c);
   else
     dv = col_total(s.parts, s.bA[g], s.bA[g + 1], s.nb_a + s.bL[g], s.nb_a + s.bL[g + 1], s.R, c);
-  if (threadIdx.x == 0) s.a[vc] = s.act[vc] ? s.rr[vc] / dv : 0.0;
+  if (threadIdx.x == 0) s.a[vc] = act ? rr / dv : 0.0;
+}
+
+// One (pole, column) pair's state for the scalar step, loaded before the column total so the
+// loads overlap its reduction.
+struct PoleIn {
+  double z, zp, ip, pn, h, tg, te, sh;
+  double cx[kW], cp[kW];           // window coefficients k < t (blocked update)
+  int live;
+};
+
+__device__ __forceinline__ void pole_load(const CGState& s, int j, int idx, int vc, const double* shift, int t,
+                                          PoleIn& q) {
+  q.z = s.z[idx];
+  q.zp = s.zp[idx];
+  q.live = s.live[idx];
+  q.sh = shift[j];
+  if (s.track) {
+    q.ip = s.ip[idx];
+    q.pn = s.pn[idx];
+    q.h = s.h[idx];
+  }
+  if (s.trace) {
+    q.tg = s.tg[idx];
+    q.te = s.te[idx];
+  }
+  if (s.T > 1) {
+    const double* cx = s.cX + (int64_t)j * (s.T + 1) * s.sld + vc;
+    const double* cp = s.cP + (int64_t)j * (s.T + 1) * s.sld + vc;
+#pragma unroll
+    for (int k = 0; k < kW; ++k)
+      if (k < t) {
+        q.cx[k] = cx[(int64_t)k * s.sld];
+        q.cp[k] = cp[(int64_t)k * s.sld];
+      }
+  }
 }
 
 // Finisher after |rn|^2 (one workgroup per (segment, column), threads over poles): every scalar
 // recurrence of polar_gpu.py:106-125 for the column, freezing, convergence and the any-active
-// flags.
+// flags.  Everything that does not depend on this iteration's |r|^2 is loaded first.
 __global__ __launch_bounds__(kBlock) void k_scalars(CGState s, int nparts, int cur, int wt) {
   if (s.flag[cur] == 0) return;
   const int g = blockIdx.x / s.R, c = blockIdx.x - g * s.R;
   if (!seg_live(s, cur, g)) return;
   const int vc = g * s.ld + c;
+  const bool act = s.act[vc] != 0;
+  const double a = s.a[vc], rr = s.rr[vc], bpv = s.bp[vc], apv = s.ap[vc], stop = s.stop[vc];
+  const double* shift = s.shift + (s.G > 1 ? s.cgrp[c] : g) * s.p;
+  const int t = wt + 1;              // window step of the blocked update
+  PoleIn q0;
+  const int j0 = threadIdx.x;
+  if (j0 < s.p) pole_load(s, j0, j0 * s.sld + vc, vc, shift, t, q0);
   int b0, b1;
   rows_parts(s, g, nparts, b0, b1);
   const double rrn = col_total(s.parts, b0, b1, 0, 0, s.R, c);
   __shared__ int nlive_s;
   if (threadIdx.x == 0) nlive_s = 0;
-  const bool act = s.act[vc] != 0;
-  const double a = s.a[vc];
-  const double be = act ? rrn / s.rr[vc] : 0.0;
-  const double cc = act ? a * s.bp[vc] / s.ap[vc] : 0.0;
+  const double be = act ? rrn / rr : 0.0;
+  const double cc = act ? a * bpv / apv : 0.0;
   const double nr = sqrt(rrn);
-  const double stop = s.stop[vc];
-  const double* shift = s.shift + (s.G > 1 ? s.cgrp[c] : g) * s.p;
   __syncthreads();
   int nlive = 0;
-  for (int j = threadIdx.x; j < s.p; j += kBlock) {
+  for (int j = j0; j < s.p; j += kBlock) {
     const int idx = j * s.sld + vc;
-    const double z = s.z[idx], zp = s.zp[idx];
-    const double zn = z / (1.0 + a * shift[j] + cc * (1.0 - z / zp));
+    PoleIn q;
+    if (j == j0) q = q0;
+    else pole_load(s, j, idx, vc, shift, t, q);
+    const double z = q.z, zp = q.zp;
+    const double zn = z / (1.0 + a * q.sh + cc * (1.0 - z / zp));
     const double fac = zn / z;
-    const bool m = (s.live[idx] != 0) && act;
+    const bool m = (q.live != 0) && act;
     const double aj = m ? a * fac : 0.0;
     const double bj = be * fac * fac;
     s.zn[idx] = zn;
@@ -366,32 +411,30 @@ __global__ __launch_bounds__(kBlock) void k_scalars(CGState s, int nparts, int c
     s.m[idx] = m ? 1 : 0;
     nlive += m ? 1 : 0;
     if (s.track) {
-      const double ip = s.ip[idx], pn = s.pn[idx];
-      s.h[idx] += 2.0 * aj * ip + aj * aj * pn;
+      s.h[idx] = q.h + (2.0 * aj * q.ip + aj * aj * q.pn);
       if (m) {
-        s.ip[idx] = bj * (ip + aj * pn);
-        s.pn[idx] = zn * zn * rrn + bj * bj * pn;
+        s.ip[idx] = bj * (q.ip + aj * q.pn);
+        s.pn[idx] = zn * zn * rrn + bj * bj * q.pn;
       }
     }
     if (s.trace) {
-      const double tg = s.tg[idx];
-      s.te[idx] += aj * tg;
-      if (m) s.tg[idx] = bj * tg;
+      s.te[idx] = q.te + aj * q.tg;
+      if (m) s.tg[idx] = bj * q.tg;
     }
     if (m) {
       s.zp[idx] = z;
       s.z[idx] = zn;
     }
     if (s.T > 1 && m) {
-      // window step t = wt + 1: X += aj P, P = zn r_t + bj P in the window basis {P_j, r_1..r_T}
-      const int t = wt + 1;
+      // window step t: X += aj P, P = zn r_t + bj P in the window basis {P_j, r_1..r_T}
       double* cx = s.cX + (int64_t)j * (s.T + 1) * s.sld + vc;
       double* cp = s.cP + (int64_t)j * (s.T + 1) * s.sld + vc;
-      for (int k = 0; k < t; ++k) {
-        const double c = cp[(int64_t)k * s.sld];
-        cx[(int64_t)k * s.sld] += aj * c;
-        cp[(int64_t)k * s.sld] = bj * c;
-      }
+#pragma unroll
+      for (int k = 0; k < kW; ++k)
+        if (k < t) {
+          cx[(int64_t)k * s.sld] = q.cx[k] + aj * q.cp[k];
+          cp[(int64_t)k * s.sld] = bj * q.cp[k];
+        }
       cp[(int64_t)t * s.sld] = zn;
       s.dirty[idx] = 1;
       s.pdirty[j * s.S + g] = 1;
@@ -799,7 +842,6 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
 // poles' coefficients are staged in LDS and shared by the block's rows, in two buffers: the next
 // pole's are staged right after the current pole is applied, so one barrier per pole suffices.
 // The changed poles come from one flag per pole (pdirty).
-constexpr int kW = 8;            // window length of the blocked update
 constexpr int kFlushRows = 1;    // rows per thread per tile (measured: 1 < 2 < 4 in step time)
 constexpr int kMaxPolesFlush = 1024;
 constexpr int kFlushLanes = 8;   // lanes per row of a flush tile (column slices of 128 bytes)
