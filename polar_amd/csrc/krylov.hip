@@ -92,34 +92,32 @@ __device__ __forceinline__ void block_colsum(const Geo& g, const DD (&acc)[CPL],
 }
 
 // Column total of the double-double partial rows b in [b0, b1) u [b2, b3) (parts[b*2R + c] hi,
-// parts[b*2R + R + c] lo) by one workgroup: strided double-double accumulation, then a fixed-shape tree; every thread gets
-// the rounded total.
+// parts[b*2R + R + c] lo) by one workgroup: strided double-double accumulation, a butterfly
+// within each wave, then the waves in order; every thread gets the rounded total.
 __device__ double col_total(const double* __restrict__ parts, int b0, int b1, int b2, int b3, int R, int c) {
-  __shared__ double sh_hi[kBlock];
-  __shared__ double sh_lo[kBlock];
+  __shared__ double sh_hi[kWaves];
+  __shared__ double sh_lo[kWaves];
   DD acc{0.0, 0.0};
   // partial rows [b0, b1) and [b2, b3)
   const int n01 = b1 - b0;
   const int nall = n01 + (b3 - b2);
+#pragma unroll 4
   for (int k = threadIdx.x; k < nall; k += kBlock) {
     const int b = k < n01 ? b0 + k : b2 + (k - n01);
     const double* pb = parts + (int64_t)b * 2 * R;
     acc = dd_merge(acc, DD{pb[c], pb[R + c]});
   }
-  sh_hi[threadIdx.x] = acc.hi;
-  sh_lo[threadIdx.x] = acc.lo;
-  __syncthreads();
-  for (int o = kBlock / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) {
-      const DD m = dd_merge(DD{sh_hi[threadIdx.x], sh_lo[threadIdx.x]}, DD{sh_hi[threadIdx.x + o], sh_lo[threadIdx.x + o]});
-      sh_hi[threadIdx.x] = m.hi;
-      sh_lo[threadIdx.x] = m.lo;
-    }
-    __syncthreads();
+  acc = dd_group_sum(acc, 64);            // butterfly within the wave, then the waves in order
+  if ((threadIdx.x & 63) == 0) {
+    sh_hi[threadIdx.x >> 6] = acc.hi;
+    sh_lo[threadIdx.x >> 6] = acc.lo;
   }
-  const double tot = sh_hi[0] + sh_lo[0];
   __syncthreads();
-  return tot;
+  DD t{sh_hi[0], sh_lo[0]};
+#pragma unroll
+  for (int w = 1; w < kWaves; ++w) t = dd_merge(t, DD{sh_hi[w], sh_lo[w]});
+  __syncthreads();
+  return t.hi + t.lo;
 }
 
 // ------------------------------------------------------------------------------------------------
