@@ -614,6 +614,10 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
       if (pc < op.n_patch && op.patch_rows[pc] == row) pidx = pc;
     }
     if (colok) {
+      // the row's own x and diagonal are loaded before its chunk sums (their latency overlaps)
+      T xv[CPL];
+      vload<T, CPL>(X + (int64_t)row * ldx + c0, xv);
+      const double dgd = (double)((const T*)op.diag)[row];
       T acc[CPL];
 #pragma unroll
       for (int e = 0; e < CPL; ++e) acc[e] = T(0);
@@ -631,8 +635,8 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
             for (int e = 0; e < CPL; ++e) acc[e] += pv[u][e];
           }
       }
-      T xv[CPL], yv[CPL];
-      finish_row<T, CPL>(op, shift, X, ldx, row, c0, pidx, acc, xv, yv, ColGroups{s.G > 1 ? s.cgrp : nullptr, s.tau, s.p});
+      T yv[CPL];
+      finish_row_with<T, CPL>(op, shift, X, ldx, c0, pidx, acc, xv, dgd, yv, ColGroups{s.G > 1 ? s.cgrp : nullptr, s.tau, s.p});
       vstore<T, CPL>(Y + (int64_t)row * ldy + c0, yv);
       if (DOT) {
 #pragma unroll
