@@ -14,7 +14,8 @@ Metric (k-round greedy wall time + SpMM GB/s at n=4M; FP32-mixed vs FP64 speedup
   * the plain FP64 form is timed on the same inputs (fp64_seconds; speedup = fp64/fp32) and its
     selections and marginals are compared with the FP32-mixed ones.
   * SpMM (L + sigma I) X at the c5 shape (Chung-Lu n=4,000,000, m=34.7M, R=128) in FP64 and FP32,
-    GB/s on compulsory bytes.
+    GB/s on compulsory bytes; beside it the same kernel on a 2000 x 2000 lattice (same n, R), a
+    locality control whose neighbour gathers hit cache.
   * roofline: the operator kernel k_opmul (the dominant kernel), its HIP-event durations from the
     library over the timed region (events around that kernel only); `kernel_profile` breaks one
     extra, untimed step down by kernel class (events around every kernel).
@@ -114,15 +115,10 @@ def run_step(pg, G, inp, sk, precision, union=None):
     return outs
 
 
-def spmm_bench(pg, torch, lib, L, n_scale, reps=10):
-    from polar_amd import configs, graphs
-    spec = configs.SPECS['c5']
-    n = int(spec['n'] * n_scale)
-    t = time.time()
-    e = graphs.chung_lu(n, spec['gamma'], spec['mean_deg'], spec['seed'], m=int(spec['m'] * n_scale))
-    log(f'c5 graph n={n} m={len(e)} built in {time.time() - t:.1f}s')
-    G = pg.Graph(n, e, spec['sigma'])
-    R = 128
+def _time_spmm(pg, torch, lib, L, G, R, reps):
+    """polar_spmm on G at width R, FP64 and FP32: ms per product (CUDA events on the launch
+    stream) and GB/s on compulsory bytes (CSR once, X read once, Y written once)."""
+    n = G.n
     res = {}
     for prec, dt, esz in (('fp64', torch.float64, 8), ('fp32', torch.float32, 4)):
         op = pg.Op(G, precision=prec)
@@ -150,7 +146,35 @@ def spmm_bench(pg, torch, lib, L, n_scale, reps=10):
                          frac=ours / ms / 1e6 / HBM_PEAK_GBPS)
         del X, Y, ws
         torch.cuda.empty_cache()
+    return res
+
+
+def spmm_bench(pg, torch, lib, L, n_scale, reps=10):
+    from polar_amd import configs, graphs
+    spec = configs.SPECS['c5']
+    n = int(spec['n'] * n_scale)
+    t = time.time()
+    e = graphs.chung_lu(n, spec['gamma'], spec['mean_deg'], spec['seed'], m=int(spec['m'] * n_scale))
+    log(f'c5 graph n={n} m={len(e)} built in {time.time() - t:.1f}s')
+    G = pg.Graph(n, e, spec['sigma'])
+    R = 128
+    res = _time_spmm(pg, torch, lib, L, G, R, reps)
     return dict(n=n, m=len(e), nnz_offdiag=G.nnz, R=R, dmax=G.dmax, **res)
+
+
+def spmm_lattice_bench(pg, torch, lib, L, n_scale, reps=10):
+    """Locality control for the SpMM: a 2-D lattice with the c5 row count (4-neighbour, side
+    sqrt(n)), whose neighbour rows i +- 1 and i +- side stay cache-resident.  Same kernel, same
+    R; it shows what the kernel reaches when the gathers hit cache."""
+    side = int(round(math.sqrt(4_000_000 * n_scale)))
+    n = side * side
+    idx = np.arange(n, dtype=np.int64).reshape(side, side)
+    e = np.concatenate([np.stack([idx[:, :-1].ravel(), idx[:, 1:].ravel()], 1),
+                        np.stack([idx[:-1, :].ravel(), idx[1:, :].ravel()], 1)])
+    G = pg.Graph(n, e, 0.02)
+    R = 128
+    res = _time_spmm(pg, torch, lib, L, G, R, reps)
+    return dict(n=n, m=len(e), nnz_offdiag=G.nnz, R=R, graph=f'{side}x{side} lattice', **res)
 
 
 def cpu_baseline(inp, step_colits, seconds_budget=20.0):
@@ -259,7 +283,8 @@ def main():
 
     if a.spmm_only:
         copy_probe(torch)
-        print(json.dumps(dict(spmm_n4m=spmm_bench(pg, torch, lib, L, a.spmm_scale))), flush=True)
+        print(json.dumps(dict(spmm_n4m=spmm_bench(pg, torch, lib, L, a.spmm_scale),
+                              spmm_lattice_n4m=spmm_lattice_bench(pg, torch, lib, L, a.spmm_scale))), flush=True)
         return
 
     t = time.time()
@@ -326,10 +351,12 @@ def main():
         agree = dict(selections_identical=same, max_rel_marginal_err=mrel, max_rel_trace_err=trel)
         log('fp64 step %.3fs' % el64, json.dumps(agree))
 
-    spmm = None
+    spmm = lattice = None
     if not a.skip_spmm:
         spmm = spmm_bench(pg, torch, lib, L, a.spmm_scale)
         log('spmm', json.dumps(spmm))
+        lattice = spmm_lattice_bench(pg, torch, lib, L, a.spmm_scale)
+        log('spmm lattice', json.dumps(lattice))
 
     colits = int(sum(o['spmv'] for o in outs32))
     dom = max(KERNELS, key=lambda k: prof[k]['ms'])       # k_opmul on every config measured
@@ -363,7 +390,7 @@ def main():
                                        'per-s': 'one greedy run per s value'}[a.schedule],
                            rhs=int(sum(o['rhs'] for o in outs32))),
                fp64_seconds=el64, fp32_vs_fp64_speedup=(el64 / el32) if el64 else None,
-               fp32_vs_fp64=agree, spmm_n4m=spmm, roofline=roofline, cpu_baseline=cpu,
+               fp32_vs_fp64=agree, spmm_n4m=spmm, spmm_lattice_n4m=lattice, roofline=roofline, cpu_baseline=cpu,
                kernel_profile=prof, kernel_profile_note='one extra untimed FP32 step with HIP events around '
                'every kernel class (the timed steps carry events around k_opmul only)')
     print(json.dumps(out), flush=True)

@@ -531,16 +531,32 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
 #pragma unroll
   for (int e = 0; e < CPL; ++e) dacc[e] = DD{0.0, 0.0};
   int pc = -1;
+  // row extents of the next row unit, loaded one unit ahead so that their latency overlaps the
+  // current unit's gathers
+  int nbeg = 0, nend = 0;
+  if (u >= nx && u < nunits) {
+    nbeg = op.indptr[r0 + (u - nx)];
+    nend = op.indptr[r0 + (u - nx) + 1];
+  }
   // the segment's units [0, nx): every chunk of its long rows (scheduled first: their latency
   // chains overlap the rest); units [nx, nx + rows): the remaining rows
   for (; u < nunits; u += stride) {
     int row, beg, end, lidx = -1, chunk = 0;
+    const int64_t un = u + stride;
     if (u >= nx) {
       row = (int)(r0 + (u - nx));
-      beg = op.indptr[row];
-      end = op.indptr[row + 1];
+      beg = nbeg;
+      end = nend;
+      if (un < nunits) {
+        nbeg = op.indptr[r0 + (un - nx)];
+        nend = op.indptr[r0 + (un - nx) + 1];
+      }
       if (end - beg > CH) continue;    // long row: all its chunks are units of the chunk list
     } else {
+      if (un >= nx && un < nunits) {
+        nbeg = op.indptr[r0 + (un - nx)];
+        nend = op.indptr[r0 + (un - nx) + 1];
+      }
       const int2 ex = reinterpret_cast<const int2*>(op.extra)[x0 + u];
       lidx = ex.x;
       chunk = ex.y;
