@@ -180,7 +180,8 @@ def timing(a):
             free, total = torch.cuda.mem_get_info()
             meta.update(device_memory_total=int(total), device_memory_free_at_start=int(free))
         res = dict(mode='time', graph=meta, params=dict(k=a.k, q=a.q, B=a.B, eps=a.eps, tol=a.tol, s=a.s[0],
-                   repetitions=a.reps, precision=a.precision, poles=a.poles or 'composite', block=a.block),
+                   repetitions=a.reps, precision=a.precision, poles=a.poles or 'composite', block=a.block,
+                   shortlist=a.shortlist),
                    env=env(), runs={})
         cfg = {'Naive': dict(naive=True), 'POLAR(B=1)': dict(B=1), f'POLAR(B={a.B})': dict(B=a.B)}
         if a.skip_naive:
@@ -199,7 +200,8 @@ def timing(a):
                 lim = a.naive_limit if nm == 'Naive' else None
                 out = pg.polar(G, cand, a.k, a.s[0], q=a.q, eps=a.eps, tol=a.tol, sketch=sk, time_limit=lim,
                                poles=_poles(a, G, a.k, a.s[0], 'frac'), precision=a.precision,
-                               progress=_progress(a, name, nm), block=_block(a), **{'B': a.B, **cfg[nm]})
+                               progress=_progress(a, name, nm), block=_block(a), shortlist=a.shortlist,
+                               **{'B': a.B, **cfg[nm]})
                 res['runs'].setdefault(nm, []).append(_clean(out))
                 print(name, nm, rep, f"{out['seconds']:.2f}s spmv={out['spmv']}", flush=True)
                 out_dir.mkdir(parents=True, exist_ok=True)
@@ -239,6 +241,8 @@ def parser():
     P.add_argument('--only', nargs='+', default=None, help='time only these runs (Naive, B1, B)')
     P.add_argument('--block', choices=['auto', 'on', 'off'], default='auto',
                    help='blocked shifted update (auto: on for fp32, off for fp64)')
+    P.add_argument('--shortlist', choices=['proxy', 'gradient'], default='proxy',
+                   help='B-edge shortlist: the reference proxy or the first-order (gradient) gain')
     return P
 
 
