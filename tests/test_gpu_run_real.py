@@ -29,3 +29,16 @@ def test_time_and_effect_modes(tmp_path):
     assert g['_evaluator'] == {'evaluator': 'eigenvalues'} and len(g['POLAR(s=0.5)']) == 4
     assert all(x > 0 for x in g['POLAR(s=0.5)'])
     assert g['POLAR(s=0.5)'][-1] >= g['Random#0'][-1]
+
+
+def test_time_mode_gradient_shortlist(tmp_path):
+    """--shortlist gradient: the first-order gain ranks the shortlist; the re-rank by exact
+    marginals is unchanged, so the run keeps the schema and the rhs counter."""
+    from polar_amd import run_real
+    run_real.main(['time', '--graphs', 'c1', '--N', '800', '--k', '4', '--q', '16', '--B', '3', '--s', '0.5',
+                   '--sigma', '0.1', '--poles', '12', '--kind', 'rademacher', '--out', str(tmp_path),
+                   '--reps', '1', '--only', 'B', '--shortlist', 'gradient'])
+    d = json.loads((tmp_path / 'time_c1.json').read_text())
+    assert d['params']['shortlist'] == 'gradient' and set(d['runs']) == {'POLAR(B=3)'}
+    r = d['runs']['POLAR(B=3)'][0]
+    assert r['rhs'] == 2 * 16 + 4 * 3 and len(r['sequence']) == 4 and len(set(r['sequence'])) == 4
