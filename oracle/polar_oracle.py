@@ -1,6 +1,6 @@
 """CPU FP64 oracle for the pole-shared Krylov core.  TEST INFRASTRUCTURE ONLY.
 
-This module is the checker, never the product: only ``tests/``, ``__graft_entry__.smoke()`` and
+This module checks the product and is never part of it: only ``tests/``, ``__graft_entry__.smoke()`` and
 ``bench.py``'s CPU-baseline leg may import it.  The product path (``polar_amd``) never imports it
 and fails loudly when its HIP library is missing.
 

@@ -1,5 +1,5 @@
 """Batched-form CLI: the 64-graph R-MAT workload run as one batch, against the one-graph-at-a-time
-loop the reference's driver performs (realworld/run_real.py:114-133 calls polar_gpu.polar per
+loop of the reference's harness (realworld/run_real.py:114-133 calls polar_gpu.polar per
 graph).
 
     python -m polar_amd.run_batched [--graphs 64] [--scale 1] [--precision fp32] [--tune]
