@@ -8,6 +8,10 @@
  * except polar_msolve (a 4-byte convergence-flag read every 8 iterations, as the reference's
  * host check at realworld/polar_gpu.py:126) and polar_select_topb / polar_rerank_pick, whose
  * results are the host-side shortlist/winner the reference copies back (polar_gpu.py:221, :236).
+ * Memory that must start at zero is cleared by a fill kernel, never by hipMemsetAsync, so a
+ * caller that captures an entry point without a host sync into a HIP graph records no memset
+ * node.  Sizes, pointers and dtype codes are checked on the host before anything is launched
+ * (POLAR_E_ARG).
  *
  * Reference interfaces replaced (chenty2333/POLAR):
  *   realworld/spmm.py:3-25          csr_spmm(n, R, ptr, idx, val, X, Y)   -> polar_spmm

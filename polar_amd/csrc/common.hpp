@@ -15,6 +15,9 @@ namespace polar {
 
 int set_error(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what, int line);
+// Clears `bytes` bytes (a multiple of 4) at device address p with a fill kernel on stream st;
+// used instead of hipMemsetAsync, so a caller that captures the stream records no memset node.
+int zero_async(void* p, size_t bytes, hipStream_t st);
 
 #define POLAR_HIP(x)                                   \
   do {                                                 \
@@ -120,5 +123,6 @@ __device__ __forceinline__ DD dd_group_sum(DD v, int L) {
 }
 
 inline int elem_size(int dtype) { return dtype == POLAR_F64 ? 8 : 4; }
+inline bool dtype_ok(int dtype) { return dtype == POLAR_F64 || dtype == POLAR_F32; }
 
 }  // namespace polar

@@ -517,7 +517,8 @@ extern "C" {
 
 int polar_pair_sqnorm(int32_t dtype, const void* X, int64_t x_sj, int64_t x_si, int32_t p, int64_t q,
                       const int32_t* ea, const int32_t* eb, int64_t N, double* out, void* stream) {
-  if (!X || !ea || !eb || !out || p <= 0 || q <= 0 || N < 0) return set_error(POLAR_E_ARG, "bad pair_sqnorm args");
+  if (!dtype_ok(dtype) || !X || !ea || !eb || !out || p <= 0 || q <= 0 || N < 0)
+    return set_error(POLAR_E_ARG, "bad pair_sqnorm args");
   if (N == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   int cpl = pick_cpl_rows(dtype, q, X, x_sj, x_si);
@@ -545,7 +546,8 @@ int polar_pair_sqnorm(int32_t dtype, const void* X, int64_t x_sj, int64_t x_si, 
 int polar_pair_wsum(int32_t dtype, const void* X, int64_t x_sj, int64_t x_si, int32_t p, int64_t q,
                     const int32_t* ea, const int32_t* eb, int64_t E, const double* om, double* out,
                     int32_t accumulate, void* stream) {
-  if (!X || !ea || !eb || !om || !out || p <= 0 || q <= 0 || E < 0) return set_error(POLAR_E_ARG, "bad pair_wsum args");
+  if (!dtype_ok(dtype) || !X || !ea || !eb || !om || !out || p <= 0 || q <= 0 || E < 0)
+    return set_error(POLAR_E_ARG, "bad pair_wsum args");
   if (E == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   const int cpl = pick_cpl_rows(dtype, q, X, x_sj, x_si);
@@ -566,7 +568,7 @@ int polar_pair_wsum(int32_t dtype, const void* X, int64_t x_sj, int64_t x_si, in
 
 int polar_proxy(int32_t obj, const double* h, const double* t, const double* tau, const double* om, int32_t p,
                 int64_t N, double sigma, const uint8_t* dead, double* score, void* stream) {
-  if (!h || !t || !tau || !om || !score || p <= 0) return set_error(POLAR_E_ARG, "bad proxy args");
+  if (!h || !t || !tau || !om || !score || p <= 0 || N < 0) return set_error(POLAR_E_ARG, "bad proxy args");
   if (N == 0) return 0;
   hipLaunchKernelGGL(k_proxy, dim3(grid_for(N, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, obj, h, t, tau,
                      om, p, N, sigma, dead, score);
@@ -605,9 +607,9 @@ int polar_select_topb(const double* score, int64_t N, int32_t Br, int32_t* out, 
 }
 
 int polar_pair_rhs(int32_t dtype, int64_t n, int32_t Br, const int32_t* a, const int32_t* b, void* W, void* stream) {
-  if (!a || !b || !W || Br <= 0 || n <= 0) return set_error(POLAR_E_ARG, "bad pair_rhs args");
+  if (!dtype_ok(dtype) || !a || !b || !W || Br <= 0 || n <= 0) return set_error(POLAR_E_ARG, "bad pair_rhs args");
   hipStream_t st = (hipStream_t)stream;
-  POLAR_HIP(hipMemsetAsync(W, 0, (size_t)n * Br * elem_size(dtype), st));
+  if (int rc = zero_async(W, (size_t)n * Br * elem_size(dtype), st)) return rc;
   if (dtype == POLAR_F64) hipLaunchKernelGGL(k_pair_rhs<double>, dim3((Br + 63) / 64), dim3(64), 0, st, Br, a, b, (double*)W);
   else hipLaunchKernelGGL(k_pair_rhs<float>, dim3((Br + 63) / 64), dim3(64), 0, st, Br, a, b, (float*)W);
   POLAR_LAUNCH_CHECK();
@@ -617,7 +619,8 @@ int polar_pair_rhs(int32_t dtype, int64_t n, int32_t Br, const int32_t* a, const
 int polar_rerank_pick(int32_t obj, int32_t dtype, const void* U, int64_t u_sj, int64_t u_si, int32_t p, int32_t Br,
                       const int32_t* sa, const int32_t* sb, const double* hh, const double* om, double* g,
                       int32_t* host_winner, double* host_value, void* stream) {
-  if (!U || !sa || !sb || !om || !g || Br <= 0 || p <= 0) return set_error(POLAR_E_ARG, "bad rerank args");
+  if (!dtype_ok(dtype) || !U || !sa || !sb || !om || !g || Br <= 0 || p <= 0)
+    return set_error(POLAR_E_ARG, "bad rerank args");
   if (obj != 2 && !hh) return set_error(POLAR_E_ARG, "rerank needs tracked norms");
   hipStream_t st = (hipStream_t)stream;
   double* res = g + Br;  // caller provides Br + 2 doubles
@@ -637,7 +640,8 @@ int polar_rerank_pick(int32_t obj, int32_t dtype, const void* U, int64_t u_sj, i
 int polar_maint_prep(int32_t dtype, const void* X, const void* Y, int64_t sj, int64_t si, int32_t p, int64_t q,
                      const void* u, int64_t u_sj, int64_t u_si, int32_t aa, int32_t bb, const double* gc,
                      double* coef, void* stream) {
-  if (!X || !Y || !u || !gc || !coef || p <= 0 || q <= 0) return set_error(POLAR_E_ARG, "bad maint_prep args");
+  if (!dtype_ok(dtype) || !X || !Y || !u || !gc || !coef || p <= 0 || q <= 0)
+    return set_error(POLAR_E_ARG, "bad maint_prep args");
   hipStream_t st = (hipStream_t)stream;
   if (dtype == POLAR_F64)
     hipLaunchKernelGGL(k_maint_prep<double>, dim3(p), dim3(kBlock), 0, st, (const double*)X, (const double*)Y, sj, si, q, (const double*)u, u_sj, u_si, aa, bb, gc, coef);
@@ -650,7 +654,9 @@ int polar_maint_prep(int32_t dtype, const void* X, const void* Y, int64_t sj, in
 int polar_maint_rank1(int32_t dtype, void* X, void* Y, int64_t sj, int64_t si, int32_t p, int64_t nA, int64_t q,
                       const void* u, int64_t u_sj, int64_t u_si, const double* coef, double* wX, double* wY,
                       double* ut, void* stream) {
-  if (!X || !Y || !u || !coef || !wX || !wY || !ut || p <= 0 || q <= 0) return set_error(POLAR_E_ARG, "bad maint_rank1 args");
+  if (!dtype_ok(dtype) || !X || !Y || !u || !coef || !wX || !wY || !ut || p <= 0 || q <= 0 || nA < 0)
+    return set_error(POLAR_E_ARG, "bad maint_rank1 args");
+  if (nA == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   int cpl = std::min(pick_cpl_rows(dtype, q, X, sj, si), pick_cpl_rows(dtype, q, Y, sj, si));
   int L = lanes_for(q, cpl);
@@ -671,7 +677,8 @@ int polar_maint_rank1(int32_t dtype, void* X, void* Y, int64_t sj, int64_t si, i
 
 int polar_maint_cache(int32_t p, int64_t N, const int32_t* ea, const int32_t* eb, const double* coef, int64_t q,
                       const double* wX, const double* wY, const double* ut, double* h, double* t, void* stream) {
-  if (!ea || !eb || !coef || !wX || !wY || !ut || !h || !t || p <= 0) return set_error(POLAR_E_ARG, "bad maint_cache args");
+  if (!ea || !eb || !coef || !wX || !wY || !ut || !h || !t || p <= 0 || q <= 0 || N < 0)
+    return set_error(POLAR_E_ARG, "bad maint_cache args");
   if (N == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   // Pairs per tile: 3 LDS doubles per (pole, pair) within 48 KiB, padded stride P + 1.
@@ -685,7 +692,7 @@ int polar_maint_cache(int32_t p, int64_t N, const int32_t* ea, const int32_t* eb
 }
 
 int polar_rows_axpy(int32_t dtype, void* F, int64_t ldf, int64_t q, int32_t a, int32_t b, const double* g, void* stream) {
-  if (!F || !g || q <= 0) return set_error(POLAR_E_ARG, "bad rows_axpy args");
+  if (!dtype_ok(dtype) || !F || !g || q <= 0 || ldf < q || a < 0 || b < 0) return set_error(POLAR_E_ARG, "bad rows_axpy args");
   hipStream_t st = (hipStream_t)stream;
   if (dtype == POLAR_F64) hipLaunchKernelGGL(k_rows_axpy<double>, dim3(1), dim3(256), 0, st, (double*)F, ldf, q, a, b, g);
   else hipLaunchKernelGGL(k_rows_axpy<float>, dim3(1), dim3(256), 0, st, (float*)F, ldf, q, a, b, g);
@@ -695,7 +702,7 @@ int polar_rows_axpy(int32_t dtype, void* F, int64_t ldf, int64_t q, int32_t a, i
 
 int polar_rademacher(int32_t dtype, uint64_t seed, int32_t stream_id, const int64_t* keys, int64_t rows, int64_t q,
                      double scale, void* out, int64_t ld, void* stream) {
-  if (!out || rows < 0 || q <= 0 || ld < q) return set_error(POLAR_E_ARG, "bad rademacher args");
+  if (!dtype_ok(dtype) || !out || rows < 0 || q <= 0 || ld < q) return set_error(POLAR_E_ARG, "bad rademacher args");
   if (rows == 0) return 0;
   const uint64_t base = splitmix64(seed * 4 + (uint64_t)stream_id);
   hipStream_t st = (hipStream_t)stream;
@@ -707,7 +714,9 @@ int polar_rademacher(int32_t dtype, uint64_t seed, int32_t stream_id, const int6
 }
 
 int polar_edge_sketch(const polar_op* op, uint64_t seed, int64_t q, double scale, void* F, int64_t ldf, void* stream) {
-  if (!op || !F || q <= 0 || ldf < q) return set_error(POLAR_E_ARG, "bad edge_sketch args");
+  if (!op || !dtype_ok(op->dtype) || op->n <= 0 || !op->indptr || (op->weighted && !op->weights) || !F || q <= 0 ||
+      ldf < q)
+    return set_error(POLAR_E_ARG, "bad edge_sketch args");
   const uint64_t base = splitmix64(seed * 4 + 1);
   hipStream_t st = (hipStream_t)stream;
   const int grid = grid_for(op->n * q, kBlock);

@@ -36,6 +36,28 @@ int hip_fail(hipError_t e, const char* what, int line) {
   return (int)e;
 }
 
+template <typename W>
+__global__ __launch_bounds__(kBlock) void k_zero(W* __restrict__ p, int64_t words) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < words; i += stride) p[i] = W{};
+}
+
+int zero_async(void* p, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return 0;
+  if (!p || bytes % 4 != 0) return set_error(POLAR_E_ARG, "zero_async: %zu bytes at %p", bytes, p);
+  if (((uintptr_t)p & 15) == 0 && bytes % 16 == 0) {
+    const int64_t words = (int64_t)(bytes / 16);
+    const int grid = (int)std::min<int64_t>(65536, (words + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_zero<uint4>, dim3(grid), dim3(kBlock), 0, st, (uint4*)p, words);
+  } else {
+    const int64_t words = (int64_t)(bytes / 4);
+    const int grid = (int)std::min<int64_t>(65536, (words + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_zero<uint32_t>, dim3(grid), dim3(kBlock), 0, st, (uint32_t*)p, words);
+  }
+  POLAR_LAUNCH_CHECK();
+  return 0;
+}
+
 __device__ __forceinline__ int lower_bound_i32(const int32_t* __restrict__ a, int n, int key) {
   int lo = 0, hi = n;
   while (lo < hi) {
@@ -230,7 +252,8 @@ struct CGState {
   double *z, *zp, *zn, *aj, *bj, *h, *ip, *pn, *te, *tg, *shift;
   int *live, *m;
   int* flag;           // [2] any-active flags, indexed by iteration parity
-  unsigned long long* stats;  // [2] sum over iterations of active columns / live (pole, column) pairs
+  unsigned long long* stats;  // [3] sums over iterations of active columns and live (pole, column)
+                              // pairs; (pole, column) pairs changed per window (blocked update)
   double* parts;       // [2*kMaxGrid][2][R] double-double workgroup partials
   int* colmap;         // [R] slot -> caller column (active-column compaction)
   int R, p;            // R: current (compacted) width
@@ -1673,16 +1696,18 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
     s.bR = m.dtab + 2 * (S + 1);
     s.bD = m.dtab + 3 * (S + 1);
     s.bS = m.dtab + 4 * (S + 1);
-    POLAR_HIP(hipMemsetAsync(s.sflag, 0, 2 * S * sizeof(int), st));
+    if (int rc = zero_async(s.sflag, 2 * S * sizeof(int), st)) return rc;
   }
   const int64_t gpb0 = (int64_t)kWaves * pl.G;
 
-  POLAR_HIP(hipMemsetAsync(s.flag, 0, 2 * sizeof(int), st));
-  POLAR_HIP(hipMemsetAsync(s.stats, 0, 3 * sizeof(unsigned long long), st));
-  if (repl) POLAR_HIP(hipMemsetAsync(s.x0, 0, sizeof(double) * (size_t)n * R, st));
+  if (int rc = zero_async(s.flag, 2 * sizeof(int), st)) return rc;
+  if (int rc = zero_async(s.stats, 3 * sizeof(unsigned long long), st)) return rc;
+  if (repl) {
+    if (int rc = zero_async(s.x0, sizeof(double) * (size_t)n * R, st)) return rc;
+  }
   if (Tw > 1) {
     // every ring slot holds finite values before a flush reads it (with zero weight or not)
-    POLAR_HIP(hipMemsetAsync(m.ring, 0, sizeof(T) * (size_t)n * R * Tw, st));
+    if (int rc = zero_async(m.ring, sizeof(T) * (size_t)n * R * Tw, st)) return rc;
     hipLaunchKernelGGL(k_coef_reset, dim3((int)std::min<int64_t>(1024, ((int64_t)p * S * R + 255) / 256)), dim3(256), 0, st, s);
     POLAR_LAUNCH_CHECK();
   }
