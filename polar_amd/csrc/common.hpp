@@ -60,6 +60,32 @@ __device__ __forceinline__ void vstore(T* __restrict__ p, const T (&o)[CPL]) {
   *reinterpret_cast<V*>(p) = v;
 }
 
+// Non-temporal forms, for data that one kernel touches exactly once (kept out of L2).
+template <typename T, int CPL>
+__device__ __forceinline__ void vload_nt(const T* __restrict__ p, T (&o)[CPL]) {
+  if constexpr (CPL == 1) {
+    o[0] = __builtin_nontemporal_load(p);
+  } else {
+    typedef T V __attribute__((ext_vector_type(CPL)));
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+#pragma unroll
+    for (int i = 0; i < CPL; ++i) o[i] = v[i];
+  }
+}
+
+template <typename T, int CPL>
+__device__ __forceinline__ void vstore_nt(T* __restrict__ p, const T (&o)[CPL]) {
+  if constexpr (CPL == 1) {
+    __builtin_nontemporal_store(o[0], p);
+  } else {
+    typedef T V __attribute__((ext_vector_type(CPL)));
+    V v;
+#pragma unroll
+    for (int i = 0; i < CPL; ++i) v[i] = o[i];
+    __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
+  }
+}
+
 // Row-lane geometry: L lanes per row (power of two <= 64), each lane owns CPL consecutive
 // columns c0 = li*CPL; G = 64/L row-groups per wave.
 struct Geo {

@@ -266,10 +266,11 @@ __global__ __launch_bounds__(kBlock) void k_maint_prep(const T* __restrict__ X, 
 }
 
 // One pass over the old rows: w = row . coef, then row -/+= u * coef / D (X with xi, Y with eta).
-// Each row group loads its next (pole, row) item before it reduces and updates the current one, so
-// a wave keeps two rows of X and Y in flight.  FP64 rows apply u * coef / D exactly as written
-// (the pinned arithmetic); FP32 rows multiply by a reciprocal of D computed once per item (the
-// double-precision difference is far below FP32 rounding).
+// FP64 rows apply u * coef / D exactly as written (the pinned arithmetic); FP32 rows multiply by
+// u / D computed once per item (the double-precision difference is far below FP32 rounding).
+// Each row of X and Y is read once and written once per call, so both go through non-temporal
+// loads and stores (kept out of L2).  A row group loads its next item before it reduces and
+// updates the current one, so a wave keeps two rows of X and Y in flight.
 template <typename T, int CPL>
 struct RankRow {
   T xv[CPL], yv[CPL];
@@ -278,16 +279,14 @@ struct RankRow {
 
 template <typename T, int CPL>
 __device__ __forceinline__ void rank_load(const T* __restrict__ X, const T* __restrict__ Y, int64_t sj, int64_t si,
-                                          int64_t nA, int64_t q, const T* __restrict__ u, int64_t u_sj,
-                                          int64_t u_si, int64_t item, int c0, RankRow<T, CPL>& r) {
-  const int j = (int)(item / nA);
-  const int64_t i = item - (int64_t)j * nA;
+                                          int64_t q, const T* __restrict__ u, int64_t u_sj, int64_t u_si, int j,
+                                          int64_t i, int c0, RankRow<T, CPL>& r) {
   r.ui = (double)u[(int64_t)j * u_sj + i * u_si];
   const T* xr = X + (int64_t)j * sj + i * si;
   const T* yr = Y + (int64_t)j * sj + i * si;
   if (c0 + CPL <= q) {
-    vload<T, CPL>(xr + c0, r.xv);
-    vload<T, CPL>(yr + c0, r.yv);
+    vload_nt<T, CPL>(xr + c0, r.xv);
+    vload_nt<T, CPL>(yr + c0, r.yv);
   } else {
 #pragma unroll
     for (int e = 0; e < CPL; ++e) {
@@ -297,6 +296,156 @@ __device__ __forceinline__ void rank_load(const T* __restrict__ X, const T* __re
   }
 }
 
+// The lane's xi/eta entries and D of one pole, reloaded only when the pole changes.
+template <int CPL>
+struct PoleCoef {
+  int j = -1;
+  double cx[CPL], cy[CPL], D;
+  __device__ __forceinline__ void load(const double* __restrict__ coef, int64_t q, int jn, int c0) {
+    j = jn;
+    const double* cj = coef + (int64_t)jn * (2 * q + 3);
+    D = cj[2 * q];
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) {
+      cx[e] = (c0 + e < q) ? cj[c0 + e] : 0.0;
+      cy[e] = (c0 + e < q) ? cj[q + c0 + e] : 0.0;
+    }
+  }
+};
+
+// Reduces and updates the item (pole j, active row i) held in `cur`.  Every lane of the row group
+// takes part (the dots are butterflies over its L lanes); `live` is false for the lanes past the
+// last column and for a row group past the last row, which then store nothing.
+template <typename T, int CPL>
+__device__ __forceinline__ void rank_item(T* __restrict__ X, T* __restrict__ Y, int64_t sj, int64_t si, int64_t q,
+                                          int L, int c0, bool live, int j, int64_t i, const RankRow<T, CPL>& cur,
+                                          const PoleCoef<CPL>& pc, double ui, double& wx, double& wy) {
+  if constexpr (sizeof(T) == 8) {  // FP64 rows: double-double dots, rounded once (pinned)
+    DD sx{0.0, 0.0}, sy{0.0, 0.0};
+    if (live) {
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        if (c0 + e < q) {
+          dd_add(sx, (double)cur.xv[e] * pc.cx[e]);
+          dd_add(sy, (double)cur.yv[e] * pc.cy[e]);
+        }
+      }
+    }
+    sx = dd_group_sum(sx, L);
+    sy = dd_group_sum(sy, L);
+    wx = sx.hi + sx.lo;
+    wy = sy.hi + sy.lo;
+  } else {                         // FP32 rows: FP64 dots (the rows carry FP32 rounding anyway)
+    wx = 0.0;
+    wy = 0.0;
+    if (live) {
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        if (c0 + e < q) {
+          wx += (double)cur.xv[e] * pc.cx[e];
+          wy += (double)cur.yv[e] * pc.cy[e];
+        }
+      }
+    }
+    for (int o = L >> 1; o > 0; o >>= 1) {
+      wx += __shfl_xor(wx, o, 64);
+      wy += __shfl_xor(wy, o, 64);
+    }
+  }
+  if (!live) return;
+  T* xr = X + (int64_t)j * sj + i * si;
+  T* yr = Y + (int64_t)j * sj + i * si;
+  T xv[CPL], yv[CPL];
+  if constexpr (sizeof(T) == 8) {
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) {
+      xv[e] = (T)((double)cur.xv[e] - ((ui * pc.cx[e]) / pc.D));
+      yv[e] = (T)((double)cur.yv[e] + ((ui * pc.cy[e]) / pc.D));
+    }
+  } else {
+    const double f = ui / pc.D;
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) {
+      xv[e] = (T)((double)cur.xv[e] - f * pc.cx[e]);
+      yv[e] = (T)((double)cur.yv[e] + f * pc.cy[e]);
+    }
+  }
+  if (c0 + CPL <= q) {
+    vstore_nt<T, CPL>(xr + c0, xv);
+    vstore_nt<T, CPL>(yr + c0, yv);
+  } else {
+#pragma unroll
+    for (int e = 0; e < CPL; ++e)
+      if (c0 + e < q) {
+        xr[c0 + e] = xv[e];
+        yr[c0 + e] = yv[e];
+      }
+  }
+}
+
+// Row tiles: a workgroup owns TR = S * (rows per pass) consecutive active rows and walks them pole
+// by pole (each pole's TR rows of X and Y are one contiguous run).  w and u of the tile are parked
+// in LDS as [row][pole] and written out as one contiguous vertex-major run per array, instead of
+// one 8-byte store per (pole, row) p doubles apart.
+template <typename T, int CPL>
+__global__ __launch_bounds__(kBlock) void k_maint_rank1_tiles(T* __restrict__ X, T* __restrict__ Y, int64_t sj,
+                                                              int64_t si, int p, int64_t nA, int64_t q,
+                                                              const T* __restrict__ u, int64_t u_sj, int64_t u_si,
+                                                              const double* __restrict__ coef, double* __restrict__ wX,
+                                                              double* __restrict__ wY, double* __restrict__ ut, int L,
+                                                              int S) {
+  extern __shared__ double s_w[];   // [3][TR * p]: w . xi, w . eta, u of (row, pole) at row * p + pole
+  Geo g(L);
+  const int c0 = g.li * CPL;
+  const bool ok = c0 < q;
+  const int gpb = kWaves * g.G;
+  const int TR = S * gpb;
+  const int slot = g.wave * g.G + g.gi;
+  const int nk = p * S;             // items of a tile: (pole, pass), pole-major
+  const int64_t ntiles = (nA + TR - 1) / TR;
+  double* s_x = s_w;
+  double* s_y = s_w + (int64_t)TR * p;
+  double* s_u = s_w + 2 * (int64_t)TR * p;
+  PoleCoef<CPL> pc;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t i0 = tile * TR;
+    RankRow<T, CPL> cur;
+    if (ok && i0 + slot < nA) rank_load<T, CPL>(X, Y, sj, si, q, u, u_sj, u_si, 0, i0 + slot, c0, cur);
+    for (int k = 0; k < nk; ++k) {
+      const int j = k / S;
+      const int r = (k - j * S) * gpb + slot;
+      const int64_t i = i0 + r;
+      const bool valid = i < nA;
+      RankRow<T, CPL> nxt;
+      if (k + 1 < nk) {
+        const int jn = (k + 1) / S;
+        const int64_t in = i0 + ((k + 1) - jn * S) * gpb + slot;
+        if (ok && in < nA) rank_load<T, CPL>(X, Y, sj, si, q, u, u_sj, u_si, jn, in, c0, nxt);
+      }
+      if (j != pc.j) pc.load(coef, q, j, c0);
+      const double ui = (ok && valid) ? cur.ui : 0.0;
+      double wx, wy;
+      rank_item<T, CPL>(X, Y, sj, si, q, L, c0, ok && valid, j, i, cur, pc, ui, wx, wy);
+      if (g.li == 0 && valid) {
+        s_x[r * p + j] = wx;
+        s_y[r * p + j] = wy;
+        s_u[r * p + j] = ui;
+      }
+      cur = nxt;
+    }
+    __syncthreads();
+    const int64_t cnt = min((int64_t)TR, nA - i0) * p;
+    for (int64_t t = threadIdx.x; t < cnt; t += kBlock) {
+      wX[i0 * p + t] = s_x[t];
+      wY[i0 * p + t] = s_y[t];
+      ut[i0 * p + t] = s_u[t];
+    }
+    __syncthreads();
+  }
+}
+
+// Pole-major walk over all (pole, row) items, for pole counts whose row tile does not fit in LDS:
+// w and u go straight to their vertex-major slots.
 template <typename T, int CPL>
 __global__ __launch_bounds__(kBlock) void k_maint_rank1(T* __restrict__ X, T* __restrict__ Y, int64_t sj, int64_t si,
                                                         int p, int64_t nA, int64_t q, const T* __restrict__ u,
@@ -307,88 +456,32 @@ __global__ __launch_bounds__(kBlock) void k_maint_rank1(T* __restrict__ X, T* __
   const int c0 = g.li * CPL;
   const int64_t gpb = (int64_t)kWaves * g.G;
   const int64_t total = (int64_t)p * nA;
-  const int64_t stride = 2 * q + 3;
   const int64_t step = (int64_t)gridDim.x * gpb;
   const bool ok = c0 < q;
   int64_t item = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi;
   RankRow<T, CPL> cur;
-  if (item < total && ok) rank_load<T, CPL>(X, Y, sj, si, nA, q, u, u_sj, u_si, item, c0, cur);
+  if (item < total && ok) {
+    const int j = (int)(item / nA);
+    rank_load<T, CPL>(X, Y, sj, si, q, u, u_sj, u_si, j, item - (int64_t)j * nA, c0, cur);
+  }
+  PoleCoef<CPL> pc;
   for (; item < total; item += step) {
     RankRow<T, CPL> nxt;
     const int64_t nitem = item + step;
-    if (nitem < total && ok) rank_load<T, CPL>(X, Y, sj, si, nA, q, u, u_sj, u_si, nitem, c0, nxt);
+    if (nitem < total && ok) {
+      const int jn = (int)(nitem / nA);
+      rank_load<T, CPL>(X, Y, sj, si, q, u, u_sj, u_si, jn, nitem - (int64_t)jn * nA, c0, nxt);
+    }
     const int j = (int)(item / nA);
     const int64_t i = item - (int64_t)j * nA;
-    const double* cj = coef + (int64_t)j * stride;
-    const double D = cj[2 * q];
-    const double ui = ok ? cur.ui : (double)u[(int64_t)j * u_sj + i * u_si];
+    if (j != pc.j) pc.load(coef, q, j, c0);
+    const double ui = ok ? cur.ui : 0.0;
     double wx, wy;
-    if constexpr (sizeof(T) == 8) {  // FP64 rows: double-double dots, rounded once (pinned)
-      DD sx{0.0, 0.0}, sy{0.0, 0.0};
-      if (ok) {
-#pragma unroll
-        for (int e = 0; e < CPL; ++e) {
-          if (c0 + e < q) {
-            dd_add(sx, (double)cur.xv[e] * cj[c0 + e]);
-            dd_add(sy, (double)cur.yv[e] * cj[q + c0 + e]);
-          }
-        }
-      }
-      sx = dd_group_sum(sx, L);
-      sy = dd_group_sum(sy, L);
-      wx = sx.hi + sx.lo;
-      wy = sy.hi + sy.lo;
-    } else {                         // FP32 rows: FP64 dots (the rows carry FP32 rounding anyway)
-      wx = 0.0;
-      wy = 0.0;
-      if (ok) {
-#pragma unroll
-        for (int e = 0; e < CPL; ++e) {
-          if (c0 + e < q) {
-            wx += (double)cur.xv[e] * cj[c0 + e];
-            wy += (double)cur.yv[e] * cj[q + c0 + e];
-          }
-        }
-      }
-      for (int o = L >> 1; o > 0; o >>= 1) {
-        wx += __shfl_xor(wx, o, 64);
-        wy += __shfl_xor(wy, o, 64);
-      }
-    }
+    rank_item<T, CPL>(X, Y, sj, si, q, L, c0, ok, j, i, cur, pc, ui, wx, wy);
     if (g.li == 0) {                 // vertex-major: all poles of active row i adjacent
       wX[i * p + j] = wx;
       wY[i * p + j] = wy;
       ut[i * p + j] = ui;
-    }
-    if (ok) {
-      T* xr = X + (int64_t)j * sj + i * si;
-      T* yr = Y + (int64_t)j * sj + i * si;
-      T xv[CPL], yv[CPL];
-      if constexpr (sizeof(T) == 8) {
-#pragma unroll
-        for (int e = 0; e < CPL; ++e) {
-          xv[e] = (T)((double)cur.xv[e] - ((ui * cj[c0 + e]) / D));
-          yv[e] = (T)((double)cur.yv[e] + ((ui * cj[q + c0 + e]) / D));
-        }
-      } else {
-        const double f = ui / D;
-#pragma unroll
-        for (int e = 0; e < CPL; ++e) {
-          xv[e] = (T)((double)cur.xv[e] - f * cj[c0 + e]);
-          yv[e] = (T)((double)cur.yv[e] + f * cj[q + c0 + e]);
-        }
-      }
-      if (c0 + CPL <= q) {
-        vstore<T, CPL>(xr + c0, xv);
-        vstore<T, CPL>(yr + c0, yv);
-      } else {
-#pragma unroll
-        for (int e = 0; e < CPL; ++e)
-          if (c0 + e < q) {
-            xr[c0 + e] = xv[e];
-            yr[c0 + e] = yv[e];
-          }
-      }
     }
     cur = nxt;
   }
@@ -507,6 +600,26 @@ static int pick_cpl_rows(int dtype, int64_t q, const void* base, int64_t sj, int
       return cpl;
   }
   return 1;
+}
+
+constexpr int kRankLds = 48 * 1024;   // LDS bytes of one k_maint_rank1_tiles row tile
+
+// Row tiles of S passes (S <= 4) when one pass of rows fits the LDS budget, else the pole-major walk.
+template <typename T, int CPL>
+static void launch_rank1(T* X, T* Y, int64_t sj, int64_t si, int p, int64_t nA, int64_t q, const T* u,
+                         int64_t u_sj, int64_t u_si, const double* coef, double* wX, double* wY, double* ut, int L,
+                         hipStream_t st) {
+  const int gpb = kWaves * (64 / L);
+  const int64_t S = std::min<int64_t>(4, kRankLds / (24 * (int64_t)gpb * p));
+  if (S >= 1) {
+    const int64_t TR = S * gpb;
+    const int grid = grid_for((nA + TR - 1) / TR, 1);
+    hipLaunchKernelGGL((k_maint_rank1_tiles<T, CPL>), dim3(grid), dim3(kBlock), (size_t)(24 * TR * p), st, X, Y, sj,
+                       si, p, nA, q, u, u_sj, u_si, coef, wX, wY, ut, L, (int)S);
+  } else {
+    hipLaunchKernelGGL((k_maint_rank1<T, CPL>), dim3(grid_for((int64_t)p * nA, gpb)), dim3(kBlock), 0, st, X, Y, sj,
+                       si, p, nA, q, u, u_sj, u_si, coef, wX, wY, ut, L);
+  }
 }
 
 }  // namespace polar
@@ -661,15 +774,13 @@ int polar_maint_rank1(int32_t dtype, void* X, void* Y, int64_t sj, int64_t si, i
   int cpl = std::min(pick_cpl_rows(dtype, q, X, sj, si), pick_cpl_rows(dtype, q, Y, sj, si));
   int L = lanes_for(q, cpl);
   if (L > 64) return set_error(POLAR_E_ARG, "q=%lld too wide for maint_rank1 at this alignment", (long long)q);
-  const int gpb = kWaves * (64 / L);
-  const int grid = grid_for((int64_t)p * nA, gpb);
   if (dtype == POLAR_F64) {
-    if (cpl == 2) hipLaunchKernelGGL((k_maint_rank1<double, 2>), dim3(grid), dim3(kBlock), 0, st, (double*)X, (double*)Y, sj, si, p, nA, q, (const double*)u, u_sj, u_si, coef, wX, wY, ut, L);
-    else hipLaunchKernelGGL((k_maint_rank1<double, 1>), dim3(grid), dim3(kBlock), 0, st, (double*)X, (double*)Y, sj, si, p, nA, q, (const double*)u, u_sj, u_si, coef, wX, wY, ut, L);
+    if (cpl == 2) launch_rank1<double, 2>((double*)X, (double*)Y, sj, si, p, nA, q, (const double*)u, u_sj, u_si, coef, wX, wY, ut, L, st);
+    else launch_rank1<double, 1>((double*)X, (double*)Y, sj, si, p, nA, q, (const double*)u, u_sj, u_si, coef, wX, wY, ut, L, st);
   } else {
-    if (cpl == 4) hipLaunchKernelGGL((k_maint_rank1<float, 4>), dim3(grid), dim3(kBlock), 0, st, (float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, ut, L);
-    else if (cpl == 2) hipLaunchKernelGGL((k_maint_rank1<float, 2>), dim3(grid), dim3(kBlock), 0, st, (float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, ut, L);
-    else hipLaunchKernelGGL((k_maint_rank1<float, 1>), dim3(grid), dim3(kBlock), 0, st, (float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, ut, L);
+    if (cpl == 4) launch_rank1<float, 4>((float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, ut, L, st);
+    else if (cpl == 2) launch_rank1<float, 2>((float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, ut, L, st);
+    else launch_rank1<float, 1>((float*)X, (float*)Y, sj, si, p, nA, q, (const float*)u, u_sj, u_si, coef, wX, wY, ut, L, st);
   }
   POLAR_LAUNCH_CHECK();
   return 0;
