@@ -851,9 +851,9 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
       T* pp = P + ((int64_t)j * nA + i) * s.ld + c0;
       T* xrow = Xo + (int64_t)j * sj + i * si;
       T pv[CPL], xv[CPL];
-      vload<T, CPL>(pp, pv);
+      vload_nt<T, CPL>(pp, pv);                   // P and X rows: touched once per iteration
       if (s.ident) {
-        vload<T, CPL>(xrow + c0, xv);
+        vload_nt<T, CPL>(xrow + c0, xv);
       } else {
 #pragma unroll
         for (int e = 0; e < CPL; ++e) xv[e] = mj[e] ? xrow[s.colmap[c0 + e]] : T(0);
@@ -866,13 +866,13 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
         }
       }
       if (s.ident) {
-        vstore<T, CPL>(xrow + c0, xv);
+        vstore_nt<T, CPL>(xrow + c0, xv);
       } else {
 #pragma unroll
         for (int e = 0; e < CPL; ++e)
           if (mj[e]) xrow[s.colmap[c0 + e]] = xv[e];
       }
-      vstore<T, CPL>(pp, pv);
+      vstore_nt<T, CPL>(pp, pv);
     }
   }
 }
@@ -963,9 +963,9 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
         T* pp = P + ((int64_t)j * nA + row[k]) * s.ld + c0;
         T* xrow = Xo + (int64_t)j * sj + row[k] * si;
         T pv[CPL], xv[CPL];
-        vload<T, CPL>(pp, pv);
+        vload_nt<T, CPL>(pp, pv);                 // P and X rows: touched once per window
         if (s.ident) {
-          vload<T, CPL>(xrow + c0, xv);
+          vload_nt<T, CPL>(xrow + c0, xv);
         } else {
 #pragma unroll
           for (int e = 0; e < CPL; ++e) xv[e] = xrow[s.colmap[c0 + e]];
@@ -983,12 +983,12 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
           pv[e] = ap;
         }
         if (s.ident) {
-          vstore<T, CPL>(xrow + c0, xv);
+          vstore_nt<T, CPL>(xrow + c0, xv);
         } else {
 #pragma unroll
           for (int e = 0; e < CPL; ++e) xrow[s.colmap[c0 + e]] = xv[e];
         }
-        vstore<T, CPL>(pp, pv);
+        vstore_nt<T, CPL>(pp, pv);
       }
       if (jj + 1 < nj) stage(s_jl[jj + 1], b ^ 1);   // b ^ 1 was last read before the previous barrier
       __syncthreads();
