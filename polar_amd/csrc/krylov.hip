@@ -1148,9 +1148,9 @@ __global__ __launch_bounds__(kBlock) void k_init_shift(int64_t nA, int p, int R,
     vload<T, CPL>(W + (int64_t)active[i] * ldw + c0, wv);
 #pragma unroll
     for (int e = 0; e < CPL; ++e) zero[e] = T(0);
-    for (int j = 0; j < p; ++j) {
-      vstore<T, CPL>(P + ((int64_t)j * nA + i) * R + c0, wv);
-      vstore<T, CPL>(Xo + (int64_t)j * sj + i * si + c0, zero);
+    for (int j = 0; j < p; ++j) {             // streaming writes: the blocks are far larger than L2
+      vstore_nt<T, CPL>(P + ((int64_t)j * nA + i) * R + c0, wv);
+      vstore_nt<T, CPL>(Xo + (int64_t)j * sj + i * si + c0, zero);
     }
   }
 }
