@@ -85,7 +85,8 @@ typedef struct polar_op {
   const int32_t* patch_rows;  /* [n_patch] ascending */
   const int32_t* patch_ptr;   /* [n_patch+1] */
   const int32_t* patch_cols;  /* [patch_ptr[n_patch]] */
-  int32_t grid_max;           /* tuning: block budget of the kernels with partial sums (0 = default) */
+  int32_t grid_max;           /* tuning: block budget of the kernels with partial sums (0 = default:
+                                 1024 blocks, 16384 for the operator kernel on >= 2^20 rows) */
   int32_t pad2;
   const int32_t* patch_grp;   /* [patch_ptr[n_patch]] column group of each patch entry, or NULL
                                  (every entry applies to every column) */
