@@ -904,8 +904,9 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
   const int vb = sp.g * s.ld;
   __shared__ int s_jl[kMaxPolesFlush];
   __shared__ int s_nj;
-  // two coefficient buffers: pole jj reads buffer jj & 1, pole jj + 1 is staged into the other
-  constexpr int kMaxR = 64 * (16 / (int)sizeof(T));   // widest solve: 64 lanes of 16-byte vectors
+  // two coefficient buffers: pole jj reads buffer jj & 1, pole jj + 1 is staged into the other;
+  // a slice is at most kFlushLanes lanes of 16-byte vectors (so the LDS allows more blocks per CU)
+  constexpr int kMaxR = kFlushLanes * (16 / (int)sizeof(T));
   __shared__ T s_cx[2][(kW + 1) * kMaxR], s_cp[2][(kW + 1) * kMaxR];
   __shared__ int s_dirty[2][kMaxR];
   if (threadIdx.x == 0) s_nj = 0;
@@ -1562,6 +1563,7 @@ static int flush_window(int cpl, const Plan& pl, const Ring<T>& ring, const int3
     // each staged pole serves 4x more rows than a full-width tile at 32 lanes
     const int Lf = pl.Lf;
     const int cw = Lf * cpl;
+    if (Lf > kFlushLanes || cpl * (int)sizeof(T) > 16) return set_error(POLAR_E_ARG, "flush slice too wide");
     const dim3 grid(pl.nb_s, (s.R + cw - 1) / cw);
     if (cpl == 2) {
       hipLaunchKernelGGL((k_flush<T, 2>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s);
