@@ -113,13 +113,15 @@ const char* polar_last_error(void);
 
 /* Live profiling of the CG iteration kernels with HIP events (no counterpart in the reference;
  * used by the benchmark).  polar_prof_set(1) resets and enables it for every class;
- * polar_prof_set(2) times class 0 only (two events per iteration instead of seven, so the
- * timed work is barely perturbed); 0 disables it.  Classes 0..5: k_opmul
- * (operator product + d.v partials), k_longfin (long-row chunk combine), k_alpha, k_resid
- * (residual update + |r|^2 partials), k_scalars (per-pole recurrences), k_update (direction and
- * shifted x/p updates); per class the summed event time (ms), launches that did work, and
- * algorithmic HBM bytes.  Class 6: solves, and (bytes) the number of live (pole, column) pairs
- * summed over iterations. */
+ * polar_prof_set(2) times class 0 only (two events per iteration instead of seven);
+ * polar_prof_set(3) times class 0 on one iteration in every 8 (the position rotates from one
+ * 8-iteration batch to the next), so the timed work is barely perturbed; 0 disables it.
+ * Classes 0..5: k_opmul (operator product + d.v partials), k_longfin (long-row chunk combine),
+ * k_alpha, k_resid (residual update + |r|^2 partials), k_scalars (per-pole recurrences),
+ * k_update (direction and shifted x/p updates); per class the summed event time (ms), the timed
+ * launches that did work, and their algorithmic HBM bytes.  Class 6: solves, and (bytes) the
+ * number of live (pole, column) pairs summed over iterations.  Class 7 (launches): every
+ * iteration that did work, timed or not.  Any other mode is rejected (POLAR_E_ARG). */
 int polar_prof_set(int32_t on);
 int polar_prof_get(int32_t cls, double* ms, int64_t* launches, double* bytes);
 

@@ -1166,15 +1166,17 @@ __global__ void k_shift(const double* __restrict__ tau, int p, int S, double* __
 // Host side
 // Live kernel profiling (HIP events around each launch of the CG iteration), off by default.
 // Classes 0..5: k_opmul, k_longfin, k_alpha, k_resid, k_scalars, k_update; class 6 counts solves
-// and, in `bytes`, the sum over iterations of live (pole, column) pairs.
+// and, in `bytes`, the sum over iterations of live (pole, column) pairs; class 7 counts every
+// iteration that did work (in mode 3 classes 0..5 cover only the timed sample of them).
 struct ProfClass {
   double ms = 0.0;
   int64_t launches = 0;
   double bytes = 0.0;
 };
 constexpr int kEv = 7;        // events per iteration (6 kernels)
-static ProfClass g_prof[kEv];
-static int g_prof_on = 0;
+constexpr int kProfClasses = kEv + 1;
+static ProfClass g_prof[kProfClasses];
+static int g_prof_on = 0;     // 0 off; 1 every class; 2 k_opmul, every iteration; 3 k_opmul, 1 in 8
 
 struct Plan {
   int cpl, L, G;
@@ -1767,6 +1769,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   std::vector<hipEvent_t> ev;
   std::vector<float> kms;        // per iteration: per-kernel milliseconds
   std::vector<int> widths;       // per iteration: processed width
+  std::vector<char> timed;       // per iteration: its events were recorded (mode 3 samples 1 in 8)
   const int esz = (int)sizeof(T);
   const int align = cpl0;        // compacted widths stay multiples of the initial vector width
   std::vector<int> permh(R);
@@ -1817,8 +1820,11 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
             POLAR_HIP(hipEventCreate(&e));
             ev.push_back(e);
           }
-          evp = ev.data() + slot;
+          // mode 3: one iteration per batch of 8, at a position that rotates from batch to batch
+          const bool take = g_prof_on != 3 || k == (int)((it0 / 8) % 8);
+          evp = take ? ev.data() + slot : nullptr;
           widths.push_back(s.R);
+          timed.push_back(take ? 1 : 0);
         }
         int rc = run_iter_any<T>(cpl, op, pl, tau0, slot[k % kW], slot[(k + 1) % kW], d, v, lpart, P, Xo, sj, si,
                                  active, nA, p, s, (int)(it & 1), k, evp, st);
@@ -1839,10 +1845,12 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
     POLAR_HIP(hipStreamSynchronize(st));
     const int flag = stat[(it0 + nit) & 1];
     if (prof) {
+      const size_t t0 = timed.size() - (size_t)nit;
       for (int k = 0; k < nit; ++k)
         for (int c = 0; c < kEv - 1; ++c) {
           float ms = 0.f;
-          if (c == 0 || g_prof_on == 1) POLAR_HIP(hipEventElapsedTime(&ms, ev[k * kEv + c], ev[k * kEv + c + 1]));
+          if (timed[t0 + k] && (c == 0 || g_prof_on == 1))
+            POLAR_HIP(hipEventElapsedTime(&ms, ev[k * kEv + c], ev[k * kEv + c + 1]));
           kms.push_back(ms);
         }
       if (Tw > 1 && g_prof_on == 1) {
@@ -1944,7 +1952,10 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
     real = std::min<int64_t>(real, (int64_t)kms.size() / (kEv - 1));
     int32_t nnz_h = 0;
     POLAR_HIP(hipMemcpy(&nnz_h, op->indptr + op->n, sizeof(int32_t), hipMemcpyDeviceToHost));
+    int64_t sampled = 0;
     for (int64_t k = 0; k < real; ++k) {
+      if (!timed[k]) continue;
+      ++sampled;
       for (int c = 0; c < kEv - 1; ++c) g_prof[c].ms += kms[(kEv - 1) * k + c];
       const double w = (double)widths[k] * S;
       g_prof[0].bytes += 4.0 * (n + 1) + 4.0 * nnz_h + (op->weighted ? (double)esz * nnz_h : 0.0) +
@@ -1953,7 +1964,8 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
       g_prof[2].bytes += 16.0 * widths[k] * (pl.grid_a + pl.grid_l);
       g_prof[4].bytes += 16.0 * widths[k] * pl.grid_rows + 120.0 * w * p;
     }
-    for (int c = 0; c < kEv - 1; ++c) g_prof[c].launches += real;
+    for (int c = 0; c < kEv - 1; ++c) g_prof[c].launches += sampled;
+    g_prof[kEv].launches += real;
     // active columns summed over iterations; with segments a column-iteration covers its
     // segment's rows only, so the row streams are counted at the mean segment length
     const double nrows = (double)n / S, narows = (double)nA / S;
@@ -2043,13 +2055,14 @@ extern "C" {
 int polar_abi_version(void) { return POLAR_ABI_VERSION; }
 
 int polar_prof_set(int32_t on) {
+  if (on < 0 || on > 3) return set_error(POLAR_E_ARG, "profiler mode %d (0..3)", on);
   g_prof_on = on;
   for (auto& c : g_prof) c = ProfClass();
   return 0;
 }
 
 int polar_prof_get(int32_t cls, double* ms, int64_t* launches, double* bytes) {
-  if (cls < 0 || cls >= kEv) return set_error(POLAR_E_ARG, "bad profile class");
+  if (cls < 0 || cls >= kProfClasses) return set_error(POLAR_E_ARG, "bad profile class");
   if (ms) *ms = g_prof[cls].ms;
   if (launches) *launches = g_prof[cls].launches;
   if (bytes) *bytes = g_prof[cls].bytes;
