@@ -86,3 +86,24 @@ def test_polar_wide_bank_split_into_block_solves():
     assert rg['sequence'] == ro['sequence'] and (rg['spmv'], rg['rhs']) == (ro['spmv'], ro['rhs'])
     assert _rel(rg['marginals'], ro['marginals']) <= 1e-12
     assert _rel(rg['trace_estimate'], ro['trace_estimate']) <= 1e-12
+
+
+@pytest.mark.parametrize('q,p', [(5, 3), (7, 40), (7, 70)])
+def test_polar_odd_sketch_width_parity(q, p):
+    """Odd q (the rank-1 pass, the pair gathers and the bank run their scalar tail columns); the
+    rank-1 pass runs row tiles of 4 passes (p = 3), of one pass (p = 40) and, at p = 70, whose
+    tile does not fit its LDS budget, the pole-major walk: same sequence, counters, marginals and
+    traces as the oracle."""
+    n = 300
+    e = graphs.barabasi_albert(n, 3, 7)
+    sigma, s = 0.2, 0.5
+    cand = graphs.candidates(n, e, 120, 5)
+    Go, G = O.Graph(n, e, sigma), pg.Graph(n, e, sigma)
+    tau, om, _ = pg.fixed_rule(s, p, sigma, G.Lam(5))
+    sk = O.sketches(Go, cand, q, 9, kind='rademacher')
+    kw = dict(q=q, B=3, tol=1e-9, sketch=sk, poles=(tau, om))
+    ro = O.polar(Go, cand, 5, s, arith='cr', **kw)
+    rg = pg.polar(G, cand, 5, s, **kw)
+    assert rg['sequence'] == ro['sequence'] and (rg['spmv'], rg['rhs']) == (ro['spmv'], ro['rhs'])
+    assert _rel(rg['marginals'], ro['marginals']) <= 1e-12
+    assert _rel(rg['trace_estimate'], ro['trace_estimate']) <= 1e-12
