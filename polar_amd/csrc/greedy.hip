@@ -493,7 +493,7 @@ constexpr int kMaintLds = 48 * 1024;   // LDS bytes of one k_maint_cache tile
 // every pole.  Phase 1 walks (pair, pole) pole-fastest, so the pair's endpoint runs of ut, wX, wY
 // (vertex-major, p doubles per vertex) are read contiguously, and parks cc, dwx, dwy in LDS.
 // Phase 2 walks (pole, pair) pair-fastest, so h[j*N + e] and t[j*N + e] are read and written in
-// contiguous runs of P.  Same arithmetic per (pole, pair) as the unfused formula.
+// contiguous runs of P (non-temporally: each entry is touched once per call).  Same arithmetic per (pole, pair) as the unfused formula.
 __global__ __launch_bounds__(kBlock) void k_maint_cache(int p, int64_t N, int P, const int32_t* __restrict__ ea,
                                                         const int32_t* __restrict__ eb, const double* __restrict__ coef,
                                                         int64_t q, const double* __restrict__ wX,
@@ -526,10 +526,10 @@ __global__ __launch_bounds__(kBlock) void k_maint_cache(int p, int64_t N, int P,
         const double nxi = cj[2 * q + 1], neta = cj[2 * q + 2];
         const double cc = s_cc[j * ld + k], dwx = s_dx[j * ld + k], dwy = s_dy[j * ld + k];
         const int64_t item = (int64_t)j * N + e0 + k;
-        const double hn = (h[item] - ((2.0 * cc) * dwx)) + ((cc * cc) * nxi);
-        const double tn = (t[item] + ((2.0 * cc) * dwy)) + ((cc * cc) * neta);
-        h[item] = hn > 0.0 ? hn : 0.0;
-        t[item] = tn > 0.0 ? tn : 0.0;
+        const double hn = (__builtin_nontemporal_load(h + item) - ((2.0 * cc) * dwx)) + ((cc * cc) * nxi);
+        const double tn = (__builtin_nontemporal_load(t + item) + ((2.0 * cc) * dwy)) + ((cc * cc) * neta);
+        __builtin_nontemporal_store(hn > 0.0 ? hn : 0.0, h + item);
+        __builtin_nontemporal_store(tn > 0.0 ? tn : 0.0, t + item);
       }
     }
     __syncthreads();
