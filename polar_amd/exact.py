@@ -1,8 +1,9 @@
 """Dense prefix-gain evaluation for the `effect` mode (realworld/exact.py:8-17, 65-82).
 
 f(S_r) for every prefix r from the eigenvalues of the dense matrix M_S = M0 + sum_{e in S} b b^T,
-computed with torch.linalg.eigvalsh in FP64 on the GPU (rocSOLVER) when one is present, else on
-the host.  Used when n <= dense_max, as run_real.evaluate does (run_real.py:52-56, 91-97).
+computed with torch.linalg.eigvalsh in FP64 on the GPU (rocSOLVER); like the rest of the product
+path it has no CPU fallback.  Used when n <= dense_max, as run_real.evaluate does
+(run_real.py:52-56, 91-97).
 """
 import numpy as np
 import torch
@@ -26,8 +27,10 @@ def frac_value(lam, s, obj):
 
 
 def _dense(M0):
-    dev = 'cuda' if torch.cuda.is_available() else 'cpu'
-    return torch.as_tensor(np.asarray(M0, dtype=np.float64), device=dev).clone()
+    if not torch.cuda.is_available():
+        raise RuntimeError('the dense evaluator needs a ROCm GPU (torch.cuda.is_available() is False); '
+                           'there is no CPU fallback')
+    return torch.as_tensor(np.asarray(M0, dtype=np.float64), device='cuda').clone()
 
 
 def eval_prefix(M0, cand, seq, s, obj='frac'):
