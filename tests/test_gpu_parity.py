@@ -293,3 +293,25 @@ def test_polar_many_poles_fp64_parity():
     assert rg['spmv'] == ro['spmv'] and rg['p'] == 150
     assert _rel(rg['marginals'], ro['marginals']) < 1e-13
     assert _rel(rg['trace_estimate'], ro['trace_estimate']) < 1e-13
+
+
+@pytest.mark.parametrize('obj', ['er', 'logdet'])
+def test_polar_er_logdet_parity(obj):
+    """The single-pole objectives through the HIP path (proxy obj code, log1p re-rank for logdet,
+    no trace tracker for logdet): the greedy and the certified marginals equal the oracle's."""
+    n = 400
+    e = graphs.barabasi_albert(n, 3, 13)
+    cand = graphs.candidates(n, e, 300, 8)
+    Go, G = O.Graph(n, e, 0.3), pg.Graph(n, e, 0.3)
+    sk = O.sketches(Go, cand, 16, 4, kind='rademacher')
+    kw = dict(q=16, B=4, tol=1e-9, obj=obj, sketch=sk)
+    ro = O.polar(Go, cand, 5, 0.5, arith='cr', **kw)
+    rg = pg.polar(G, cand, 5, 0.5, **kw)
+    assert rg['sequence'] == ro['sequence']
+    assert (rg['spmv'], rg['rhs'], rg['p']) == (ro['spmv'], ro['rhs'], 1)
+    assert _rel(rg['marginals'], ro['marginals']) < 1e-13
+    if obj == 'er':
+        assert _rel(rg['trace_estimate'], ro['trace_estimate']) < 1e-13
+    g_o, c_o = O.true_marginals(Go, cand, ro['sequence'], 0.5, obj=obj, arith='cr')
+    g_g, c_g = pg.true_marginals(G, cand, ro['sequence'], 0.5, obj=obj)
+    assert _rel(g_g, g_o) < 1e-12 and c_g['spmv'] == c_o['spmv'] and c_g['p'] == 1

@@ -364,3 +364,22 @@ def test_slq_greedy_estimates_are_trace_differences():
         want.append(base - fq(Mb))
     assert np.allclose(out['estimates'][0], want, rtol=1e-8, atol=1e-12)
     assert out['sequence'][0] == int(np.argmax(want))
+
+
+@pytest.mark.parametrize('obj', ['er', 'logdet'])
+def test_er_logdet_marginals_match_eigenvalues(obj):
+    """The single-pole objectives (polar_gpu.py:39-43: one pole at 0, weight 1): certified marginals
+    and the B > 1 greedy's re-rank marginals equal the dense prefix gains, sum 1/lambda (er) and
+    log det (logdet, the matrix determinant lemma log(1 + b^T M^-1 b))."""
+    n = 150
+    e = graphs.barabasi_albert(n, 3, 21)
+    G = O.Graph(n, e, 0.3)
+    cand = graphs.candidates(n, e, 80, 6)
+    seq = O.random_set(cand, 4, 3)
+    g, cert = O.true_marginals(G, cand, seq, 0.5, obj=obj)
+    exact = np.diff(np.r_[0.0, O.eval_prefix(G.M0.toarray(), cand, seq, 0.5, obj=obj)])
+    assert cert['p'] == 1
+    assert np.max(np.abs(g - exact) / np.abs(exact)) < 1e-8
+    r = O.polar(G, cand, 4, 0.5, q=16, B=4, tol=1e-10, obj=obj, kind='rademacher', seed=5)
+    ex = np.diff(np.r_[0.0, O.eval_prefix(G.M0.toarray(), cand, r['sequence'], 0.5, obj=obj)])
+    assert np.max(np.abs(np.array(r['marginals']) - ex) / np.abs(ex)) < 1e-8
