@@ -315,3 +315,36 @@ def test_polar_er_logdet_parity(obj):
     g_o, c_o = O.true_marginals(Go, cand, ro['sequence'], 0.5, obj=obj, arith='cr')
     g_g, c_g = pg.true_marginals(G, cand, ro['sequence'], 0.5, obj=obj)
     assert _rel(g_g, g_o) < 1e-12 and c_g['spmv'] == c_o['spmv'] and c_g['p'] == 1
+
+
+@pytest.mark.parametrize('kind', ['block', 'gauss'])
+def test_polar_reference_sketch_kinds_parity(kind):
+    """polar() generating its own sketches of the reference's default kinds (block: stacked
+    orthogonal rows; gauss) from the seed: same sequence, counters and marginals as the oracle."""
+    n = 500
+    e = graphs.barabasi_albert(n, 3, 17)
+    cand = graphs.candidates(n, e, 200, 9)
+    Go, G = O.Graph(n, e, 0.2), pg.Graph(n, e, 0.2)
+    tau, om, _ = pg.fixed_rule(0.5, 8, 0.2, G.Lam(4))
+    kw = dict(q=16, B=3, tol=1e-9, seed=31, kind=kind, poles=(tau, om))
+    ro = O.polar(Go, cand, 4, 0.5, arith='cr', **kw)
+    rg = pg.polar(G, cand, 4, 0.5, **kw)
+    assert rg['sequence'] == ro['sequence'] and (rg['spmv'], rg['rhs']) == (ro['spmv'], ro['rhs'])
+    assert _rel(rg['marginals'], ro['marginals']) < 1e-13
+    assert _rel(rg['trace_estimate'], ro['trace_estimate']) < 1e-13
+
+
+def test_polar_time_limit_truncates_after_first_round():
+    """time_limit (polar_gpu.py:215): the limit is checked before every round after the first, so
+    a zero limit returns one round, truncated, with the oracle's first pick and counters."""
+    n = 300
+    e = graphs.barabasi_albert(n, 3, 5)
+    cand = graphs.candidates(n, e, 100, 3)
+    Go, G = O.Graph(n, e, 0.2), pg.Graph(n, e, 0.2)
+    tau, om, _ = pg.fixed_rule(0.5, 6, 0.2, G.Lam(4))
+    sk = O.sketches(Go, cand, 8, 2, kind='rademacher')
+    kw = dict(q=8, B=2, tol=1e-9, sketch=sk, poles=(tau, om), time_limit=0.0)
+    ro = O.polar(Go, cand, 4, 0.5, arith='cr', **kw)
+    rg = pg.polar(G, cand, 4, 0.5, **kw)
+    assert rg['rounds'] == ro['rounds'] == 1 and rg['truncated'] and ro['truncated']
+    assert rg['sequence'] == ro['sequence'] and (rg['spmv'], rg['rhs']) == (ro['spmv'], ro['rhs'])
