@@ -348,3 +348,19 @@ def test_polar_time_limit_truncates_after_first_round():
     rg = pg.polar(G, cand, 4, 0.5, **kw)
     assert rg['rounds'] == ro['rounds'] == 1 and rg['truncated'] and ro['truncated']
     assert rg['sequence'] == ro['sequence'] and (rg['spmv'], rg['rhs']) == (ro['spmv'], ro['rhs'])
+
+
+@pytest.mark.parametrize('obj', ['er', 'logdet'])
+def test_fp32_mixed_single_pole_objectives(obj):
+    """FP32-mixed with the single-pole objectives against the FP64 HIP path: same selections on a
+    graph with distinct gains; marginals within 1e-5 relative."""
+    n = 400
+    e = graphs.barabasi_albert(n, 3, 13)
+    cand = graphs.candidates(n, e, 300, 8)
+    G = pg.Graph(n, e, 0.3)
+    sk = O.sketches(O.Graph(n, e, 0.3), cand, 16, 4, kind='rademacher')
+    kw = dict(q=16, B=4, tol=1e-9, obj=obj, sketch=sk)
+    r64 = pg.polar(G, cand, 5, 0.5, **kw)
+    r32 = pg.polar(G, cand, 5, 0.5, precision='fp32', **kw)
+    assert r32['sequence'] == r64['sequence']
+    assert _rel(r32['marginals'], r64['marginals']) < 1e-5
