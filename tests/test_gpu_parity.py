@@ -364,3 +364,22 @@ def test_fp32_mixed_single_pole_objectives(obj):
     r32 = pg.polar(G, cand, 5, 0.5, precision='fp32', **kw)
     assert r32['sequence'] == r64['sequence']
     assert _rel(r32['marginals'], r64['marginals']) < 1e-5
+
+
+def test_polar_weighted_hub_graph_parity():
+    """A weighted graph with two hubs of degree > 700 (weighted operator kernels, long-row chunks
+    and their finisher inside every solve, added unit-weight edges on top): the FP64 greedy equals
+    the oracle; FP32-mixed keeps the selections."""
+    n, e, w = hub_graph(n=1500, seed=9, weighted=True)
+    cand = graphs.candidates(n, e, 250, 4)
+    Go, G = O.Graph(n, e, 0.2, weights=w), pg.Graph(n, e, 0.2, weights=w)
+    tau, om, _ = pg.fixed_rule(0.5, 8, 0.2, G.Lam(4))
+    sk = O.sketches(Go, cand, 8, 6, kind='rademacher')
+    kw = dict(q=8, B=3, tol=1e-9, sketch=sk, poles=(tau, om))
+    ro = O.polar(Go, cand, 4, 0.5, arith='cr', **kw)
+    rg = pg.polar(G, cand, 4, 0.5, **kw)
+    assert rg['sequence'] == ro['sequence'] and (rg['spmv'], rg['rhs']) == (ro['spmv'], ro['rhs'])
+    assert _rel(rg['marginals'], ro['marginals']) < 1e-13
+    assert _rel(rg['trace_estimate'], ro['trace_estimate']) < 1e-13
+    r32 = pg.polar(G, cand, 4, 0.5, precision='fp32', **kw)
+    assert r32['sequence'] == ro['sequence'] and _rel(r32['marginals'], ro['marginals']) < 1e-4
