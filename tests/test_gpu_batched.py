@@ -195,3 +195,20 @@ def test_polar_columns_equals_per_s(precision):
         assert got['marginals'] == ref['marginals'], g
         assert got['trace_estimate'] == ref['trace_estimate'], g
         assert (got['spmv'], got['rhs']) == (ref['spmv'], ref['rhs']), g
+
+
+@pytest.mark.parametrize('obj', ['er', 'logdet'])
+def test_polar_batched_single_pole_objectives_equal_per_graph(obj):
+    """The batched greedy with the single-pole objectives (one pole at 0; logdet without the trace
+    tracker): one result per graph, equal to polar_gpu.polar run on that graph alone."""
+    Gs = _graphs()
+    k, B, q = 4, 3, 16
+    cands = [graphs.candidates(G.n, G.edges, 200 + 30 * g, 60 + g) for g, G in enumerate(Gs)]
+    sk = [pg.sketches(G, cands[g], q, 70 + g, kind='rademacher', on_device=True, precision='fp64')
+          for g, G in enumerate(Gs)]
+    out = batched.polar_batched(Gs, cands, k, 0.5, q=q, B=B, sketch=sk, obj=obj, precision='fp64')
+    for g, G in enumerate(Gs):
+        ref = pg.polar(G, cands[g], k, 0.5, q=q, B=B, sketch=sk[g], obj=obj, precision='fp64')
+        assert out[g]['sequence'] == ref['sequence'], g
+        assert out[g]['marginals'] == ref['marginals'], g
+        assert (out[g]['spmv'], out[g]['rhs'], out[g]['p']) == (ref['spmv'], ref['rhs'], 1), g
