@@ -4,31 +4,38 @@
 Metric (k-round greedy wall time + SpMM GB/s at n=4M; FP32-mixed vs FP64 speedup):
   * step = one full POLAR greedy workload of the c2 configuration: Chung-Lu n=100,000 (gamma 2.5,
     mean degree 20, seed 1), sigma=0.05, q=64 Rademacher, 24 poles, N=1,000,000 candidates,
-    k=20 rounds, B=8, for each s in {0.25, 0.5, 0.75}: the three greedy runs advance together -- the
-    bank as one solve over three row segments, every re-rank solve over the three runs' shortlists
-    as column groups of one operator (one adjacency pass); --schedule columns|segments|per-s for
-    the alternatives, all with the same results; inputs resident in HBM.
-    `value` = seconds of one step in the FP32-mixed form, timed as the reference times `seconds`
-    (polar_gpu.py:196-253: after the inputs are resident, through the final sync);
-    `step_wall_s` is the wall clock around the whole call including its host-side setup.
-  * the plain FP64 form is timed on the same inputs (fp64_seconds; speedup = fp64/fp32) and its
-    selections and marginals are compared with the FP32-mixed ones.
-  * SpMM (L + sigma I) X at the c5 shape (Chung-Lu n=4,000,000, m=34.7M, R=128) in FP64 and FP32,
-    GB/s on compulsory bytes; beside it the same kernel on a 2000 x 2000 lattice (same n, R), a
-    locality control whose neighbour gathers hit cache.
-  * roofline: the operator kernel k_opmul (the dominant kernel), its HIP-event durations from the
-    library over the timed region (events around that kernel only); `kernel_profile` breaks one
-    extra, untimed step down by kernel class (events around every kernel).
-  * cpu_baseline: the oracle (NumPy/SciPy FP64, one core) on a bounded sample at the c2 shape,
-    extrapolated to the step's column-iterations.
+    k=20 rounds, B=8, for each s in {0.25, 0.5, 0.75}, in the FP32-mixed form (FP32 vectors and
+    SpMM, FP64 recurrences and reductions, FP64 residual replacement every 24 iterations): the three
+    greedy runs advance together -- the bank as one solve over three row segments, every re-rank
+    solve over the three runs' shortlists as column groups of one operator; inputs resident in HBM.
+    `value` = seconds of one step, timed as the reference times `seconds` (polar_gpu.py:196-253:
+    after the inputs are resident, through the final sync); K steps are timed after W warmups.
+  * beside it on the same inputs: the FP32-mixed step without residual replacement (its cost), the
+    plain FP64 step (pinned per-iteration shifted update) and the FP64 step with the blocked
+    update (so the FP32 speedup can be read for precision alone), with the FP32-vs-FP64 agreement.
+  * c5_greedy: the headline configuration -- Chung-Lu n=4,000,000, m=34.7M, sigma=0.02, s=0.5,
+    q=128, 32 poles, N=10,000,000, k=50, B=16 -- run once in the FP32-mixed form and once in each
+    FP64 form: seconds, column-iterations, speedups, shared selections, marginal and trace errors,
+    and a per-round record of the picks that differ (isolated-pair picks, whose exact gains are
+    equal, with the FP64 re-rank score gap).
+  * SpMM (L + sigma I) X on the c5 graph at R=128 in FP64 and FP32, GB/s on compulsory bytes, and
+    the same kernel on a 2000 x 2000 lattice (same n, R): a locality control.
+  * roofline: the operator kernel k_opmul of the c2 step (the dominant kernel), its HIP-event
+    durations from the library over the timed region (events around that kernel only);
+    `kernel_profile` breaks one extra, untimed step down by kernel class.
+  * cpu_baseline: the oracle (NumPy/SciPy FP64) polar() on the c2 inputs for s=0.5, its block
+    solves' columns split over worker processes, under the reference's time_limit convention
+    (polar_gpu.py:215-216), run as a child process on the host while the GPU runs the c5 legs;
+    plus the c1 workload run to completion on one core beside the FP64 HIP path.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--skip-c5] [--skip-cpu] ...
 """
 import argparse
 import json
 import math
 import os
 import platform
+import subprocess
 import sys
 import time
 
@@ -40,10 +47,11 @@ sys.path.insert(0, ROOT)
 METRIC = 'k-round greedy wall time + SpMM GB/s at n=4M; FP32-mixed vs FP64 speedup'
 KERNELS = ['k_opmul', 'k_longfin', 'k_alpha', 'k_resid', 'k_scalars', 'k_update']
 HBM_PEAK_GBPS = 8000.0
+T_START = time.time()
 
 
 def log(*a):
-    print('[bench]', *a, file=sys.stderr, flush=True)
+    print('[bench %5.0fs]' % (time.time() - T_START), *a, file=sys.stderr, flush=True)
 
 
 def traffic_of(kernel, workload):
@@ -75,7 +83,7 @@ def copy_probe(torch):
 
 
 GRAPH = False     # direct launches: measured faster than hipGraph replay on the c2 step (0.997 vs 1.007 s)
-REPLACE = False
+REPLACE = None    # FP64 residual replacement: None = the library default (on for FP32-mixed)
 
 
 def step_seconds(outs, union):
@@ -89,7 +97,7 @@ def step_seconds(outs, union):
 SCHEDULE = 'hybrid'
 
 
-def run_step(pg, G, inp, sk, precision, union=None):
+def run_step(pg, G, inp, sk, precision, union=None, replace=None, block=None):
     """One step: the greedy for every s value.  Schedules (same results):
     'columns'  -- the s values are column groups of one operator (one adjacency pass per phase);
     'segments' -- `union` (a UnionGraph of one copy of G per s) runs them as row segments;
@@ -99,7 +107,7 @@ def run_step(pg, G, inp, sk, precision, union=None):
         from polar_amd import batched
         return batched.polar_columns(G, inp['cand'], inp['k'], list(inp['s_all']), q=inp['q'], B=inp['B'],
                                      tol=inp['tol'], sketch=sk, poles=poles, precision=precision, graph=GRAPH,
-                                     replace=REPLACE and precision == 'fp32',
+                                     replace=replace, block=block,
                                      bank_union=union if SCHEDULE == 'hybrid' else None)
     if union is not None:
         from polar_amd import batched
@@ -107,11 +115,11 @@ def run_step(pg, G, inp, sk, precision, union=None):
         return batched.polar_batched([G] * S, [inp['cand']] * S, inp['k'], list(inp['s_all']), q=inp['q'],
                                      B=inp['B'], tol=inp['tol'], sketch=[sk] * S, poles=poles,
                                      precision=precision, union=union, graph=GRAPH,
-                                     replace=REPLACE and precision == 'fp32')
+                                     replace=replace, block=block)
     outs = []
     for s, po in zip(inp['s_all'], poles):
         outs.append(pg.polar(G, inp['cand'], inp['k'], s, q=inp['q'], B=inp['B'], tol=inp['tol'], sketch=sk,
-                             poles=po, precision=precision, graph=GRAPH, replace=REPLACE and precision == 'fp32'))
+                             poles=po, precision=precision, graph=GRAPH, replace=replace, block=block))
     return outs
 
 
@@ -149,17 +157,11 @@ def _time_spmm(pg, torch, lib, L, G, R, reps):
     return res
 
 
-def spmm_bench(pg, torch, lib, L, n_scale, reps=10):
-    from polar_amd import configs, graphs
-    spec = configs.SPECS['c5']
-    n = int(spec['n'] * n_scale)
-    t = time.time()
-    e = graphs.chung_lu(n, spec['gamma'], spec['mean_deg'], spec['seed'], m=int(spec['m'] * n_scale))
-    log(f'c5 graph n={n} m={len(e)} built in {time.time() - t:.1f}s')
-    G = pg.Graph(n, e, spec['sigma'])
+def spmm_bench(pg, torch, lib, L, G, reps=10):
+    """polar_spmm on the c5 graph (Chung-Lu n=4M, m=34.7M) at R=128, FP64 and FP32."""
     R = 128
     res = _time_spmm(pg, torch, lib, L, G, R, reps)
-    return dict(n=n, m=len(e), nnz_offdiag=G.nnz, R=R, dmax=G.dmax, **res)
+    return dict(n=G.n, m=G.m, nnz_offdiag=G.nnz, R=R, dmax=G.dmax, **res)
 
 
 def spmm_lattice_bench(pg, torch, lib, L, n_scale, reps=10):
@@ -177,35 +179,129 @@ def spmm_lattice_bench(pg, torch, lib, L, n_scale, reps=10):
     return dict(n=n, m=len(e), nnz_offdiag=G.nnz, R=R, graph=f'{side}x{side} lattice', **res)
 
 
-def cpu_baseline(inp, step_colits, seconds_budget=20.0):
-    """Oracle msolve at the c2 shape on an 8-column block, bounded by iteration count; the cost
-    per column-iteration is scaled by the GPU step's column-iterations."""
-    from oracle import polar_oracle as O
-    Go = O.Graph(inp['n'], inp['edges'], inp['sigma'])
-    Z, F, _ = O.sketches(Go, inp['cand'][:16], inp['q'], inp['seed'], kind='rademacher')
-    active = np.unique(inp['cand'])
-    tau, om, _ = O.fixed_rule(0.5, inp['p'], inp['sigma'], Go.Lam(inp['k']))
-    W = Z[:, :8]
-    its = 8
-    t0 = time.perf_counter()
-    while True:
-        try:
-            O.msolve(O.Op(Go), W, tau, active, inp['tol'], maxit=its)
-        except RuntimeError:
-            pass
-        el = time.perf_counter() - t0
-        if el > seconds_budget / 4 or its >= 4096:
-            break
-        its *= 2
-        t0 = time.perf_counter()
-    per_colit = el / (its * W.shape[1])
-    return dict(value=per_colit * step_colits, unit='s', cores=1, kind='port',
-                sample=f'oracle msolve on 8 columns x {its} CG iterations at the c2 shape '
-                       f'(n={inp["n"]}, p={inp["p"]}, nA={len(active)}): {per_colit * 1e3:.3f} ms per '
-                       f'column-iteration, scaled by the step\'s {step_colits} column-iterations',
-                cpu_model=_cpu_model(), os_cpu_count=os.cpu_count(),
-                affinity=len(os.sched_getaffinity(0)), blas_threads=_blas_threads(),
-                note='scipy.sparse products are single-threaded; the oracle runs on one core')
+def _rel_max(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    n = min(len(a), len(b))
+    if n == 0:
+        return None
+    return float(np.max(np.abs(a[:n] - b[:n]) / np.maximum(np.abs(b[:n]), 1e-300)))
+
+
+def agreement(o32, o64):
+    """FP32-mixed against FP64 on the same inputs: selections, marginals, trace estimates."""
+    same = all(a['sequence'] == b['sequence'] for a, b in zip(o32, o64))
+    shared = [len(set(a['sequence']) & set(b['sequence'])) for a, b in zip(o32, o64)]
+    return dict(selections_identical=same, shared_selections=shared,
+                max_rel_marginal_err=max(_rel_max(a['marginals'], b['marginals']) for a, b in zip(o32, o64)),
+                max_rel_trace_err=max(_rel_max(a['trace_estimate'], b['trace_estimate']) for a, b in zip(o32, o64)))
+
+
+def tie_record(c, deg, runs, ref='fp64', other='fp32'):
+    """Rounds where two runs pick differently.  A pick (a, b) joining two vertices of degree 0 in
+    that run's current graph (base edges plus its earlier picks) has the exact fractional gain
+    sigma^-s - (sigma+2)^-s, whatever the pair; for such rounds the two picks' exact gains are equal
+    and the choice is a tie.  Also the re-rank scores of the reference run's shortlist: the top-2
+    relative gap and how many shortlisted pairs lie within 1e-6 of the best."""
+    cand = c['cand']
+    sig, s = c['sigma'], c['s']
+    closed = sig ** (-s) - (sig + 2.0) ** (-s)
+
+    def isolated_flags(seq):
+        used = set()
+        out = []
+        for e in seq:
+            a, b = (int(x) for x in cand[e])
+            out.append(bool(deg[a] == 0 and deg[b] == 0 and a not in used and b not in used))
+            used.update((a, b))
+        return out
+
+    ra, rb = runs[ref], runs[other]
+    ia, ib = isolated_flags(ra['sequence']), isolated_flags(rb['sequence'])
+    rows = []
+    for r, (ea, eb) in enumerate(zip(ra['sequence'], rb['sequence'])):
+        if ea == eb:
+            continue
+        sc = np.asarray(ra['trace_rounds'][r]['scores'], float)
+        best = float(sc.max()) if len(sc) else float('nan')
+        rows.append(dict(round=r, **{f'{ref}_pick': ea, f'{other}_pick': eb,
+                                     f'{ref}_isolated_pair': ia[r], f'{other}_isolated_pair': ib[r],
+                                     f'{ref}_marginal': ra['marginals'][r], f'{other}_marginal': rb['marginals'][r],
+                                     f'{ref}_rerank_gap': ra['rerank_gap'][r], f'{other}_rerank_gap': rb['rerank_gap'][r],
+                                     f'{ref}_near_ties_1e-6': int(np.sum(np.abs(sc - best) <= 1e-6 * abs(best)))}))
+    return dict(closed_form_isolated_pair_gain=closed, rounds_differing=len(rows),
+                all_differing_picks_isolated_pairs=all(x[f'{ref}_isolated_pair'] and x[f'{other}_isolated_pair']
+                                                       for x in rows),
+                max_ref_rerank_gap_differing=max((x[f'{ref}_rerank_gap'] for x in rows), default=None),
+                isolated_pair_picks={ref: int(sum(ia)), other: int(sum(ib))},
+                differing=rows)
+
+
+def c5_greedy(pg, torch, c, G, k, legs):
+    """The headline configuration, once per form, on the same graph, pool, sketch and poles."""
+    sk = pg.sketches(G, c['cand'], c['q'], c['seed'], kind='rademacher', on_device='lazy')
+    kw = dict(q=c['q'], B=c['B'], tol=c['tol'], sketch=sk, poles=(c['tau'], c['om']))
+    forms = dict(fp32=dict(precision='fp32'), fp64=dict(precision='fp64'),
+                 fp64_blocked=dict(precision='fp64', block=True))
+    runs = {}
+    for name in legs:
+        tr = []
+        t = time.time()
+
+        def prog(stage, info, name=name):
+            if stage == 'bank' or info.get('round', 0) % 10 == 9:
+                log(f'c5 {name} {stage} {json.dumps({a: info[a] for a in info if a != "marginal"})}')
+        r = pg.polar(G, c['cand'], k, c['s'], trace=tr, progress=prog, **forms[name], **kw)
+        r['trace_rounds'] = tr
+        r['wall_s'] = time.time() - t
+        runs[name] = r
+        log(f'c5 {name}: {r["seconds"]:.1f} s, spmv {r["spmv"]}')
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    out = dict(workload=f'c5: Chung-Lu n={c["n"]}, gamma 2.2, m={c["m"]}, seed {c["seed"]}; sigma={c["sigma"]}, '
+                        f's={c["s"]}, q={c["q"]} Rademacher, p={c["p"]} poles, N={c["N"]}, k={k}, B={c["B"]}, '
+                        f'tol={c["tol"]}',
+               legs={n: dict(seconds=r['seconds'], wall_s=r['wall_s'], column_iterations=r['spmv'], rhs=r['rhs'],
+                             replace=r['replace'], rounds=r['rounds'],
+                             form={'fp32': 'FP32-mixed (blocked update, FP64 residual replacement every 24 '
+                                           'iterations)', 'fp64': 'FP64, pinned per-iteration shifted update',
+                                   'fp64_blocked': 'FP64, blocked shifted update'}[n])
+                     for n, r in runs.items()})
+    if 'fp32' in runs:
+        for ref in ('fp64', 'fp64_blocked'):
+            if ref in runs:
+                out[f'speedup_vs_{ref}'] = runs[ref]['seconds'] / runs['fp32']['seconds']
+        if 'fp64' in runs:
+            deg = np.bincount(c['edges'][:, 0], minlength=c['n']) + np.bincount(c['edges'][:, 1], minlength=c['n'])
+            a, b = runs['fp32'], runs['fp64']
+            first = next((r for r, (x, y) in enumerate(zip(a['sequence'], b['sequence'])) if x != y), None)
+            out['fp32_vs_fp64'] = dict(agreement([a], [b]), first_differing_round=first,
+                                       marginals_fp32=a['marginals'], marginals_fp64=b['marginals'])
+            out['ties'] = tie_record(c, deg, runs)
+    if 'fp64' in runs and 'fp64_blocked' in runs:
+        out['fp64_blocked_vs_fp64'] = agreement([runs['fp64_blocked']], [runs['fp64']])
+    return out
+
+
+def start_cpu_child(a):
+    """The CPU-baseline child process, started before this process touches the GPU; it prepares
+    its inputs, then waits for a line on stdin before it starts its timed run."""
+    cmd = [sys.executable, '-m', 'oracle.cpu_baseline', '--config', 'c2', '--s', '0.5', '--workers',
+           str(a.cpu_workers), '--time-limit', str(a.cpu_time_limit), '--wait']
+    return subprocess.Popen(cmd, cwd=ROOT, stdin=subprocess.PIPE, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                            text=True)
+
+
+def finish_cpu_child(proc, deadline):
+    try:
+        out, err = proc.communicate(timeout=max(1.0, deadline - time.time()))
+    except subprocess.TimeoutExpired:
+        proc.kill()
+        proc.communicate()
+        return dict(error=f'not finished by the bench deadline ({deadline - T_START:.0f} s after start)')
+    lines = [x for x in out.splitlines() if x.startswith('{')]
+    if proc.returncode != 0 or not lines:
+        return dict(error=f'exit {proc.returncode}: {err[-500:]}')
+    return json.loads(lines[-1])
 
 
 def cpu_c1(pg):
@@ -250,30 +346,38 @@ def _cpu_model():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--config', default='c2')
     ap.add_argument('--scale', type=float, default=1.0)
     ap.add_argument('--skip-fp64', action='store_true')
     ap.add_argument('--skip-cpu', action='store_true')
     ap.add_argument('--skip-spmm', action='store_true')
-    ap.add_argument('--spmm-scale', type=float, default=1.0)
+    ap.add_argument('--skip-c5', action='store_true')
+    ap.add_argument('--c5-k', type=int, default=50)
+    ap.add_argument('--c5-legs', default='fp32,fp64,fp64_blocked')
+    ap.add_argument('--cpu-workers', type=int, default=12)
+    ap.add_argument('--cpu-time-limit', type=float, default=60.0)
+    ap.add_argument('--deadline', type=float, default=540.0,
+                    help='seconds after start by which the CPU-baseline child must have finished')
     ap.add_argument('--schedule', choices=['hybrid', 'columns', 'segments', 'per-s'], default='hybrid',
                     help='how the s values of a step share the GPU (same results)')
     ap.add_argument('--graph', choices=['on', 'off'], default='off',
-                    help='hipGraph replay of the CG iterations of unprofiled solves (the FP64 leg)')
-    ap.add_argument('--replace', action='store_true', help='FP64 residual replacement in the FP32-mixed step')
+                    help='hipGraph replay of the CG iterations')
+    ap.add_argument('--replace', choices=['default', 'on', 'off'], default='default',
+                    help='FP64 residual replacement in the FP32-mixed step (default: on)')
     ap.add_argument('--spmm-only', action='store_true',
                     help='only the n=4M SpMM leg and the 1 GiB copy probe (for counter runs)')
     a = ap.parse_args()
     global GRAPH, REPLACE, SCHEDULE
     SCHEDULE = a.schedule
     GRAPH = a.graph == 'on'
-    REPLACE = a.replace
+    REPLACE = {'default': None, 'on': True, 'off': False}[a.replace]
     if a.gpus != 1:
         print(f'bench.py measures one GPU only (asked for --gpus {a.gpus}); the pole-sharded multi-GPU '
               'layout is a design note, not built.', file=sys.stderr)
         sys.exit(2)
+    cpu_proc = None if (a.skip_cpu or a.spmm_only) else start_cpu_child(a)
 
     import torch
     from polar_amd import _lib as L
@@ -284,8 +388,10 @@ def main():
 
     if a.spmm_only:
         copy_probe(torch)
-        print(json.dumps(dict(spmm_n4m=spmm_bench(pg, torch, lib, L, a.spmm_scale),
-                              spmm_lattice_n4m=spmm_lattice_bench(pg, torch, lib, L, a.spmm_scale))), flush=True)
+        c = configs.make('c5')
+        G5 = pg.Graph(c['n'], c['edges'], c['sigma'])
+        print(json.dumps(dict(spmm_n4m=spmm_bench(pg, torch, lib, L, G5),
+                              spmm_lattice_n4m=spmm_lattice_bench(pg, torch, lib, L, 1.0))), flush=True)
         return
 
     t = time.time()
@@ -303,8 +409,9 @@ def main():
         union = batched.UnionGraph([G] * len(inp['s_all']))
 
     for w in range(a.warmup):
-        run_step(pg, G, inp, sk, 'fp32', union)
+        run_step(pg, G, inp, sk, 'fp32', union, replace=REPLACE)
         log(f'warmup {w} done')
+
     def read_prof():
         out = {}
         for cls, name in enumerate(KERNELS + ['solves', 'iterations']):
@@ -322,7 +429,7 @@ def main():
     outs32 = None
     timed = 0.0
     for i in range(a.steps):
-        outs32 = run_step(pg, G, inp, sk, 'fp32', union)
+        outs32 = run_step(pg, G, inp, sk, 'fp32', union, replace=REPLACE)
         timed += step_seconds(outs32, union)
         log(f'step {i} done')
     torch.cuda.synchronize()
@@ -332,34 +439,57 @@ def main():
     op_prof, op_total = prof_t['k_opmul'], prof_t['iterations']['launches']
     # one more, untimed step with events around every kernel class (the per-class breakdown)
     lib.polar_prof_set(1)
-    run_step(pg, G, inp, sk, 'fp32', union)
+    run_step(pg, G, inp, sk, 'fp32', union, replace=REPLACE)
     torch.cuda.synchronize()
     prof = read_prof()
     lib.polar_prof_set(0)
-    log('fp32 step %.3fs' % el32, json.dumps(prof))
+    replaced = bool(outs32[0]['replace'])
+    log('fp32 step %.3fs (replacement %s)' % (el32, replaced), json.dumps(prof))
 
-    el64 = None
-    agree = None
+    # the same step with the other replacement setting: the cost of residual replacement
+    other = run_step(pg, G, inp, sk, 'fp32', union, replace=not replaced)
+    torch.cuda.synchronize()
+    el_other = step_seconds(other, union)
+    repl_cost = dict(step_with_replacement_s=el32 if replaced else el_other,
+                     step_without_replacement_s=el_other if replaced else el32)
+    repl_cost['ratio'] = repl_cost['step_with_replacement_s'] / repl_cost['step_without_replacement_s']
+    log('replacement cost', json.dumps(repl_cost))
+
+    fp64 = {}
     if not a.skip_fp64:
-        run_step(pg, G, inp, sk, 'fp64', union) if a.warmup else None
-        torch.cuda.synchronize()
-        outs64 = run_step(pg, G, inp, sk, 'fp64', union)
-        torch.cuda.synchronize()
-        el64 = step_seconds(outs64, union)
-        same = all(o32['sequence'] == o64['sequence'] for o32, o64 in zip(outs32, outs64))
-        mrel = max(float(np.max(np.abs(np.array(o32['marginals']) - o64['marginals']) / np.abs(o64['marginals'])))
-                   for o32, o64 in zip(outs32, outs64))
-        trel = max(float(abs(o32['trace_estimate'][0] - o64['trace_estimate'][0]) / abs(o64['trace_estimate'][0]))
-                   for o32, o64 in zip(outs32, outs64))
-        agree = dict(selections_identical=same, max_rel_marginal_err=mrel, max_rel_trace_err=trel)
-        log('fp64 step %.3fs' % el64, json.dumps(agree))
+        for name, blk in (('fp64', None), ('fp64_blocked', True)):
+            if a.warmup:
+                run_step(pg, G, inp, sk, 'fp64', union, block=blk)
+            torch.cuda.synchronize()
+            o64 = run_step(pg, G, inp, sk, 'fp64', union, block=blk)
+            torch.cuda.synchronize()
+            fp64[name] = dict(seconds=step_seconds(o64, union), outs=o64)
+            log(f'{name} step %.3fs' % fp64[name]['seconds'])
 
-    spmm = lattice = None
-    if not a.skip_spmm:
-        spmm = spmm_bench(pg, torch, lib, L, a.spmm_scale)
-        log('spmm', json.dumps(spmm))
-        lattice = spmm_lattice_bench(pg, torch, lib, L, a.spmm_scale)
-        log('spmm lattice', json.dumps(lattice))
+    if cpu_proc is not None:
+        cpu_proc.stdin.write('go\n')
+        cpu_proc.stdin.flush()
+        log('cpu baseline child started its timed run')
+
+    spmm = lattice = c5 = None
+    if not (a.skip_spmm and a.skip_c5):
+        t = time.time()
+        c = configs.make('c5')
+        G5 = pg.Graph(c['n'], c['edges'], c['sigma'])
+        log(f'c5 inputs (graph, pool) in {time.time() - t:.1f}s')
+        if not a.skip_spmm:
+            spmm = spmm_bench(pg, torch, lib, L, G5)
+            log('spmm', json.dumps(spmm))
+            lattice = spmm_lattice_bench(pg, torch, lib, L, 1.0)
+            log('spmm lattice', json.dumps(lattice))
+        if not a.skip_c5:
+            try:
+                c5 = c5_greedy(pg, torch, c, G5, a.c5_k, [x for x in a.c5_legs.split(',') if x])
+            except Exception as ex:          # the c2 record still prints
+                c5 = dict(error=repr(ex))
+                log('c5 failed', repr(ex))
+        del G5, c
+        torch.cuda.empty_cache()
 
     colits = int(sum(o['spmv'] for o in outs32))
     dom = max(KERNELS, key=lambda k: prof[k]['ms'])       # k_opmul on every config measured
@@ -376,29 +506,56 @@ def main():
                     timed_launches=d['launches'], launches=op_total,
                     share_of_timed_region=d['ms'] / max(d['launches'], 1) * op_total * 1e-3 / (el32 * a.steps))
     cpu = None
-    if not a.skip_cpu:
-        cpu = cpu_baseline(inp, colits)
+    if cpu_proc is not None:
+        run = finish_cpu_child(cpu_proc, T_START + a.deadline)
+        log('cpu child', json.dumps({x: run[x] for x in run if x != 'sequence'}))
+        gpu_s05 = outs32[len(outs32) // 2] if len(outs32) > 1 else outs32[0]
+        if 'error' in run:
+            cpu = dict(value=None, unit='s', cores=a.cpu_workers, kind='port', error=run['error'])
+        else:
+            same = run['sequence'] == gpu_s05['sequence'][:len(run['sequence'])]
+            cpu = dict(value=run['seconds'], unit='s', cores=run['workers'], kind='port',
+                       sample=f'oracle polar() (NumPy/SciPy FP64) on the c2 inputs for s=0.5 (one of the step\'s '
+                              f'three s values), time_limit={run["time_limit"]:.0f} s in the reference\'s convention: '
+                              f'>{run["seconds"]:.0f} s ({run["rounds"]} of {run["k"]} rounds); bank (all '
+                              f'{2 * run["q"]} columns) {run["bank_seconds"]:.1f} s, then {run["per_round_seconds"]:.1f} s '
+                              f'per round; block-solve columns split over {run["workers"]} worker processes',
+                       rounds=run['rounds'], of_rounds=run['k'], truncated=run['truncated'],
+                       bank_seconds=run['bank_seconds'], per_round_seconds=run['per_round_seconds'],
+                       column_iterations=run['spmv'], sequence_prefix_matches_gpu=same,
+                       cpu_model=run['cpu_model'], os_cpu_count=run['os_cpu_count'], affinity=run['affinity'],
+                       blas_threads=_blas_threads(),
+                       note='ran on the host concurrently with the GPU c5 legs; scipy.sparse products are '
+                            'single-threaded within each worker')
         cpu['c1_to_completion'] = cpu_c1(pg)
         log('cpu', json.dumps(cpu))
+    agree = {n: agreement(outs32, v['outs']) for n, v in fp64.items()}
+    el64 = fp64.get('fp64', {}).get('seconds')
+    el64b = fp64.get('fp64_blocked', {}).get('seconds')
     out = dict(metric=METRIC, value=el32, unit='s', n_gpus=1, steps=a.steps, warmup=a.warmup,
                ms_per_step=el32 * 1e3, step_wall_s=wall32, higher_is_better=False, scaling='weak', vs_baseline=None,
                dtype='fp32 vectors / fp64 recurrences', data='synthetic',
                config=dict(workload=f'{a.config}: Chung-Lu n={inp["n"]}, m={inp["m"]}, sigma={inp["sigma"]}, '
                                     f's={list(inp["s_all"])}, q={inp["q"]}, p={inp["p"]}, N={inp["N"]}, '
                                     f'k={inp["k"]}, B={inp["B"]}, tol={inp["tol"]}',
+                           headline_workload='c5 (see c5_greedy): Chung-Lu n=4M, m=34.7M, k=50, q=128, 32 poles, '
+                                             'N=1e7, B=16',
                            precision='fp32-mixed', column_iterations=colits,
-                           hipgraph=GRAPH, residual_replacement=REPLACE,
-                           residual_replacement_note=('FP64 residual replacement every 24 iterations'
-                                                      if REPLACE else 'off by default: --replace turns on FP64 '
-                                                      'residual replacement every 24 iterations (measured: same '
-                                                      'FP32 error, 33% slower step)'),
+                           hipgraph=GRAPH, residual_replacement=replaced,
+                           residual_replacement_note='FP64 residual replacement every 24 iterations (three '
+                                                     '8-iteration windows of the blocked update; the c2 '
+                                                     'configuration names 25)',
                            s_schedule={'hybrid': f'{len(inp["s_all"])} s values: bank as row segments, re-rank as column groups',
                                        'columns': f'{len(inp["s_all"])} s values as column groups of one operator',
                                        'segments': f'{len(inp["s_all"])} s values as segments of one batched solve',
                                        'per-s': 'one greedy run per s value'}[a.schedule],
                            rhs=int(sum(o['rhs'] for o in outs32))),
-               fp64_seconds=el64, fp32_vs_fp64_speedup=(el64 / el32) if el64 else None,
-               fp32_vs_fp64=agree, spmm_n4m=spmm, spmm_lattice_n4m=lattice, roofline=roofline, cpu_baseline=cpu,
+               replacement_cost=repl_cost,
+               fp64_seconds=el64, fp64_blocked_seconds=el64b,
+               fp32_vs_fp64_speedup=(el64 / el32) if el64 else None,
+               fp32_vs_fp64_blocked_speedup=(el64b / el32) if el64b else None,
+               fp32_vs_fp64=agree.get('fp64'), fp32_vs_fp64_blocked=agree.get('fp64_blocked'),
+               c5_greedy=c5, spmm_n4m=spmm, spmm_lattice_n4m=lattice, roofline=roofline, cpu_baseline=cpu,
                kernel_profile=prof, kernel_profile_note='one extra untimed FP32 step with HIP events around '
                'every kernel class (the timed steps carry events around k_opmul on one iteration in 8)')
     print(json.dumps(out), flush=True)

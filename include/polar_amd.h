@@ -3,6 +3,8 @@
  * Every entry point takes raw device pointers owned by the caller, sizes as int64_t, and an
  * explicit hipStream_t (passed as void*).  The library allocates no device memory: temporaries
  * live in a caller-provided workspace whose size is returned by the matching *_ws_bytes query.
+ * Host-side state: polar_msolve keeps, per calling thread, a small pinned buffer for its
+ * convergence checks and (with POLAR_GRAPH) a capture stream; polar_release() frees them.
  * Return value: 0 on success, a positive hipError_t, or a negative POLAR_E_* code; the text of
  * the last error is returned by polar_last_error().  No entry point synchronises the stream
  * except polar_msolve (a 4-byte convergence-flag read every 8 iterations, as the reference's
@@ -14,7 +16,8 @@
  * (POLAR_E_ARG).
  *
  * Reference interfaces replaced (chenty2333/POLAR):
- *   realworld/spmm.py:3-25          csr_spmm(n, R, ptr, idx, val, X, Y)   -> polar_spmm
+ *   realworld/spmm.py:3-14          csr_spmm(n, R, ptr, idx, val, X, Y)   -> polar_csr_spmm
+ *   realworld/spmm.py:18-25         spmm(G.M0, X) on the graph Laplacian  -> polar_spmm
  *   realworld/polar_gpu.py:63-80    Op.__call__(x, shift)                  -> polar_spmm (patch)
  *   realworld/polar_gpu.py:88-133   msolve(op, W, tau, active, tol, ...)   -> polar_msolve
  *   realworld/polar_gpu.py:205      bank gather-reduce h, t                -> polar_pair_sqnorm
@@ -37,7 +40,7 @@
 extern "C" {
 #endif
 
-#define POLAR_ABI_VERSION 4
+#define POLAR_ABI_VERSION 5
 
 enum polar_dtype { POLAR_F64 = 0, POLAR_F32 = 1 };
 
@@ -111,6 +114,11 @@ typedef struct polar_segs {
 int polar_abi_version(void);
 const char* polar_last_error(void);
 
+/* Frees the calling thread's host-side solver resources (the pinned convergence-check buffer and
+ * the hipGraph capture stream); the next polar_msolve call recreates them.  Returns 0 or the
+ * hipError_t of a failed free.  No counterpart in the reference (CuPy owns its pools). */
+int polar_release(void);
+
 /* Live profiling of the CG iteration kernels with HIP events (no counterpart in the reference;
  * used by the benchmark).  polar_prof_set(1) resets and enables it for every class;
  * polar_prof_set(2) times class 0 only (two events per iteration instead of seven);
@@ -125,7 +133,18 @@ const char* polar_last_error(void);
 int polar_prof_set(int32_t on);
 int polar_prof_get(int32_t cls, double* ms, int64_t* launches, double* bytes);
 
-/* Y[:, :R] = (M0 + shift I + sum_S b b^T) X[:, :R]; X, Y row-major with leading dims ldx, ldy. */
+/* General CSR product Y[:, :R] = A X[:, :R] (realworld/spmm.py:3-14, csr_spmm(n, R, ptr, idx, val,
+ * X, Y)): A is n rows of CSR with int32 indptr[n+1] and indices[nnz] and values[nnz] of `dtype`
+ * (any values, diagonal stored like the reference's G.M0), X and Y row-major with leading dims
+ * ldx, ldy (the reference's kernel has ldx = ldy = R).  Each row is summed in entry order; rows
+ * with no entries give 0.  Any R >= 1 (the reference hands R < 8 to cuSPARSE, the same product).
+ * Allocates nothing, needs no workspace; asynchronous on `stream`. */
+int polar_csr_spmm(int32_t dtype, int64_t n, int64_t R, const int32_t* indptr, const int32_t* indices,
+                   const void* values, const void* X, int64_t ldx, void* Y, int64_t ldy, void* stream);
+
+/* Y[:, :R] = (M0 + shift I + sum_S b b^T) X[:, :R]; X, Y row-major with leading dims ldx, ldy:
+ * the Laplacian operator of polar_op (value-free adjacency + diagonal + added-edge patch), the
+ * form every solve uses (realworld/polar_gpu.py:71-80 Op.__call__ through spmm.py:18-25). */
 int64_t polar_spmm_ws_bytes(const polar_op* op, int64_t R);
 int polar_spmm(const polar_op* op, double shift, const void* X, int64_t ldx, void* Y, int64_t ldy,
                int64_t R, void* ws, int64_t ws_bytes, void* stream);

@@ -310,6 +310,40 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
     return out
 
 
+# Column-parallel msolve for the CPU baseline: the columns of a block solve are independent
+# (per-column CG scalars, per-column reductions), so splitting them over processes gives the same
+# results as one call.  The operator's base matrix reaches the workers through fork (module state
+# set before the pool starts); each task carries its added edges and its columns.
+_POOL_OP = None
+
+
+def _msolve_task(args):
+    A, B, W, tau, active, tol, kw = args
+    op = _POOL_OP
+    op.A, op.B = list(A), list(B)
+    return msolve(op, W, tau, active, tol, **kw)
+
+
+def pmsolve(pool, nworkers, op, W, tau, active, tol, counter=None, **kw):
+    """msolve over a process pool (same outputs as msolve): columns split into nworkers blocks."""
+    R = W.shape[1]
+    cuts = np.linspace(0, R, min(nworkers, R) + 1).astype(int)
+    tasks = [(op.A, op.B, np.ascontiguousarray(W[:, a:b]), tau, active, tol, kw)
+             for a, b in zip(cuts[:-1], cuts[1:])]
+    parts = pool.map(_msolve_task, tasks)
+    X = np.concatenate([o[0] for o in parts], axis=2)
+    h = None if parts[0][1] is None else np.concatenate([o[1] for o in parts], axis=1)
+    iters = np.concatenate([o[2] for o in parts])
+    if counter is not None:
+        counter.spmv += int(iters.sum())
+        counter.rhs += R
+        counter.solves.append(int(iters.sum()))
+    out = (X, h, iters)
+    if kw.get('trace'):
+        out = out + (np.concatenate([o[3] for o in parts], axis=1),)
+    return out
+
+
 # ------------------------------------------------------------------------------------------------
 # Sketches.  polar_gpu.py:137-165 ('block', 'gauss') plus the counter-hash Rademacher kind.
 def _block_rows(rows, q, rng, center=False):
@@ -393,13 +427,34 @@ def chunk_width(n, p, nA, budget=10e9):
 # ------------------------------------------------------------------------------------------------
 def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=None, naive=False,
           seed=2026, kind='block', trace=None, time_limit=None, poles=None, width=None,
-          freeze=True, arith='numpy'):
+          freeze=True, arith='numpy', workers=1):
     """POLAR greedy with cache backend (polar_gpu.py:180-254).
 
     Extensions: `poles=(tau, om)` overrides the rule; `width` overrides chunk_width; the result
     also carries `marginals` (per-round G_q(e*;S) = sum_j om_j ||u_j||^2/(1+b^T u_j) of the winner;
-    the winner's proxy score for naive) and `trace_estimate` (sketched sum_j om_j tr(Z^T(M_S+tau_j)^-1 Z)
-    after each round, index 0 = before any addition)."""
+    the winner's proxy score for naive), `trace_estimate` (sketched sum_j om_j tr(Z^T(M_S+tau_j)^-1 Z)
+    after each round, index 0 = before any addition) and `bank_seconds`.  workers > 1 splits the
+    columns of every block solve over that many forked processes (same results; used to time the
+    CPU baseline on a many-core host)."""
+    if workers > 1:
+        import multiprocessing
+        global _POOL_OP
+        _POOL_OP = Op(G, arith=arith)
+        pool = multiprocessing.get_context('fork').Pool(workers)
+        try:
+            return _polar(G, cand, k, s, q, B, eps, tol, obj, sketch, naive, seed, kind, trace, time_limit,
+                          poles, width, freeze, arith, (pool, workers))
+        finally:
+            pool.terminate()
+            _POOL_OP = None
+    return _polar(G, cand, k, s, q, B, eps, tol, obj, sketch, naive, seed, kind, trace, time_limit,
+                  poles, width, freeze, arith, None)
+
+
+def _polar(G, cand, k, s, q, B, eps, tol, obj, sketch, naive, seed, kind, trace, time_limit, poles, width,
+           freeze, arith, par):
+    def solve(*a, **kw):
+        return pmsolve(par[0], par[1], *a, **kw) if par else msolve(*a, **kw)
     n, N = G.n, len(cand)
     sigma = G.sigma
     if poles is None:
@@ -443,8 +498,8 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
         RHS = np.concatenate([Z, F], axis=1)
         parts, terms = [], []
         for lo in range(0, 2 * q, width):
-            out = msolve(op, RHS[:, lo:lo + width], tau, active, tol, counter=cnt,
-                         freeze=freeze, trace=want_trace)
+            out = solve(op, RHS[:, lo:lo + width], tau, active, tol, counter=cnt,
+                        freeze=freeze, trace=want_trace)
             parts.append(out[0])
             if want_trace:
                 ncol = min(max(q - lo, 0), out[0].shape[2])
@@ -462,6 +517,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
         return polesum(om_c * h / (1 + t + (sigma + tau_c) * h))
 
     X, Y, h, t, tr = bank()
+    bank_seconds = time.perf_counter() - t0
     traces.append(tr if want_trace else None)
     for rnd in range(k):
         if rnd and time_limit is not None and time.perf_counter() - t0 > time_limit:  # :215
@@ -481,8 +537,8 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             Wb[cand[short, 0], np.arange(Br)] = 1.0
             Wb[cand[short, 1], np.arange(Br)] = -1.0                  # :225-226
             need_track = obj != 'logdet'
-            U, hh, _ = msolve(op, Wb, tau, active, tol, track=need_track, counter=cnt,
-                              freeze=freeze)[:3]
+            U, hh, _ = solve(op, Wb, tau, active, tol, track=need_track, counter=cnt,
+                             freeze=freeze)[:3]
             ar = np.arange(Br)
             bu = U[:, es[short, 0], ar] - U[:, es[short, 1], ar]      # :230
             if obj == 'logdet':
@@ -494,7 +550,8 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             u = U[:, :, j]
             marg.append(float(gsc[j]))
         if trace is not None:
-            trace.append(dict(shortlist=[int(x) for x in short], chosen=e))
+            trace.append(dict(shortlist=[int(x) for x in short], chosen=e, proxy=sc[short].tolist(),
+                              scores=[] if naive else np.asarray(gsc, float).tolist()))
         seq.append(e)
         live[e] = False
         aa, bb = es[e]
@@ -527,7 +584,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
         traces = traces[:len(seq)]
     return dict(sequence=seq, seconds=time.perf_counter() - t0, spmv=cnt.spmv, rhs=cnt.rhs, p=p,
                 rounds=len(seq), truncated=len(seq) < k, marginals=marg,
-                trace_estimate=traces, solves=list(cnt.solves))
+                trace_estimate=traces, solves=list(cnt.solves), bank_seconds=bank_seconds)
 
 
 # ------------------------------------------------------------------------------------------------

@@ -13,7 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / 'libpolar_amd.so'
 
 POLAR_F64, POLAR_F32 = 0, 1
 POLAR_TRACK, POLAR_TRACE, POLAR_NOFREEZE, POLAR_GRAPH, POLAR_BLOCK8, POLAR_REPLACE = 1, 2, 4, 8, 16, 32
-ABI_VERSION = 4
+ABI_VERSION = 5
 POLAR_E_ITERCAP, POLAR_E_ARG = -2, -3
 CHUNK = 64  # POLAR_CHUNK: default long-row chunk length
 
@@ -23,7 +23,7 @@ EXPORTS = [
     'polar_select_ws_bytes', 'polar_select_topb', 'polar_pair_rhs', 'polar_rerank_pick',
     'polar_maint_prep', 'polar_maint_rank1', 'polar_maint_cache', 'polar_rows_axpy',
     'polar_rademacher', 'polar_edge_sketch', 'polar_prof_set', 'polar_prof_get', 'polar_pair_wsum',
-    'polar_msolve_batched_ws_bytes', 'polar_msolve_batched',
+    'polar_msolve_batched_ws_bytes', 'polar_msolve_batched', 'polar_csr_spmm', 'polar_release',
 ]
 
 vp = ctypes.c_void_p
@@ -58,6 +58,8 @@ _SIGS = {
     'polar_last_error': (ctypes.c_char_p, []),
     'polar_spmm_ws_bytes': (i64, [OPP, i64]),
     'polar_spmm': (i32, [OPP, f64, vp, i64, vp, i64, i64, vp, i64, vp]),
+    'polar_csr_spmm': (i32, [i32, i64, i64, vp, vp, vp, vp, i64, vp, i64, vp]),
+    'polar_release': (i32, []),
     'polar_msolve_ws_bytes': (i64, [OPP, i64, i32, i64, i32]),
     'polar_msolve': (i32, [OPP, vp, i64, i64, vp, i32, vp, i64, f64, i64, i32, vp, i64, i64, vp, vp,
                            vp, vp, i64, vp]),

@@ -151,7 +151,7 @@ def _seq(x, S):
 
 def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac', sketch=None, seed=2026,
                   kind='rademacher', poles=None, precision='fp32', graph=None, width=None, progress=None,
-                  tune=None, block=None, union=None, replace=False):
+                  tune=None, block=None, union=None, replace=None):
     """POLAR greedy (polar_gpu.py:180-254, cache backend) on S graphs in lockstep.
 
     graphs: list of Graph; cands: list of (N_g, 2) candidate arrays; s: one value or one per graph;
@@ -160,12 +160,14 @@ def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac'
     graph: hipGraph replay (None: the tuned choice, else True); tune: a launch-shape dict from
     polar_amd.autotune or True to tune the batched operator here (before the timed region);
     union: a UnionGraph of these graphs built earlier (reused: its device arrays stay resident).
+    replace: FP64 residual replacement in every solve; None = on for FP32, off for FP64.
     The same graph may appear several times (e.g. one graph under several s values).
     Returns one result dict per graph with the keys of polar_gpu.polar (`seconds` is the whole
     batch's wall time)."""
     lib = L.require_gpu()
     dev = pg._device()
     dtype = pg._dt(precision)
+    replace = pg._replacing(replace, block, dtype)
     dcode = L.dtype_code(dtype)
     stream = L.stream_ptr()
     S = len(graphs)
@@ -253,7 +255,7 @@ def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac'
 
     XY = torch.empty((p, nA, 2 * q), dtype=dtype, device=dev)
     if width is None:
-        width = pg.fit_width(n, p, nA, 2 * q, dtype, block=block)
+        width = pg.fit_width(n, p, nA, 2 * q, dtype, block=block, replace=replace)
     terms = [[] for _ in range(S)]
     for lo in range(0, 2 * q, width):
         hi = min(2 * q, lo + width)
@@ -375,7 +377,7 @@ def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac'
                 tr.append(tr[-1] - math.fsum(oms[g] * dn_h[:, 1] / dn_h[:, 0]))
         out.append(dict(sequence=seqs[g], seconds=seconds, spmv=int(spmv[g]), rhs=int(rhs[g]), p=p,
                         rounds=len(seqs[g]), truncated=len(seqs[g]) < k, marginals=margs[g],
-                        trace_estimate=tr[:max(len(seqs[g]), 1)],
+                        trace_estimate=tr[:max(len(seqs[g]), 1)], replace=replace,
                         precision='fp64' if dtype == torch.float64 else 'fp32'))
     return out
 
@@ -389,7 +391,7 @@ def single_segment(G, nA):
 
 def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac', sketch=None, seed=2026,
                   kind='rademacher', poles=None, precision='fp32', graph=None, width=None, block=None,
-                  replace=False, progress=None, bank_union=None):
+                  replace=None, progress=None, bank_union=None):
     """POLAR greedy (polar_gpu.py:180-254) for several s values on ONE graph, in lockstep, as column
     groups of one operator: group g is the run for s_list[g], with its own poles, its own bank
     columns [Z | F_g], its own caches and its own selected edges (patch entries tagged g).  Every
@@ -398,10 +400,12 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
 
     bank_union: a UnionGraph of len(s_list) copies of G; the bank then runs as one segmented solve
     on it (the groups' 2q columns side by side in rows, finished groups leave the grid) and the
-    re-rank solves as column groups (same results either way)."""
+    re-rank solves as column groups (same results either way).  replace: FP64 residual replacement
+    in every solve; None = on for FP32, off for FP64."""
     lib = L.require_gpu()
     dev = pg._device()
     dtype = pg._dt(precision)
+    replace = pg._replacing(replace, block, dtype)
     dcode = L.dtype_code(dtype)
     stream = L.stream_ptr()
     Gn = len(s_list)
@@ -478,7 +482,7 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
         for g in range(Gn):
             RHS_u[U.row0[g]:U.row0[g + 1]].copy_(RHS[:, g * W2:(g + 1) * W2])
         XYu = torch.empty((p, Gn * nA, W2), dtype=dtype, device=dev)
-        wu = width or pg.fit_width(U.n, p, Gn * nA, W2, dtype, block=block)
+        wu = width or pg.fit_width(U.n, p, Gn * nA, W2, dtype, block=block, replace=replace)
         for lo in range(0, W2, wu):
             hi = min(W2, lo + wu)
             res = msolve_batched(op_u, segs_u, RHS_u[:, lo:hi], tau_all, act_u, tol, trace=want_trace and lo < q,
@@ -496,7 +500,7 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
         C = W2 * Gn
         XY = torch.empty((p, nA, C), dtype=dtype, device=dev)
         if width is None:
-            width = pg.fit_width(n, p, nA, C, dtype, block=block)
+            width = pg.fit_width(n, p, nA, C, dtype, block=block, replace=replace)
         col_of = np.arange(C) // W2                                  # group of every bank column
         for lo in range(0, C, width):
             hi = min(C, lo + width)
@@ -609,6 +613,6 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
                 tr.append(tr[-1] - math.fsum(oms[g] * dn_h[:, 1] / dn_h[:, 0]))
         out.append(dict(sequence=seqs[g], seconds=seconds, spmv=int(spmv[g]), rhs=int(rhs[g]), p=p,
                         rounds=len(seqs[g]), truncated=len(seqs[g]) < k, marginals=margs[g],
-                        trace_estimate=tr[:max(len(seqs[g]), 1)],
+                        trace_estimate=tr[:max(len(seqs[g]), 1)], replace=replace,
                         precision='fp64' if dtype == torch.float64 else 'fp32'))
     return out

@@ -37,6 +37,7 @@ inline int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 template <typename T, int CPL> struct VecT;
 template <> struct VecT<double, 1> { using type = double; };
 template <> struct VecT<double, 2> { using type = double2; };
+template <> struct VecT<double, 4> { using type = double4; };   // FP64 rows beside FP32 vectors
 template <> struct VecT<float, 1> { using type = float; };
 template <> struct VecT<float, 2> { using type = float2; };
 template <> struct VecT<float, 4> { using type = float4; };

@@ -146,8 +146,10 @@ __device__ double col_total(const double* __restrict__ parts, int b0, int b1, in
 // Gather-accumulate acc[:] += sum_k w_k X[idx_k, c0:c0+CPL] over k in [beg, end), in k order.
 // The group's L lanes load 8 consecutive column indices each (8L per index round); batches of 8
 // gathers are issued before their ordered adds, so 8 row loads per lane are in flight.
-template <typename T, int CPL, bool WEIGHTED>
-__device__ __forceinline__ void gather_acc(const int32_t* __restrict__ idx, const T* __restrict__ wt,
+// Every lane of the group takes part (the lanes share the index loads), whether its columns are in
+// range (colok) or not.  TW: type of the stored weights (converted to T).
+template <typename T, int CPL, bool WEIGHTED, typename TW = T>
+__device__ __forceinline__ void gather_acc(const int32_t* __restrict__ idx, const TW* __restrict__ wt,
                                            const T* __restrict__ X, int64_t ldx, int beg, int end,
                                            const Geo& g, bool colok, int c0, T (&acc)[CPL]) {
   const int gbase = g.gi * g.L;
@@ -159,7 +161,7 @@ __device__ __forceinline__ void gather_acc(const int32_t* __restrict__ idx, cons
     for (int u = 0; u < 8; ++u) {
       const int t = g.li * 8 + u;
       myj[u] = (t < cnt) ? idx[k0 + t] : 0;
-      if (WEIGHTED) myw[u] = (t < cnt) ? wt[k0 + t] : T(0);
+      if (WEIGHTED) myw[u] = (t < cnt) ? (T)wt[k0 + t] : T(0);
     }
     for (int b = 0; 8 * b < cnt; ++b) {
       int j[8];
@@ -289,7 +291,10 @@ struct CGState {
   int* pdirty;                // [p][S] pole j has a changed column of segment g
   // Residual replacement: the seed iterate x0 (n x ld, FP64) accumulated beside the recurrences;
   // every few windows the residual becomes w - (M + tau0) x0, computed in FP64 from the
-  // operator values the solve uses.  NULL when off.
+  // operator values the solve uses.  NULL when off.  Per-iteration updates (T == 1): x0 += a d in
+  // k_resid.  Blocked updates (T > 1): x0 advances once per window like a shifted block, from the
+  // window coefficients of pole 0 (the seed pole: shift 0, zeta = 1, so its aj, bj are the seed's
+  // alpha, beta) over the basis {d at the window start, r_1..r_T}.
   double* x0;
   // Column groups (one segment): G groups, cgrp[slot] = group of the column in that slot.
   int G;
@@ -731,7 +736,7 @@ __global__ __launch_bounds__(kBlock) void k_resid(int64_t n, int R, int L, const
 #pragma unroll
         for (int e = 0; e < CPL; ++e)
           if (act[e]) rv[e] = rv[e] - (a[e] * vv[e]);
-        if (s.x0) {                          // x0 += alpha d (FP64), for residual replacement
+        if (s.x0 && s.T <= 1) {              // x0 += alpha d (FP64), for residual replacement
           T dv[CPL];
           vload<T, CPL>(d + i * s.ld + c0, dv);
           double* xp = s.x0 + i * s.ld + c0;
@@ -1021,7 +1026,9 @@ __global__ void k_coef_reset(CGState s) {
 
 // Residual replacement: r = w - (M + tau0) x0 on the rows of live segments, in FP64 from the
 // operator's own values (the product is the one the recurrences approximate), for active
-// columns; |r|^2 partials for k_replace_fin.  Each row is summed by one lane group.
+// columns; |r|^2 partials for k_replace_fin.  Each row is summed by one lane group, 8 FP64 x0
+// rows in flight per lane.  W is in the caller's column order: after active-column compaction
+// slot c holds the caller's column colmap[c].
 template <typename T, int CPL, bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void k_replace(polar_op op, double shift, const T* __restrict__ W,
                                                     int64_t ldw, T* __restrict__ r, int R, int L, CGState s,
@@ -1038,59 +1045,110 @@ __global__ __launch_bounds__(kBlock) void k_replace(polar_op op, double shift, c
   DD dacc[CPL];
 #pragma unroll
   for (int e = 0; e < CPL; ++e) dacc[e] = DD{0.0, 0.0};
-  if (colok) {
-    bool act[CPL];
+  bool act[CPL];
+  int wcol[CPL], gcol[CPL];
 #pragma unroll
-    for (int e = 0; e < CPL; ++e) act[e] = s.act[vb + c0 + e] != 0;
-    const int gpb = kWaves * g.G;
-    int pc = -1;
-    for (int64_t i = r0 + (int64_t)sp.rank * gpb + g.wave * g.G + g.gi; i < r1; i += (int64_t)sp.nb * gpb) {
-      const int row = (int)i;
-      double acc[CPL];
+  for (int e = 0; e < CPL; ++e) {
+    act[e] = colok && s.act[vb + c0 + e] != 0;
+    wcol[e] = !colok ? 0 : (s.ident ? c0 + e : s.colmap[c0 + e]);
+    gcol[e] = (colok && s.G > 1) ? s.cgrp[c0 + e] : 0;
+  }
+  const int gpb = kWaves * g.G;
+  int pc = -1;
+  // every lane of a group runs the same rows (the group shares the index loads)
+  for (int64_t i = r0 + (int64_t)sp.rank * gpb + g.wave * g.G + g.gi; i < r1; i += (int64_t)sp.nb * gpb) {
+    const int row = (int)i;
+    double xr[CPL];
+    double dgr = 0.0;
+    if (colok) {
+      vload<double, CPL>(s.x0 + i * s.ld + c0, xr);
+      dgr = (double)((const T*)op.diag)[row];
+    }
+    double acc[CPL];
 #pragma unroll
-      for (int e = 0; e < CPL; ++e) acc[e] = 0.0;
-      const int beg = op.indptr[row], end = op.indptr[row + 1];
-      for (int k = beg; k < end; ++k) {
-        const int64_t j = op.indices[k];
-        const double wk = WEIGHTED ? (double)((const T*)op.weights)[k] : 1.0;
-        const double* xj = s.x0 + j * s.ld + c0;
+    for (int e = 0; e < CPL; ++e) acc[e] = 0.0;
+    gather_acc<double, CPL, WEIGHTED, T>(op.indices, (const T*)op.weights, s.x0, s.ld, op.indptr[row],
+                                         op.indptr[row + 1], g, colok, c0, acc);
+    if (!colok) continue;
+    double y[CPL];
 #pragma unroll
-        for (int e = 0; e < CPL; ++e) acc[e] = acc[e] + wk * xj[e];
-      }
-      const double* xr = s.x0 + i * s.ld + c0;
-      double y[CPL];
-      int gcol[CPL];
+    for (int e = 0; e < CPL; ++e) {
+      const double dg = dgr + (s.G > 1 ? s.tau[gcol[e] * s.p] : shift);
+      y[e] = dg * xr[e] - acc[e];
+    }
+    if (op.n_patch > 0) {
+      if (pc < 0) pc = lower_bound_i32(op.patch_rows, op.n_patch, row);
+      while (pc < op.n_patch && op.patch_rows[pc] < row) ++pc;
+      if (pc < op.n_patch && op.patch_rows[pc] == row) {
+        for (int q = op.patch_ptr[pc]; q < op.patch_ptr[pc + 1]; ++q) {
+          const double* xq = s.x0 + (int64_t)op.patch_cols[q] * s.ld + c0;
+          const int pg = op.patch_grp ? op.patch_grp[q] : -1;
 #pragma unroll
-      for (int e = 0; e < CPL; ++e) {
-        gcol[e] = s.G > 1 ? s.cgrp[c0 + e] : 0;
-        const double dg = (double)((const T*)op.diag)[row] + (s.G > 1 ? s.tau[gcol[e] * s.p] : shift);
-        y[e] = dg * xr[e] - acc[e];
-      }
-      if (op.n_patch > 0) {
-        if (pc < 0) pc = lower_bound_i32(op.patch_rows, op.n_patch, row);
-        while (pc < op.n_patch && op.patch_rows[pc] < row) ++pc;
-        if (pc < op.n_patch && op.patch_rows[pc] == row) {
-          for (int q = op.patch_ptr[pc]; q < op.patch_ptr[pc + 1]; ++q) {
-            const double* xq = s.x0 + (int64_t)op.patch_cols[q] * s.ld + c0;
-            const int pg = op.patch_grp ? op.patch_grp[q] : -1;
-#pragma unroll
-            for (int e = 0; e < CPL; ++e)
-              if (pg < 0 || pg == gcol[e]) y[e] = y[e] + (xr[e] - xq[e]);
-          }
+          for (int e = 0; e < CPL; ++e)
+            if (pg < 0 || pg == gcol[e]) y[e] = y[e] + (xr[e] - xq[e]);
         }
       }
-      T wv[CPL], rv[CPL];
-      vload<T, CPL>(W + i * ldw + c0, wv);
-      vload<T, CPL>(r + i * s.ld + c0, rv);
-#pragma unroll
-      for (int e = 0; e < CPL; ++e)
-        if (act[e]) rv[e] = (T)((double)wv[e] - y[e]);
-      vstore<T, CPL>(r + i * s.ld + c0, rv);
-#pragma unroll
-      for (int e = 0; e < CPL; ++e) dd_add(dacc[e], (double)rv[e] * (double)rv[e]);
     }
+    T rv[CPL];
+    vload<T, CPL>(r + i * s.ld + c0, rv);
+#pragma unroll
+    for (int e = 0; e < CPL; ++e)
+      if (act[e]) rv[e] = (T)((double)W[i * ldw + wcol[e]] - y[e]);
+    vstore<T, CPL>(r + i * s.ld + c0, rv);
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) dd_add(dacc[e], (double)rv[e] * (double)rv[e]);
   }
   block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)blockIdx.x * 2 * R);
+}
+
+// Seed iterate of the blocked update, once per window (before the coefficients are reset):
+// x0 += cX0 dsave + sum_t cXt r_t with pole 0's window coefficients (pole 0 is the seed pole, so
+// its coefficients describe x0 and d in the window basis {d at the window start, r_1..r_T}); then
+// dsave = d, the next window's starting direction.  All rows of the live segments; FP64 x0.
+template <typename T, int CPL>
+__global__ __launch_bounds__(kBlock) void k_seed_flush(int64_t n, int R, int L, Ring<T> ring, const T* __restrict__ d,
+                                                       T* __restrict__ dsave, CGState s) {
+  const SegPart sp = seg_part(s.bD, s.S, blockIdx.x, gridDim.x);
+  int64_t r0, r1;
+  seg_range(s.seg_row, s.S, sp.g, n, r0, r1);
+  const int vb = sp.g * s.ld;
+  Geo g(L);
+  const int c0 = g.li * CPL;
+  if (c0 >= R) return;     // no barrier in this kernel
+  double cx[kW + 1][CPL];
+  bool any = false;
+#pragma unroll
+  for (int t = 0; t <= kW; ++t)
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) {
+      cx[t][e] = s.cX[(int64_t)t * s.sld + vb + c0 + e];      // pole 0: rows [0, T] of cX
+      any |= cx[t][e] != 0.0;
+    }
+  const int gpb = kWaves * g.G;
+  for (int64_t i = r0 + (int64_t)sp.rank * gpb + g.wave * g.G + g.gi; i < r1; i += (int64_t)sp.nb * gpb) {
+    T dv[CPL];
+    vload<T, CPL>(d + i * s.ld + c0, dv);
+    if (any) {
+      T b[CPL];
+      double xv[CPL];
+      double* xp = s.x0 + i * s.ld + c0;
+      vload<double, CPL>(xp, xv);
+      vload<T, CPL>(dsave + i * s.ld + c0, b);
+      double inc[CPL];
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) inc[e] = cx[0][e] * (double)b[e];
+#pragma unroll
+      for (int t = 1; t <= kW; ++t) {
+        vload<T, CPL>(ring.s[t % kW] + i * s.ld + c0, b);
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) inc[e] = inc[e] + cx[t][e] * (double)b[e];
+      }
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) xv[e] = xv[e] + inc[e];
+      vstore<double, CPL>(xp, xv);
+    }
+    vstore<T, CPL>(dsave + i * s.ld + c0, dv);
+  }
 }
 
 // After a replacement: rr of every active (segment, column) is the replaced residual's norm.
@@ -1331,6 +1389,7 @@ struct SolveMem {
   CGState s{};
   int* perm;
   T *ring, *d, *v, *t2, *P, *lpart;   // ring: Tw residual vectors (Tw = 1: the residual itself)
+  T* dsave;            // residual replacement with the blocked update: d at the window start
   int* dtab;           // 5 (S+1) block prefix tables
   int64_t* dseg;       // 4 (S+1) segment ranges
 };
@@ -1364,6 +1423,7 @@ static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int
     s.pdirty = cv.take<int>((int64_t)p * S);
   }
   if (repl) s.x0 = cv.take<double>(op->n * R);
+  m.dsave = (repl && Tw > 1) ? cv.take<T>(op->n * R) : nullptr;
   s.cgrp = cv.take<int>(R);
   s.colmap = cv.take<int>(R);
   s.sflag = cv.take<int>(2 * S);
@@ -1416,6 +1476,43 @@ static int spmm_impl(const polar_op* op, double shift, const T* X, int64_t ldx, 
   CGState s{};
   s.S = 1;
   return launch_opmul<T, CPL>(op, pl, shift, X, ldx, Y, ldy, (int)R, lpart, s, 0, false, st);
+}
+
+// ---------------------------------------------------------------- general CSR SpMM (csr_spmm)
+// Y[row, :] = sum_k val[k] X[idx[k], :] over the stored entries of each row (diagonal included, any
+// values), in entry order, for the column slab blockIdx.y (64 lanes x CPL columns at most).  One
+// lane group per row; rows are independent, so nothing needs scratch memory.
+template <typename T, int CPL>
+__global__ __launch_bounds__(kBlock) void k_csr_spmm(int64_t n, int R, int L, const int32_t* __restrict__ ptr,
+                                                     const int32_t* __restrict__ idx, const T* __restrict__ val,
+                                                     const T* __restrict__ X, int64_t ldx, T* __restrict__ Y,
+                                                     int64_t ldy) {
+  Geo g(L);
+  const int slab = (int)blockIdx.y * L * CPL;
+  const int c0 = slab + g.li * CPL;
+  const bool colok = c0 < R;
+  const int gpb = kWaves * g.G;
+  for (int64_t row = (int64_t)blockIdx.x * gpb + g.wave * g.G + g.gi; row < n; row += (int64_t)gridDim.x * gpb) {
+    T acc[CPL];
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) acc[e] = T(0);
+    gather_acc<T, CPL, true>(idx, val, X, ldx, ptr[row], ptr[row + 1], g, colok, c0, acc);
+    if (colok) vstore<T, CPL>(Y + row * ldy + c0, acc);
+  }
+}
+
+template <typename T, int CPL>
+static int csr_spmm_impl(int64_t n, int64_t R, const int32_t* ptr, const int32_t* idx, const T* val, const T* X,
+                         int64_t ldx, T* Y, int64_t ldy, hipStream_t st) {
+  const int L = std::min(64, lanes_for(R, CPL));
+  const int64_t gpb = (int64_t)kWaves * (64 / L);
+  const int64_t slabs = (R + (int64_t)L * CPL - 1) / ((int64_t)L * CPL);
+  if (slabs > 65535) return set_error(POLAR_E_ARG, "R=%lld too wide", (long long)R);
+  const int grid = (int)std::min<int64_t>(kOpGridLarge, std::max<int64_t>(1, (n + gpb - 1) / gpb));
+  hipLaunchKernelGGL((k_csr_spmm<T, CPL>), dim3(grid, (unsigned)slabs), dim3(kBlock), 0, st, n, (int)R, L, ptr, idx,
+                     val, X, ldx, Y, ldy);
+  POLAR_LAUNCH_CHECK();
+  return 0;
 }
 
 // ---------------------------------------------------------------- msolve
@@ -1556,10 +1653,12 @@ static int replace_residual(int cpl, const polar_op* op, const Plan& pl, double 
   return 0;
 }
 
-// End of a window of the blocked update: apply the coefficients to X, P, then start a new window.
+// End of a window of the blocked update: apply the coefficients to X, P (and, with residual
+// replacement, to the seed iterate x0), then start a new window.
 template <typename T>
 static int flush_window(int cpl, const Plan& pl, const Ring<T>& ring, const int32_t* active, int64_t nA, int p,
-                        T* P, T* Xo, int64_t sj, int64_t si, const CGState& s, hipStream_t st) {
+                        T* P, T* Xo, int64_t sj, int64_t si, const CGState& s, int64_t n, const T* d, T* dsave,
+                        hipStream_t st) {
   if (nA > 0 && pl.nb_s > 0) {
     // column slices of pl.Lf <= 8 lanes (128-byte row pieces): a block's tile has 32 rows, and
     // each staged pole serves 4x more rows than a full-width tile at 32 lanes
@@ -1573,6 +1672,16 @@ static int flush_window(int cpl, const Plan& pl, const Ring<T>& ring, const int3
       if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_flush<T, 4>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s);
     } else {
       hipLaunchKernelGGL((k_flush<T, 1>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s);
+    }
+    POLAR_LAUNCH_CHECK();
+  }
+  if (dsave) {
+    if (cpl == 2) {
+      hipLaunchKernelGGL((k_seed_flush<T, 2>), dim3(pl.nb_d), dim3(kBlock), 0, st, n, s.R, pl.L, ring, d, dsave, s);
+    } else if (cpl == 4) {
+      if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_seed_flush<T, 4>), dim3(pl.nb_d), dim3(kBlock), 0, st, n, s.R, pl.L, ring, d, dsave, s);
+    } else {
+      hipLaunchKernelGGL((k_seed_flush<T, 1>), dim3(pl.nb_d), dim3(kBlock), 0, st, n, s.R, pl.L, ring, d, dsave, s);
     }
     POLAR_LAUNCH_CHECK();
   }
@@ -1598,10 +1707,17 @@ static int cpl_for(int64_t Rc, int64_t ld, int cpl_max) {
   return 1;
 }
 
+// Per-thread host resources of the solver, released by polar_release(): the pinned buffer of the
+// convergence checks and the stream that hipGraph captures are recorded on.
+static thread_local int* t_status = nullptr;
+static thread_local int64_t t_status_cap = 0;
+static thread_local hipStream_t t_capture = nullptr;
+static thread_local int t_capture_dev = -1;
+
 // Pinned host buffer of a thread for the convergence checks (a DMA copy, not a staged one).
 static int* status_buffer(int64_t count, int& rc) {
-  static thread_local int* buf = nullptr;
-  static thread_local int64_t cap = 0;
+  int*& buf = t_status;
+  int64_t& cap = t_status_cap;
   rc = 0;
   if (count > cap) {
     if (buf) (void)hipHostFree(buf);
@@ -1618,8 +1734,8 @@ static int* status_buffer(int64_t count, int& rc) {
 
 // Stream of a thread's captures (default priority; captured work runs on the caller's stream).
 static hipStream_t capture_stream(int& rc) {
-  static thread_local hipStream_t cs = nullptr;
-  static thread_local int dev = -1;
+  hipStream_t& cs = t_capture;
+  int& dev = t_capture_dev;
   int cur = 0;
   rc = (int)hipGetDevice(&cur);
   if (rc) return nullptr;
@@ -1627,6 +1743,7 @@ static hipStream_t capture_stream(int& rc) {
     hipStream_t s = nullptr;
     rc = (int)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     if (rc) return nullptr;
+    if (cs) (void)hipStreamDestroy(cs);      // the previous device's stream
     cs = s;
     dev = cur;
   }
@@ -1739,6 +1856,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   POLAR_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_init_fin, dim3(pl.fin), dim3(kBlock), 0, st, s, pl.grid_rows);
   POLAR_LAUNCH_CHECK();
+  if (m.dsave) POLAR_HIP(hipMemcpyAsync(m.dsave, d, sizeof(T) * (size_t)n * R, hipMemcpyDeviceToDevice, st));
   if (nA > 0) {
     const int grid_sh = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (nA + gpb0 - 1) / gpb0));
     if (cpl == 2) {
@@ -1797,7 +1915,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
         for (int k = 0; k < 8 && !rc; ++k)
           rc = run_iter_any<T>(cpl, op, pl, tau0, slot[k % kW], slot[(k + 1) % kW], d, v, lpart, P, Xo, sj, si, active,
                                nA, p, s, k & 1, k, nullptr, cs);
-        if (!rc && Tw > 1) rc = flush_window<T>(cpl, pl, ring_of(), active, nA, p, P, Xo, sj, si, s, cs);
+        if (!rc && Tw > 1) rc = flush_window<T>(cpl, pl, ring_of(), active, nA, p, P, Xo, sj, si, s, n, d, m.dsave, cs);
         hipGraph_t gr = nullptr;
         const hipError_t ec = hipStreamEndCapture(cs, &gr);
         if (rc) {
@@ -1837,7 +1955,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
             if (!e) POLAR_HIP(hipEventCreate(&e));
           POLAR_HIP(hipEventRecord(evf[0], st));
         }
-        if (int rc = flush_window<T>(cpl, pl, ring_of(), active, nA, p, P, Xo, sj, si, s, st)) return rc;
+        if (int rc = flush_window<T>(cpl, pl, ring_of(), active, nA, p, P, Xo, sj, si, s, n, d, m.dsave, st)) return rc;
         if (fprof) POLAR_HIP(hipEventRecord(evf[1], st));
       }
     }
@@ -1915,6 +2033,8 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
       std::swap(d, t2);
       if (repl)
         if (int e = compact_rows<double>(n, R, Rn, perm, s.x0, st)) return e;
+      if (m.dsave)
+        if (int e = compact_rows<T>(n, R, Rn, perm, m.dsave, st)) return e;
       if (int e = compact_rows<T>((int64_t)p * nA, R, Rn, perm, P, st)) return e;
       for (double* a : {s.rr, s.stop, s.a, s.be, s.ap, s.bp})
         if (int e = compact_rows<double>(1, R, Rn, perm, a, st)) return e;
@@ -2071,6 +2191,23 @@ int polar_prof_get(int32_t cls, double* ms, int64_t* launches, double* bytes) {
 
 const char* polar_last_error(void) { return g_err; }
 
+int polar_release(void) {
+  int rc = 0;
+  if (t_status) {
+    const hipError_t e = hipHostFree(t_status);
+    if (e != hipSuccess) rc = hip_fail(e, "hipHostFree", __LINE__);
+    t_status = nullptr;
+    t_status_cap = 0;
+  }
+  if (t_capture) {
+    const hipError_t e = hipStreamDestroy(t_capture);
+    if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamDestroy", __LINE__);
+    t_capture = nullptr;
+    t_capture_dev = -1;
+  }
+  return rc;
+}
+
 int64_t polar_spmm_ws_bytes(const polar_op* op, int64_t R) {
   if (check_op(op)) return -1;
   return align_up((int64_t)op->n_partial * R * elem_size(op->dtype), 256) + 512;
@@ -2090,6 +2227,29 @@ int polar_spmm(const polar_op* op, double shift, const void* X, int64_t ldx, voi
   if (cpl == 4) return spmm_impl<float, 4>(op, shift, (const float*)X, ldx, (float*)Y, ldy, R, ws, ws_bytes, st);
   if (cpl == 2) return spmm_impl<float, 2>(op, shift, (const float*)X, ldx, (float*)Y, ldy, R, ws, ws_bytes, st);
   return spmm_impl<float, 1>(op, shift, (const float*)X, ldx, (float*)Y, ldy, R, ws, ws_bytes, st);
+}
+
+int polar_csr_spmm(int32_t dtype, int64_t n, int64_t R, const int32_t* indptr, const int32_t* indices,
+                   const void* values, const void* X, int64_t ldx, void* Y, int64_t ldy, void* stream) {
+  if (!dtype_ok(dtype)) return set_error(POLAR_E_ARG, "csr_spmm: dtype %d", dtype);
+  if (n <= 0 || R <= 0 || ldx < R || ldy < R) return set_error(POLAR_E_ARG, "csr_spmm: bad n/R/ld");
+  if (!indptr || !indices || !values || !X || !Y) return set_error(POLAR_E_ARG, "csr_spmm: null pointer");
+  if (n > INT32_MAX) return set_error(POLAR_E_ARG, "csr_spmm: n exceeds int32 row ids");
+  hipStream_t st = (hipStream_t)stream;
+  // widest vector (<= 16 bytes) that divides R and keeps every row of X and Y aligned; column
+  // slabs of 64 lanes cover any R
+  const int esz = elem_size(dtype);
+  int cpl = 16 / esz;
+  while (cpl > 1 && !(R % cpl == 0 && (uintptr_t)X % (cpl * esz) == 0 && (uintptr_t)Y % (cpl * esz) == 0 &&
+                      ldx % cpl == 0 && ldy % cpl == 0))
+    cpl /= 2;
+  if (dtype == POLAR_F64) {
+    if (cpl == 2) return csr_spmm_impl<double, 2>(n, R, indptr, indices, (const double*)values, (const double*)X, ldx, (double*)Y, ldy, st);
+    return csr_spmm_impl<double, 1>(n, R, indptr, indices, (const double*)values, (const double*)X, ldx, (double*)Y, ldy, st);
+  }
+  if (cpl == 4) return csr_spmm_impl<float, 4>(n, R, indptr, indices, (const float*)values, (const float*)X, ldx, (float*)Y, ldy, st);
+  if (cpl == 2) return csr_spmm_impl<float, 2>(n, R, indptr, indices, (const float*)values, (const float*)X, ldx, (float*)Y, ldy, st);
+  return csr_spmm_impl<float, 1>(n, R, indptr, indices, (const float*)values, (const float*)X, ldx, (float*)Y, ldy, st);
 }
 
 int64_t polar_msolve_ws_bytes(const polar_op* op, int64_t R, int32_t p, int64_t nA, int32_t flags) {

@@ -22,9 +22,10 @@ import torch
 
 from . import _lib as L
 from .quadrature import composite, fixed_rule, poles  # noqa: F401  (re-exported API)
+from .spmm import csr_spmm, spmm  # noqa: F401  (re-exported API; Op's product is _apply below)
 
 __all__ = ['composite', 'fixed_rule', 'poles', 'Graph', 'Op', 'Counter', 'msolve', 'sketches',
-           'chunk_width', 'polar', 'degree_greedy', 'random_set', 'true_marginals', 'spmm',
+           'chunk_width', 'polar', 'degree_greedy', 'random_set', 'true_marginals', 'spmm', 'csr_spmm',
            'RademacherRows', 'DeviceSketch']
 
 
@@ -43,12 +44,17 @@ def _dt(precision):
 
 # ------------------------------------------------------------------------------------------------
 class LaplacianCSR:
-    """M0 = diag(deg + sigma) - A held as diagonal + off-diagonal adjacency (polar_gpu.py:52-55).
-    `.get()` returns the full host CSR (with the diagonal), as the reference's cupyx matrix does."""
+    """M0 = diag(deg + sigma) - A (polar_gpu.py:52-55).  The solver uses the graph's value-free
+    adjacency + diagonal; the attributes of the reference's cupyx csr_matrix are also here:
+    `.indptr`, `.indices` (int32) and `.data` (float64) are device tensors of the full CSR with the
+    diagonal stored (built on first use), `.shape`, `.nnz`, `.dtype`, `.get()` (host SciPy CSR)
+    and `A @ X` (the Laplacian operator kernel)."""
 
     def __init__(self, G):
         self._G = G
         self.shape = (G.n, G.n)
+        self.dtype = np.dtype(np.float64)
+        self._csr = None
 
     def get(self):
         G = self._G
@@ -56,6 +62,34 @@ class LaplacianCSR:
         M = (sp.diags(G.deg + G.sigma) - A).tocsr()
         M.sort_indices()
         return M
+
+    def _device_csr(self):
+        if self._csr is None:
+            M = self.get()
+            dev = _device()
+            self._csr = (torch.from_numpy(M.indptr.astype(np.int32)).to(dev),
+                         torch.from_numpy(M.indices.astype(np.int32)).to(dev),
+                         torch.from_numpy(M.data.astype(np.float64)).to(dev))
+        return self._csr
+
+    @property
+    def indptr(self):
+        return self._device_csr()[0]
+
+    @property
+    def indices(self):
+        return self._device_csr()[1]
+
+    @property
+    def data(self):
+        return self._device_csr()[2]
+
+    @property
+    def nnz(self):
+        return self._G.n + self._G.nnz
+
+    def __matmul__(self, X):
+        return spmm(self, X)
 
 
 class Graph:
@@ -71,6 +105,9 @@ class Graph:
         A = sp.coo_matrix((np.r_[w, w], (np.r_[u, v], np.r_[v, u])), shape=(self.n, self.n)).tocsr()
         A.sum_duplicates()
         A.sort_indices()
+        # simple graph: no repeated (unordered) edge and no self-loop, i.e. every input edge kept
+        # its own two adjacency entries
+        self.simple = bool(A.nnz == 2 * len(u))
         if A.nnz >= 2 ** 31:
             raise ValueError('graphs with >= 2^31 adjacency entries need int64 indices')
         self._indptr = A.indptr.astype(np.int32)
@@ -228,14 +265,6 @@ def _apply(op, x, shift):
     return Y.reshape(-1) if one else Y
 
 
-def spmm(A, X):
-    """Y = M0 X (realworld/spmm.py:18-25); A is a Graph, its M0, or an Op."""
-    if isinstance(A, LaplacianCSR):
-        A = A._G
-    op = A if isinstance(A, Op) else Op(A, precision=torch.as_tensor(X).dtype)
-    return _apply(op, X, 0.0)
-
-
 class Counter:
     """polar_gpu.py:83-86 (plus `solves`: column-iterations of each msolve call)."""
 
@@ -252,6 +281,23 @@ def _max_cols(dtype):
 def _blocked(block, dtype):
     """Blocked shifted update: on by default for FP32, off for FP64 (the pinned path)."""
     return (dtype == torch.float32) if block is None else bool(block)
+
+
+def _replacing(replace, block, dtype):
+    """FP64 residual replacement in the greedy's solves: on by default for the FP32-mixed form (with
+    its blocked update, where the seed iterate advances once per window), off for FP64."""
+    if replace is None:
+        return dtype == torch.float32 and _blocked(block, dtype)
+    return bool(replace)
+
+
+def _col_bytes(n, p, nA, dtype, block=None, replace=False):
+    """Device bytes per solved column: the n-row CG vectors (with the residual ring of the blocked
+    update), the FP64 seed iterate and saved direction of residual replacement, and the shifted P
+    block."""
+    esz = 8 if dtype == torch.float64 else 4
+    nvec = 11 if _blocked(block, dtype) else 4
+    return esz * (nvec * n + p * nA) + ((8 + esz) * n if replace else 0) + 64
 
 
 def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, freeze=True,
@@ -401,6 +447,11 @@ def sketches(G, cand, q, seed, kind='block', chunk=4_000_000, on_device=False, p
     n = G.n
     if kind == 'rademacher':
         scale = 1.0 / math.sqrt(q)
+        if on_device and not G.simple:
+            # the host form adds one hashed row per input edge; the device kernel hashes each merged
+            # adjacency entry once, which differs for a repeated edge or a self-loop
+            raise ValueError('the device Rademacher sketch needs a simple graph (no repeated edge, no '
+                             'self-loop); use on_device=False for a multigraph')
         if on_device == 'lazy':
             return DeviceSketch(G, seed, q, len(cand))
         if on_device:
@@ -453,21 +504,21 @@ def chunk_width(n, p, nA, budget=10e9):
     return int(max(8, min(cap, budget // per_col)))
 
 
-def bank_width(n, p, nA, q, dtype, budget=None, block=None):
+def bank_width(n, p, nA, q, dtype, budget=None, block=None, replace=False):
     """Columns per bank block solve on one MI355X: as wide as one wave covers (128 FP64 / 256
     FP32 columns) within a memory budget for the shifted P block and the CG vectors."""
-    return fit_width(n, p, nA, 2 * q, dtype, budget, block)
+    return fit_width(n, p, nA, 2 * q, dtype, budget, block, replace)
 
 
-def fit_width(n, p, nA, R, dtype, budget=None, block=None):
-    """Columns (at most R) whose msolve workspace fits in `budget` bytes (default: half of the
-    free device memory): the n-row CG vectors (with the residual ring of the blocked update) and
-    the shifted P block per column."""
+def fit_width(n, p, nA, R, dtype, budget=None, block=None, replace=False):
+    """Columns (at most R) whose msolve workspace fits in `budget` bytes (default: 70% of the
+    free device memory): per column the n-row CG vectors (with the residual ring of the blocked
+    update and the vectors of residual replacement) and the shifted P block."""
     esz = 8 if dtype == torch.float64 else 4
     if budget is None:
         free, _ = torch.cuda.mem_get_info()
         budget = 0.7 * free
-    per_col = esz * ((11 if _blocked(block, dtype) else 4) * n + p * nA) + 64
+    per_col = _col_bytes(n, p, nA, dtype, block, replace)
     w = int(min(R, _max_cols(dtype), max(1, budget // per_col)))
     # equal chunks (e.g. 2 x 64, not 120 + 8), multiples of 8 where possible
     chunks = -(-R // w)
@@ -509,7 +560,7 @@ class _Phases:
 def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=None, naive=False,
           seed=2026, kind='block', trace=None, time_limit=None, poles=None, width=None,
           freeze=True, precision='fp64', profile=False, progress=None, graph=None, tune=None, block=None,
-          shortlist='proxy', replace=False):
+          shortlist='proxy', replace=None):
     """POLAR greedy, cache backend (polar_gpu.py:180-254), on the GPU.
 
     profile=True adds `phases`: seconds per phase (bank, select, rerank_solve, rerank_pick,
@@ -519,16 +570,23 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     the timed region).  block: blocked shifted update in every solve (see msolve; None = FP32 on,
     FP64 off).  shortlist='gradient' ranks candidates by the first-order gain sum_j om_j h[j,e]
     (minus the sketched trace's derivative, see polar_amd.adjoint) instead of the reference's
-    proxy; the re-rank by exact marginals is unchanged.  replace=True: FP64 residual replacement
-    every 24 iterations in every solve (see msolve).
+    proxy; the re-rank by exact marginals is unchanged.  replace: FP64 residual replacement every
+    24 iterations in every solve (see msolve); None = on for FP32 (the FP32-mixed form), off for FP64.
+    trace: a list that receives per round dict(shortlist, chosen) as in the reference, plus `proxy`
+    (the shortlist's proxy scores) and `scores` (its re-rank scores; empty for naive).
     progress: optional callable(stage, info) called after the bank ('bank', {seconds, spmv}) and
     after each round ('round', {round, edge, marginal, seconds, spmv}); it reads host values only.
 
     Returns dict(sequence, seconds, spmv, rhs, p, rounds, truncated, marginals, trace_estimate,
-    solves, precision)."""
+    solves, precision, proxy_gap, rerank_gap, replace).  proxy_gap[r] / rerank_gap[r]: the relative
+    gap (top1 - top2) / |top1| between the two best proxy scores and between the two best re-rank
+    scores of round r (the reference's picks are the first of a stable argsort, :221, and the
+    first argmax, :236, so a gap at the CG-tolerance level marks a near-tie); NaN where fewer than
+    two candidates (or shortlisted candidates) compete."""
     lib = L.require_gpu()
     dev = _device()
     dtype = _dt(precision)
+    replace = _replacing(replace, block, dtype)
     dcode = L.dtype_code(dtype)
     stream = L.stream_ptr()
     n, N = G.n, len(cand)
@@ -562,8 +620,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     # are independent, so the split does not change any result.
     esz = 8 if dtype == torch.float64 else 4
     free = torch.cuda.mem_get_info()[0]
-    nvec = 11 if _blocked(block, dtype) else 4
-    joint = (p * nA * 2 * q + 2 * n * q + 2 * q * (nvec * n + p * nA)) * esz < 0.8 * free
+    joint = (p * nA * 2 * q + 2 * n * q) * esz + 2 * q * _col_bytes(n, p, nA, dtype, block, replace) < 0.8 * free
     if joint:
         RHS = torch.empty((n, 2 * q), dtype=dtype, device=dev)
         Zb, Fb = RHS[:, :q], RHS[:, q:]
@@ -599,6 +656,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     score = torch.empty(N, dtype=torch.float64, device=dev)
     sel_ws = torch.empty(lib.polar_select_ws_bytes(N), dtype=torch.uint8, device=dev)
     sel_out = torch.empty(64, dtype=torch.int32, device=dev)
+    gaps_d = torch.full((max(k, 1), 4), float('nan'), dtype=torch.float64, device=dev)   # top-2 proxy, top-2 re-rank
     coef = torch.empty((p, 2 * q + 3), dtype=torch.float64, device=dev)
     wX = wY = wU = None                 # maintenance buffers, allocated after the bank solves
     gvec = torch.empty(max(B, 1) + 2, dtype=torch.float64, device=dev)
@@ -610,7 +668,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
 
     XY = torch.empty((p, nA, 2 * q), dtype=dtype, device=dev)
     if width is None:
-        width = bank_width(n, p, nA, q, dtype, block=block)      # sized on what is free after the caches
+        width = bank_width(n, p, nA, q, dtype, block=block, replace=replace)   # sized on what is free after the caches
 
     if joint:
         blocks = [(lambda: RHS, 0, 2 * q)]
@@ -654,10 +712,14 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             ph.mark('bank')
         L.check(lib.polar_proxy(3 if shortlist == 'gradient' else objc, L.ptr(h), L.ptr(t), L.ptr(tau_d),
                                 L.ptr(om_d), p, N, sigma, L.ptr(dead), L.ptr(score), stream), 'proxy')
-        Br = 1 if naive else min(B, int(live.sum()))
-        short = np.zeros(Br, dtype=np.int32)
-        L.check(lib.polar_select_topb(L.ptr(score), N, Br, L.ptr(sel_out), short.ctypes.data_as(L.vp),
+        nlive = int(live.sum())
+        Br = 1 if naive else min(B, nlive)
+        Bsel = min(max(Br, 2), nlive)        # at least the top two, for the proxy gap record
+        short = np.zeros(Bsel, dtype=np.int32)
+        L.check(lib.polar_select_topb(L.ptr(score), N, Bsel, L.ptr(sel_out), short.ctypes.data_as(L.vp),
                                       L.ptr(sel_ws), sel_ws.numel(), stream), 'select_topb')   # :221
+        short = short[:Br]
+        gaps_d[rnd, :min(Bsel, 2)] = score[sel_out[:min(Bsel, 2)].long()]
         if naive:
             e = int(short[0])
             marg.append(float(score[e].item()))
@@ -686,8 +748,12 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             e = int(short[j])
             u = U[:, :, j]
             marg.append(float(val[0]))
+            if Br >= 2:
+                gaps_d[rnd, 2:4] = torch.topk(gvec[:Br], 2).values
         if trace is not None:
-            trace.append(dict(shortlist=[int(x) for x in short], chosen=e))
+            trace.append(dict(shortlist=[int(x) for x in short], chosen=e,
+                              proxy=score[torch.from_numpy(short.astype(np.int64)).to(dev)].cpu().tolist(),
+                              scores=[] if naive else gvec[:Br].cpu().tolist()))
         seq.append(e)
         live[e] = False
         dead[e] = 1
@@ -727,9 +793,14 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
         for dn in tr_hist:
             dn_h = dn.cpu().numpy()
             traces.append(traces[-1] - math.fsum(om * dn_h[:, 1] / dn_h[:, 0]))
+    gh = gaps_d[:len(seq)].cpu().numpy()
+    with np.errstate(invalid='ignore', divide='ignore'):
+        pgap = (gh[:, 0] - gh[:, 1]) / np.abs(gh[:, 0])
+        rgap = (gh[:, 2] - gh[:, 3]) / np.abs(gh[:, 2])
     return dict(sequence=seq, seconds=seconds, spmv=cnt.spmv, rhs=cnt.rhs, p=p, rounds=len(seq),
                 truncated=len(seq) < k, marginals=marg, trace_estimate=traces[:max(len(seq), 1)],
                 solves=list(cnt.solves), precision='fp64' if dtype == torch.float64 else 'fp32',
+                proxy_gap=pgap.tolist(), rerank_gap=rgap.tolist(), replace=replace,
                 **({'phases': ph.acc} if profile else {}))
 
 

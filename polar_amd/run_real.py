@@ -30,6 +30,7 @@ OUT = Path.cwd() / 'results'
 SUF = ''
 SIGMA = 1.0
 POOL = 'uniform'
+POOL_GEN = 'reference'
 PSEED = 0
 
 
@@ -41,7 +42,14 @@ def env():
 
 
 def setup(name, N, seed_offset=None, sigma=None):
-    """run_real.py:32-45; synthetic config names build their graph and pool with configs.make."""
+    """run_real.py:32-45; synthetic config names build their graph and pool with configs.make.
+
+    N=None: the reference's default of 2000 candidates for a named graph, the configuration's own
+    pool for a synthetic one; N=0: every non-edge (named graph) or the configuration's pool; N > 0:
+    that many (a synthetic pool keeps its first N).  Named graphs draw the uniform pool with the
+    reference's generator (graphs.candidates, realworld/graphs.py:61-69) unless POOL_GEN is
+    'fast' (graphs.candidates_fast: same distribution, vectorised, its own draw order); meta
+    records which."""
     sigma = SIGMA if sigma is None else sigma
     seed_offset = PSEED if seed_offset is None else seed_offset
     if name in configs.SPECS or name.startswith('c4_'):
@@ -50,18 +58,23 @@ def setup(name, N, seed_offset=None, sigma=None):
         G = pg.Graph(n, E, sigma, weights=c['weights'])
         cand = c['cand'] if not N else c['cand'][:N]
         meta = dict(name=name, source='synthetic', n=int(n), m=int(len(E)),
-                    preprocessing='generated (configs.py): undirected, deduplicated, no self-loops')
+                    preprocessing='generated (configs.py): undirected, deduplicated, no self-loops',
+                    pool_generator='configs.make')
         seed = c['seed']
     else:
         n, E, meta = graphs.load(name)
         G = pg.Graph(n, E, sigma)
         seed = 20270000 + zlib.crc32(name.encode()) % 10000 + seed_offset
+        N = 2000 if N is None else N
         if not N:
-            cand = graphs.all_nonedges(n, E)
+            cand, gen = graphs.all_nonedges(n, E), 'all_nonedges'
         elif POOL == '2hop':
-            cand = graphs.candidates_2hop(n, E, N, seed)
+            cand, gen = graphs.candidates_2hop(n, E, N, seed), 'candidates_2hop'
+        elif POOL_GEN == 'fast':
+            cand, gen = graphs.candidates_fast(n, E, N, seed), 'candidates_fast'
         else:
-            cand = graphs.candidates(n, E, N, seed) if N <= 20000 else graphs.candidates_fast(n, E, N, seed)
+            cand, gen = graphs.candidates(n, E, N, seed), 'candidates'
+        meta['pool_generator'] = gen
     meta.update(N=len(cand), candidate_seed=seed, pool=POOL, dmax=G.dmax, sigma=sigma,
                 n_active=int(len(np.unique(cand))))
     return G, cand, meta
@@ -212,7 +225,8 @@ def parser():
     P = argparse.ArgumentParser()
     P.add_argument('mode', choices=['effect', 'time'])
     P.add_argument('--graphs', nargs='+', required=True)
-    P.add_argument('--N', type=int, default=2000)
+    P.add_argument('--N', type=int, default=None,
+                   help='candidates (default: 2000 for a named graph, the configuration pool for c1..c5)')
     P.add_argument('--k', type=int, default=20)
     P.add_argument('--q', type=int, default=128)
     P.add_argument('--B', type=int, default=16)
@@ -227,6 +241,8 @@ def parser():
     P.add_argument('--skip-naive', action='store_true')
     P.add_argument('--cross-only', action='store_true', help='evaluate Random only under the fractional targets')
     P.add_argument('--pool', choices=['uniform', '2hop'], default='uniform')
+    P.add_argument('--pool-gen', choices=['reference', 'fast'], default='reference',
+                   help='uniform pool generator of a named graph (fast: vectorised, its own draw order)')
     P.add_argument('--slq', action='store_true')
     P.add_argument('--slq-probes', type=int, default=20)
     P.add_argument('--slq-steps', type=int, default=40)
@@ -247,9 +263,10 @@ def parser():
 
 
 def main(argv=None):
-    global POOL, PSEED, SUF, SIGMA
+    global POOL, POOL_GEN, PSEED, SUF, SIGMA
     a = parser().parse_args(argv)
     POOL = a.pool
+    POOL_GEN = a.pool_gen
     PSEED = a.pool_seed
     SIGMA = a.sigma
     SUF = ('' if a.pool == 'uniform' else '_2hop') + (f'_ps{a.pool_seed}' if a.pool_seed else '') + (f'_{a.tag}' if a.tag else '')

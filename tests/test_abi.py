@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, s), s
     assert set(declared_symbols()) == set(_lib.EXPORTS)
     loaded = _lib.load()
-    assert loaded.polar_abi_version() == _lib.ABI_VERSION == 4
+    assert loaded.polar_abi_version() == _lib.ABI_VERSION == 5
 
 
 def test_polar_op_struct_layout():
@@ -79,3 +79,19 @@ def test_workspace_queries_through_ctypes():
     s.dtype = 7
     assert lib.polar_spmm_ws_bytes(s, 16) == -1
     assert lib.polar_select_ws_bytes(10 ** 6) > 0
+
+
+def test_csr_spmm_and_release_host_checks():
+    """polar_csr_spmm checks its arguments on the host before any launch (no GPU needed), and
+    polar_release with nothing allocated is a no-op."""
+    from polar_amd import _lib as L
+    if not L.LIB_PATH.exists():
+        pytest.skip('libpolar_amd.so not built')
+    lib = L.load()
+    dummy = ctypes.c_void_p(256)
+    assert lib.polar_csr_spmm(7, 10, 4, dummy, dummy, dummy, dummy, 4, dummy, 4, None) == L.POLAR_E_ARG
+    assert lib.polar_csr_spmm(L.POLAR_F64, 10, 4, dummy, dummy, dummy, dummy, 3, dummy, 4, None) == L.POLAR_E_ARG
+    assert lib.polar_csr_spmm(L.POLAR_F32, 10, 0, dummy, dummy, dummy, dummy, 4, dummy, 4, None) == L.POLAR_E_ARG
+    assert lib.polar_csr_spmm(L.POLAR_F64, 10, 4, None, dummy, dummy, dummy, 4, dummy, 4, None) == L.POLAR_E_ARG
+    assert b'csr_spmm' in lib.polar_last_error()
+    assert lib.polar_release() == 0

@@ -107,6 +107,35 @@ def test_residual_replacement_fp32(block):
     assert np.all(itr >= 0) and itr.max() < 5 * max(itp.max(), 1)
 
 
+@pytest.mark.parametrize('precision,block', [('fp64', False), ('fp64', True), ('fp32', True)])
+def test_residual_replacement_after_compaction(precision, block):
+    """Replacements after an active-column compaction: the even columns are zero, so 8 of 16 slots
+    stay active and the solver compacts at its first check; tol 1e-10 keeps it iterating well past
+    iteration 24, so every replacement reads the right-hand side through the slot -> column map.
+    FP64: X with replacement equals X without it within 1e-8 relative (per-iteration and blocked
+    seed iterate).  FP32 (blocked): both within 1e-4 of the FP64 solution, the replaced one no
+    worse than twice the plain one."""
+    G, W, tau, act = _problem(R=16)
+    W[:, 0::2] = 0.0
+    W[:, 5] *= 1e-3                              # columns of very different scales
+    dt = torch.float64 if precision == 'fp64' else torch.float32
+    op = pg.Op(G, precision=precision)
+    Wd = torch.from_numpy(W).cuda().to(dt)
+    Xp, _, itp = pg.msolve(op, Wd, tau, act, 1e-10, block=block)
+    Xr, _, itr = pg.msolve(op, Wd, tau, act, 1e-10, block=block, replace=True)
+    assert np.all(itp[0::2] == 0) and np.all(itr[0::2] == 0) and itp.max() > 48
+    cols = [c for c in range(1, 16, 2) if np.any(W[:, c])]       # column 3 of the fixture is zero
+    assert len(cols) == 7
+    if precision == 'fp64':
+        for c in cols:
+            assert _rel(Xr[:, :, c], Xp[:, :, c]) <= 1e-8, c
+    else:
+        X64 = pg.msolve(pg.Op(G), torch.from_numpy(W).cuda(), tau, act, 1e-11)[0]
+        for c in cols:
+            ep, er = _rel(Xp[:, :, c].double(), X64[:, :, c]), _rel(Xr[:, :, c].double(), X64[:, :, c])
+            assert er <= 1e-4 and er <= 2 * ep + 1e-7, (c, ep, er)
+
+
 def test_residual_replacement_fp64_matches_pinned_within_tolerance():
     G, W, tau, act = _problem(R=8)
     op = pg.Op(G)

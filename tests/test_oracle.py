@@ -9,7 +9,7 @@ import pytest
 import scipy.special as ss
 
 from oracle import polar_oracle as O
-from polar_amd import graphs, quadrature
+from polar_amd import configs, graphs, quadrature
 import scalability_inputs as SI
 
 GOLD = Path(__file__).resolve().parent / 'golden'
@@ -383,3 +383,22 @@ def test_er_logdet_marginals_match_eigenvalues(obj):
     r = O.polar(G, cand, 4, 0.5, q=16, B=4, tol=1e-10, obj=obj, kind='rademacher', seed=5)
     ex = np.diff(np.r_[0.0, O.eval_prefix(G.M0.toarray(), cand, r['sequence'], 0.5, obj=obj)])
     assert np.max(np.abs(np.array(r['marginals']) - ex) / np.abs(ex)) < 1e-8
+
+
+def test_polar_column_parallel_equals_serial():
+    """The CPU baseline splits every block solve's columns over forked processes; columns are
+    independent, so sequence and counters equal the one-process run; with the pinned arithmetic
+    (order-independent reductions) the marginals and trace estimates are bit-identical, with NumPy's
+    native reductions (whose order follows the block shape) within 1e-12."""
+    c = configs.make('c2', 0.01)
+    Go = O.Graph(c['n'], c['edges'], c['sigma'])
+    sk = O.sketches(Go, c['cand'], 16, c['seed'], kind='rademacher')
+    kw = dict(q=16, B=4, tol=c['tol'], sketch=sk, poles=(c['tau'], c['om']), width=32)
+    for arith, bound in (('cr', 0.0), ('numpy', 1e-12)):
+        a = O.polar(Go, c['cand'], 4, c['s'], arith=arith, **kw)
+        b = O.polar(Go, c['cand'], 4, c['s'], arith=arith, workers=3, **kw)
+        assert a['sequence'] == b['sequence'] and (a['spmv'], a['rhs']) == (b['spmv'], b['rhs'])
+        assert a['solves'] == b['solves']
+        for key in ('marginals', 'trace_estimate'):
+            x, y = np.asarray(a[key]), np.asarray(b[key])
+            assert np.max(np.abs(x - y) / np.abs(x)) <= bound, (arith, key)

@@ -50,3 +50,21 @@ def test_sketch_properties(kind):
         np.add.at(deg_w, e[:, 0], np.sqrt(w))
         np.add.at(deg_w, e[:, 1], np.sqrt(w))
         assert np.all(np.abs(F) <= deg_w[:, None] / np.sqrt(q) + 1e-12)
+
+
+def test_device_rademacher_sketch_rejects_multigraphs():
+    """The device Rademacher sketch hashes each merged adjacency entry once; the host form adds one
+    row per input edge.  They agree on simple graphs only, so a repeated edge or a self-loop is
+    refused on the device path (before any GPU call) and the host path still accepts it."""
+    from polar_amd import polar_gpu as pg
+    e = np.array([[0, 1], [1, 2], [2, 3]])
+    cand = np.array([[0, 2], [1, 3]])
+    assert pg.Graph(4, e, 0.1).simple
+    for bad in (np.r_[e, [[1, 0]]], np.r_[e, [[2, 2]]]):
+        G = pg.Graph(4, bad, 0.1)
+        assert not G.simple
+        for mode in (True, 'lazy'):
+            with pytest.raises(ValueError, match='simple graph'):
+                pg.sketches(G, cand, 8, 3, kind='rademacher', on_device=mode)
+        Z, F, Gc = pg.sketches(G, cand, 8, 3, kind='rademacher')
+        assert F.shape == (4, 8)
