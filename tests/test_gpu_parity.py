@@ -77,18 +77,19 @@ def test_spmm_dropin_no_patch():
 
 
 def _general_csr(n=2500, m=1800, seed=3):
-    """A rectangular, non-Laplacian CSR (n x m, signed random values): empty rows, a row with 1,500
-    entries (longer than any chunk), duplicate-free sorted columns."""
+    """A non-Laplacian CSR (n x m, signed random values): empty rows, a row with min(1500, m) entries
+    (longer than any chunk), duplicate-free sorted columns."""
     rng = np.random.default_rng(seed)
     A = sp.random(n, m, density=0.004, random_state=seed, format='lil')
-    A[10, :1500] = rng.standard_normal(1500)
+    long = min(1500, m)
+    A[10, :long] = rng.standard_normal(long)
     A = A.tocsr()
     A.data = rng.uniform(-2.0, 3.0, A.nnz)
     for r in (0, 5, n - 1):
         A.data[A.indptr[r]:A.indptr[r + 1]] = 0.0
     A.eliminate_zeros()
     A.sort_indices()
-    assert np.diff(A.indptr)[[0, 5, n - 1]].sum() == 0 and A.indptr[11] - A.indptr[10] >= 1500
+    assert np.diff(A.indptr)[[0, 5, n - 1]].sum() == 0 and A.indptr[11] - A.indptr[10] >= long
     return A
 
 
