@@ -133,6 +133,17 @@ int polar_release(void);
 int polar_prof_set(int32_t on);
 int polar_prof_get(int32_t cls, double* ms, int64_t* launches, double* bytes);
 
+/* Zero-allocation form of polar_spmm, for callers that cannot allocate device memory or pass a
+ * workspace (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same product and kernels as polar_spmm; the long-row chunk
+ * partials live in a file-scope device array of polar_spmm_static_max_bytes() bytes (32 MiB) that
+ * each GPU holds until the process ends.  Rejects (POLAR_E_ARG) an operator and width whose
+ * partials need more (polar_spmm_ws_bytes(op, R) > the maximum).  Calls no allocator.  Calls that
+ * overlap on one GPU (other streams or threads) share the array and race: serialise them. */
+int64_t polar_spmm_static_max_bytes(void);
+int polar_spmm_static(const polar_op* op, double shift, const void* X, int64_t ldx, void* Y, int64_t ldy,
+                      int64_t R, void* stream);
+
 /* General CSR product Y[:, :R] = A X[:, :R] (realworld/spmm.py:3-14, csr_spmm(n, R, ptr, idx, val,
  * X, Y)): A is n rows of CSR with int32 indptr[n+1] and indices[nnz] and values[nnz] of `dtype`
  * (any values, diagonal stored like the reference's G.M0), X and Y row-major with leading dims

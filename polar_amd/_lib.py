@@ -24,6 +24,7 @@ EXPORTS = [
     'polar_maint_prep', 'polar_maint_rank1', 'polar_maint_cache', 'polar_rows_axpy',
     'polar_rademacher', 'polar_edge_sketch', 'polar_prof_set', 'polar_prof_get', 'polar_pair_wsum',
     'polar_msolve_batched_ws_bytes', 'polar_msolve_batched', 'polar_csr_spmm', 'polar_release',
+    'polar_spmm_static', 'polar_spmm_static_max_bytes',
 ]
 
 vp = ctypes.c_void_p
@@ -60,6 +61,8 @@ _SIGS = {
     'polar_spmm': (i32, [OPP, f64, vp, i64, vp, i64, i64, vp, i64, vp]),
     'polar_csr_spmm': (i32, [i32, i64, i64, vp, vp, vp, vp, i64, vp, i64, vp]),
     'polar_release': (i32, []),
+    'polar_spmm_static': (i32, [OPP, f64, vp, i64, vp, i64, i64, vp]),
+    'polar_spmm_static_max_bytes': (i64, []),
     'polar_msolve_ws_bytes': (i64, [OPP, i64, i32, i64, i32]),
     'polar_msolve': (i32, [OPP, vp, i64, i64, vp, i32, vp, i64, f64, i64, i32, vp, i64, i64, vp, vp,
                            vp, vp, i64, vp]),

@@ -15,6 +15,7 @@
 //    convergence no-ops, so the host reads it only every 8 iterations.
 #include "common.hpp"
 
+#include <mutex>
 #include <vector>
 
 namespace polar {
@@ -2166,6 +2167,33 @@ static int msolve_dispatch(const polar_op* op, const Segs& sg, const void* W, in
                         h_out, te_out, iters_out, ws, ws_bytes, st);
 }
 
+// ---------------------------------------------------------------- zero-allocation SpMM
+// polar_spmm_static keeps the long-row chunk partials in this file-scope device array instead of a
+// caller workspace.  Every GPU the library runs on holds its own copy until the process ends.  In
+// host code the array's name is not a device address: its address on the current device is looked
+// up once per device (hipGetSymbolAddress, with that device current) and only that address is
+// passed to kernels.  Calls that overlap on one GPU share the array and race.
+constexpr int64_t kScratchBytes = int64_t(32) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_address(void** out) {
+  int dev = 0;
+  POLAR_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(POLAR_E_ARG, "device %d beyond the %d supported", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    POLAR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_error(POLAR_E_ARG, "no device address for the static scratch on device %d", dev);
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace polar
 
 using namespace polar;
@@ -2227,6 +2255,23 @@ int polar_spmm(const polar_op* op, double shift, const void* X, int64_t ldx, voi
   if (cpl == 4) return spmm_impl<float, 4>(op, shift, (const float*)X, ldx, (float*)Y, ldy, R, ws, ws_bytes, st);
   if (cpl == 2) return spmm_impl<float, 2>(op, shift, (const float*)X, ldx, (float*)Y, ldy, R, ws, ws_bytes, st);
   return spmm_impl<float, 1>(op, shift, (const float*)X, ldx, (float*)Y, ldy, R, ws, ws_bytes, st);
+}
+
+int64_t polar_spmm_static_max_bytes(void) { return kScratchBytes; }
+
+int polar_spmm_static(const polar_op* op, double shift, const void* X, int64_t ldx, void* Y, int64_t ldy,
+                      int64_t R, void* stream) {
+  if (int rc = check_op(op)) return rc;
+  if (R <= 0 || ldx < R || ldy < R) return set_error(POLAR_E_ARG, "bad R/ld");
+  const int64_t need = polar_spmm_ws_bytes(op, R);
+  if (need > kScratchBytes)
+    return set_error(POLAR_E_ARG,
+                     "polar_spmm_static: %lld scratch bytes needed (%d long-row chunks x R=%lld) exceed the static "
+                     "maximum of %lld; call polar_spmm with a workspace",
+                     (long long)need, op->n_partial, (long long)R, (long long)kScratchBytes);
+  void* ws = nullptr;
+  if (int rc = scratch_address(&ws)) return rc;
+  return polar_spmm(op, shift, X, ldx, Y, ldy, R, ws, kScratchBytes, stream);
 }
 
 int polar_csr_spmm(int32_t dtype, int64_t n, int64_t R, const int32_t* indptr, const int32_t* indices,
