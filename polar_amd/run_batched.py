@@ -30,15 +30,15 @@ def _make(args):
 
 
 def build_inputs(count, scale, workers=None):
-    """Host inputs of graphs 0..count-1 (generated in worker processes, before any GPU use)."""
+    """Host inputs of graphs 0..count-1, generated in worker processes (forked before any GPU use;
+    started as fresh interpreters once this process has initialised the GPU runtime)."""
     workers = workers or min(16, os.cpu_count() or 1, count)
     torch = sys.modules.get('torch')
-    if torch is not None and torch.cuda.is_initialized():
-        workers = 1                            # no fork once the GPU runtime is up
+    ctx = 'spawn' if torch is not None and torch.cuda.is_initialized() else 'fork'
     jobs = [(i, scale) for i in range(count)]
     if workers <= 1:
         return [_make(j) for j in jobs]
-    with mp.get_context('fork').Pool(workers) as pool:
+    with mp.get_context(ctx).Pool(workers) as pool:
         return pool.map(_make, jobs, chunksize=1)
 
 

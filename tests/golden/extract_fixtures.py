@@ -10,6 +10,11 @@
   `residual_rho`, `p`.  These pin the composite rule's pole count and smallest node and the
   `rhs` formulas without the (absent) SNAP datasets.
 
+* result_schema.json -- the key trees (keys, nesting, value types; lists by their first element)
+  of three of the reference's stored result files: time_com-Amazon.json (time mode),
+  effect_ca-GrQc.json (effect mode, dense eigenvalue evaluator) and effect_com-Youtube.json
+  (effect mode, certified quadrature evaluator).  run_real's output must contain every path.
+
 Run where the reference is mounted:  python tests/golden/extract_fixtures.py /root/reference
 """
 import glob
@@ -49,6 +54,18 @@ def main(ref_root):
                                       k=e['params']['k'], target=key, quadrature_bound=ev['quadrature_bound'],
                                       residual_rho=ev['residual_rho'], p=ev['p']))
     (here / 'realworld_meta.json').write_text(json.dumps(meta, indent=1))
+
+    def tree(x):
+        if isinstance(x, dict):
+            return {k: tree(v) for k, v in x.items()}
+        if isinstance(x, list):
+            return [tree(x[0])] if x else []
+        return type(x).__name__
+    schema = dict(source='realworld/results/{time_com-Amazon,effect_ca-GrQc,effect_com-Youtube}.json key trees')
+    for key, f in (('time', 'time_com-Amazon.json'), ('effect_dense', 'effect_ca-GrQc.json'),
+                   ('effect_certified', 'effect_com-Youtube.json')):
+        schema[key] = tree(json.loads((root / 'realworld' / 'results' / f).read_text()))
+    (here / 'result_schema.json').write_text(json.dumps(schema, indent=1))
     print('wrote fixtures to', here)
 
 

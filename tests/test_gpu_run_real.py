@@ -29,6 +29,9 @@ def test_time_and_effect_modes(tmp_path):
     assert g['_evaluator'] == {'evaluator': 'eigenvalues'} and len(g['POLAR(s=0.5)']) == 4
     assert all(x > 0 for x in g['POLAR(s=0.5)'])
     assert g['POLAR(s=0.5)'][-1] >= g['Random#0'][-1]
+    import result_schema as RS
+    assert RS.missing(RS.reference('time'), RS.tree(d)) == []
+    assert RS.missing(RS.reference('effect_dense'), RS.tree(e), skip=('Exact(s=*)', 'Exact-ER', 'Exact-Logdet')) == []
 
 
 def test_time_mode_gradient_shortlist(tmp_path):
