@@ -297,6 +297,10 @@ struct CGState {
   // window coefficients of pole 0 (the seed pole: shift 0, zeta = 1, so its aj, bj are the seed's
   // alpha, beta) over the basis {d at the window start, r_1..r_T}.
   double* x0;
+  // With the blocked update, x0 advances once per window: inside the window flush on the active rows
+  // (whose residuals it already holds) and in k_seed_flush on the other rows; amask[i] != 0 marks
+  // the active rows.
+  const uint8_t* amask;
   // Column groups (one segment): G groups, cgrp[slot] = group of the column in that slot.
   int G;
   int* cgrp;
@@ -536,7 +540,9 @@ __global__ __launch_bounds__(kBlock) void k_init_fin(CGState s, int nparts) {
 
 // ------------------------------------------------------------------------------------------------
 // Kernel A: Y = Op(X) for complete rows; long-row chunk partials to lpart; optional d.v partials.
-template <typename T, int CPL, bool WEIGHTED, bool DOT>
+// T: vectors; TW: the operator's stored values (diagonal, weights) -- T except for the residual
+// replacement product, which applies the FP32 operator to the FP64 seed iterate in FP64.
+template <typename T, int CPL, bool WEIGHTED, bool DOT, typename TW = T>
 __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, const T* __restrict__ X,
                                                   int64_t ldx, T* __restrict__ Y, int64_t ldy, int R,
                                                   int L, T* __restrict__ lpart, CGState s, int cur) {
@@ -602,9 +608,9 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
     double dgd = 0.0;
     if (lidx < 0 && colok) {
       vload<T, CPL>(X + (int64_t)row * ldx + c0, xv);
-      dgd = (double)((const T*)op.diag)[row];
+      dgd = (double)((const TW*)op.diag)[row];
     }
-    gather_acc<T, CPL, WEIGHTED>(op.indices, (const T*)op.weights, X, ldx, beg, end, g, colok, c0, acc);
+    gather_acc<T, CPL, WEIGHTED, TW>(op.indices, (const TW*)op.weights, X, ldx, beg, end, g, colok, c0, acc);
     if (lidx < 0) {
       int pidx = -1;
       if (op.n_patch > 0) {
@@ -631,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
 }
 
 // Kernel L: combine long-row chunk partials in chunk order, finish the rows, d.v partials.
-template <typename T, int CPL, bool DOT>
+template <typename T, int CPL, bool DOT, typename TW = T>
 __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, const T* __restrict__ X,
                                                     int64_t ldx, T* __restrict__ Y, int64_t ldy, int R,
                                                     int L, const T* __restrict__ lpart, CGState s,
@@ -662,7 +668,7 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
       // the row's own x and diagonal are loaded before its chunk sums (their latency overlaps)
       T xv[CPL];
       vload<T, CPL>(X + (int64_t)row * ldx + c0, xv);
-      const double dgd = (double)((const T*)op.diag)[row];
+      const double dgd = (double)((const TW*)op.diag)[row];
       T acc[CPL];
 #pragma unroll
       for (int e = 0; e < CPL; ++e) acc[e] = T(0);
@@ -900,7 +906,8 @@ template <typename T> struct Ring {
 template <typename T, int CPL>
 __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int L, int cw, Ring<T> ring,
                                                   const int32_t* __restrict__ active, T* __restrict__ P,
-                                                  T* __restrict__ Xo, int64_t sj, int64_t si, CGState s) {
+                                                  T* __restrict__ Xo, int64_t sj, int64_t si, CGState s,
+                                                  const T* __restrict__ dvec, T* __restrict__ dsave) {
   // column slice blockIdx.y: columns [cbase, cbase + Rs) (cw = L * CPL columns per slice)
   const int cbase = blockIdx.y * cw;
   const int Rs = min(cw, R - cbase);
@@ -915,10 +922,18 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
   constexpr int kMaxR = kFlushLanes * (16 / (int)sizeof(T));
   __shared__ T s_cx[2][(kW + 1) * kMaxR], s_cp[2][(kW + 1) * kMaxR];
   __shared__ int s_dirty[2][kMaxR];
+  // residual replacement: pole 0's window coefficients in FP64, for the seed iterate x0
+  __shared__ double s_c0[(kW + 1) * kMaxR];
+  const bool seed = dsave != nullptr;
   if (threadIdx.x == 0) s_nj = 0;
   __syncthreads();
   for (int j = threadIdx.x; j < p; j += kBlock)
     if (s.pdirty[j * s.S + sp.g]) s_jl[atomicAdd(&s_nj, 1)] = j;   // poles are independent: any order
+  if (seed)
+    for (int q = threadIdx.x; q < (kW + 1) * Rs; q += kBlock) {
+      const int t = q / Rs, c = q - t * Rs;
+      s_c0[q] = s.cX[(int64_t)t * s.sld + vb + cbase + c];
+    }
   __syncthreads();
   const int nj = s_nj;
   if (nj == 0) return;                           // uniform over the block
@@ -950,6 +965,26 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
         const int64_t ar = active[row[k]];
 #pragma unroll
         for (int t = 1; t <= kW; ++t) vload<T, CPL>(ring.s[t % kW] + ar * s.ld + c0, rr[k][t - 1]);
+        if (seed) {
+          // seed iterate on this active row: x0 += cX0 d(window start) + sum_t cXt r_t with pole 0's
+          // coefficients (the seed pole), the residuals already in registers; then the saved
+          // direction becomes d, the next window's start
+          T b0[CPL], dv[CPL];
+          double xv0[CPL];
+          vload<T, CPL>(dsave + ar * s.ld + c0, b0);
+          vload<T, CPL>(dvec + ar * s.ld + c0, dv);
+          double* xp = s.x0 + ar * s.ld + c0;
+          vload<double, CPL>(xp, xv0);
+#pragma unroll
+          for (int e = 0; e < CPL; ++e) {
+            double inc = s_c0[cl + e] * (double)b0[e];
+#pragma unroll
+            for (int t = 1; t <= kW; ++t) inc = inc + s_c0[t * Rs + cl + e] * (double)rr[k][t - 1][e];
+            xv0[e] = xv0[e] + inc;
+          }
+          vstore<double, CPL>(xp, xv0);
+          vstore<T, CPL>(dsave + ar * s.ld + c0, dv);
+        }
       }
     }
     __syncthreads();                             // the previous tile's last buffer is consumed
@@ -1025,20 +1060,18 @@ __global__ void k_coef_reset(CGState s) {
   if (nd) atomicAdd(&s.stats[2], nd);
 }
 
-// Residual replacement: r = w - (M + tau0) x0 on the rows of live segments, in FP64 from the
-// operator's own values (the product is the one the recurrences approximate), for active
-// columns; |r|^2 partials for k_replace_fin.  Each row is summed by one lane group, 8 FP64 x0
-// rows in flight per lane.  W is in the caller's column order: after active-column compaction
-// slot c holds the caller's column colmap[c].
-template <typename T, int CPL, bool WEIGHTED>
-__global__ __launch_bounds__(kBlock) void k_replace(polar_op op, double shift, const T* __restrict__ W,
-                                                    int64_t ldw, T* __restrict__ r, int R, int L, CGState s,
-                                                    int cur) {
+// Residual replacement, second step: r = w - y on the rows of live segments for active columns, where
+// y = (M + tau0) x0 was formed in FP64 by the operator kernel from the solve's own operator values;
+// |r|^2 partials for k_replace_fin.  W is in the caller's column order: after active-column
+// compaction slot c holds the caller's column colmap[c].
+template <typename T, int CPL>
+__global__ __launch_bounds__(kBlock) void k_replace_apply(int64_t n, const T* __restrict__ W, int64_t ldw,
+                                                          const double* __restrict__ y, T* __restrict__ r, int R,
+                                                          int L, CGState s, int cur) {
   const SegPart sp = seg_part(s.bR, s.S, blockIdx.x, gridDim.x);
   if (s.flag[cur] == 0 || !seg_live(s, cur, sp.g)) return;
-  if (s.S > 1) shift = s.tau[sp.g * s.p];
   int64_t r0, r1;
-  seg_range(s.seg_row, s.S, sp.g, op.n, r0, r1);
+  seg_range(s.seg_row, s.S, sp.g, n, r0, r1);
   const int vb = sp.g * s.ld;
   Geo g(L);
   const int c0 = g.li * CPL;
@@ -1046,66 +1079,36 @@ __global__ __launch_bounds__(kBlock) void k_replace(polar_op op, double shift, c
   DD dacc[CPL];
 #pragma unroll
   for (int e = 0; e < CPL; ++e) dacc[e] = DD{0.0, 0.0};
-  bool act[CPL];
-  int wcol[CPL], gcol[CPL];
-#pragma unroll
-  for (int e = 0; e < CPL; ++e) {
-    act[e] = colok && s.act[vb + c0 + e] != 0;
-    wcol[e] = !colok ? 0 : (s.ident ? c0 + e : s.colmap[c0 + e]);
-    gcol[e] = (colok && s.G > 1) ? s.cgrp[c0 + e] : 0;
-  }
-  const int gpb = kWaves * g.G;
-  int pc = -1;
-  // every lane of a group runs the same rows (the group shares the index loads)
-  for (int64_t i = r0 + (int64_t)sp.rank * gpb + g.wave * g.G + g.gi; i < r1; i += (int64_t)sp.nb * gpb) {
-    const int row = (int)i;
-    double xr[CPL];
-    double dgr = 0.0;
-    if (colok) {
-      vload<double, CPL>(s.x0 + i * s.ld + c0, xr);
-      dgr = (double)((const T*)op.diag)[row];
-    }
-    double acc[CPL];
-#pragma unroll
-    for (int e = 0; e < CPL; ++e) acc[e] = 0.0;
-    gather_acc<double, CPL, WEIGHTED, T>(op.indices, (const T*)op.weights, s.x0, s.ld, op.indptr[row],
-                                         op.indptr[row + 1], g, colok, c0, acc);
-    if (!colok) continue;
-    double y[CPL];
+  if (colok) {
+    bool act[CPL];
+    int wcol[CPL];
 #pragma unroll
     for (int e = 0; e < CPL; ++e) {
-      const double dg = dgr + (s.G > 1 ? s.tau[gcol[e] * s.p] : shift);
-      y[e] = dg * xr[e] - acc[e];
+      act[e] = s.act[vb + c0 + e] != 0;
+      wcol[e] = s.ident ? c0 + e : s.colmap[c0 + e];
     }
-    if (op.n_patch > 0) {
-      if (pc < 0) pc = lower_bound_i32(op.patch_rows, op.n_patch, row);
-      while (pc < op.n_patch && op.patch_rows[pc] < row) ++pc;
-      if (pc < op.n_patch && op.patch_rows[pc] == row) {
-        for (int q = op.patch_ptr[pc]; q < op.patch_ptr[pc + 1]; ++q) {
-          const double* xq = s.x0 + (int64_t)op.patch_cols[q] * s.ld + c0;
-          const int pg = op.patch_grp ? op.patch_grp[q] : -1;
+    const int gpb = kWaves * g.G;
+    for (int64_t i = r0 + (int64_t)sp.rank * gpb + g.wave * g.G + g.gi; i < r1; i += (int64_t)sp.nb * gpb) {
+      T rv[CPL];
+      double yv[CPL];
+      vload<T, CPL>(r + i * s.ld + c0, rv);
+      vload<double, CPL>(y + i * s.ld + c0, yv);
 #pragma unroll
-          for (int e = 0; e < CPL; ++e)
-            if (pg < 0 || pg == gcol[e]) y[e] = y[e] + (xr[e] - xq[e]);
-        }
-      }
+      for (int e = 0; e < CPL; ++e)
+        if (act[e]) rv[e] = (T)((double)W[i * ldw + wcol[e]] - yv[e]);
+      vstore<T, CPL>(r + i * s.ld + c0, rv);
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) dd_add(dacc[e], (double)rv[e] * (double)rv[e]);
     }
-    T rv[CPL];
-    vload<T, CPL>(r + i * s.ld + c0, rv);
-#pragma unroll
-    for (int e = 0; e < CPL; ++e)
-      if (act[e]) rv[e] = (T)((double)W[i * ldw + wcol[e]] - y[e]);
-    vstore<T, CPL>(r + i * s.ld + c0, rv);
-#pragma unroll
-    for (int e = 0; e < CPL; ++e) dd_add(dacc[e], (double)rv[e] * (double)rv[e]);
   }
   block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)blockIdx.x * 2 * R);
 }
 
-// Seed iterate of the blocked update, once per window (before the coefficients are reset):
+// Seed iterate of the blocked update on the rows outside the active set (the window flush advances
+// it on the active rows), once per window before the coefficients are reset:
 // x0 += cX0 dsave + sum_t cXt r_t with pole 0's window coefficients (pole 0 is the seed pole, so
 // its coefficients describe x0 and d in the window basis {d at the window start, r_1..r_T}); then
-// dsave = d, the next window's starting direction.  All rows of the live segments; FP64 x0.
+// dsave = d, the next window's starting direction.  FP64 x0.
 template <typename T, int CPL>
 __global__ __launch_bounds__(kBlock) void k_seed_flush(int64_t n, int R, int L, Ring<T> ring, const T* __restrict__ d,
                                                        T* __restrict__ dsave, CGState s) {
@@ -1127,6 +1130,7 @@ __global__ __launch_bounds__(kBlock) void k_seed_flush(int64_t n, int R, int L, 
     }
   const int gpb = kWaves * g.G;
   for (int64_t i = r0 + (int64_t)sp.rank * gpb + g.wave * g.G + g.gi; i < r1; i += (int64_t)sp.nb * gpb) {
+    if (s.amask[i]) continue;
     T dv[CPL];
     vload<T, CPL>(d + i * s.ld + c0, dv);
     if (any) {
@@ -1391,6 +1395,7 @@ struct SolveMem {
   int* perm;
   T *ring, *d, *v, *t2, *P, *lpart;   // ring: Tw residual vectors (Tw = 1: the residual itself)
   T* dsave;            // residual replacement with the blocked update: d at the window start
+  double *y64, *lpart64;   // residual replacement: the FP64 product and its long-row partials
   int* dtab;           // 5 (S+1) block prefix tables
   int64_t* dseg;       // 4 (S+1) segment ranges
 };
@@ -1424,7 +1429,13 @@ static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int
     s.pdirty = cv.take<int>((int64_t)p * S);
   }
   if (repl) s.x0 = cv.take<double>(op->n * R);
+  // blocked update with replacement: the active-row mask, and the saved direction of the rows
+  // outside the active set (when there are any)
+  s.amask = (repl && Tw > 1) ? cv.take<uint8_t>(align_up(op->n, 16)) : nullptr;
   m.dsave = (repl && Tw > 1) ? cv.take<T>(op->n * R) : nullptr;
+  // the replacement product (M + tau0) x0 in FP64, and its long-row chunk partials
+  m.y64 = repl ? cv.take<double>(op->n * R) : nullptr;
+  m.lpart64 = repl ? cv.take<double>((int64_t)op->n_partial * R) : nullptr;
   s.cgrp = cv.take<int>(R);
   s.colmap = cv.take<int>(R);
   s.sflag = cv.take<int>(2 * S);
@@ -1568,6 +1579,12 @@ __global__ void k_writeback(CGState s, int Rorig, double* __restrict__ h_out, do
   }
 }
 
+// mask[active[k]] = 1 (the mask is zeroed first)
+__global__ void k_mark(const int32_t* __restrict__ active, int64_t nA, uint8_t* __restrict__ mask) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nA; k += (int64_t)gridDim.x * blockDim.x)
+    mask[active[k]] = 1;
+}
+
 __global__ void k_iota(int* __restrict__ a, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = i;
 }
@@ -1628,30 +1645,42 @@ static int run_iter_any(int cpl, const polar_op* op, const Plan& pl, double tau0
   return run_iter<T, 1>(op, pl, tau0, rin, rout, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, wt, ev, st);
 }
 
-// Residual replacement on the current residual r (after a window's flush).
-template <typename T>
-static int replace_residual(int cpl, const polar_op* op, const Plan& pl, double tau0, const T* W, int64_t ldw, T* r,
-                            const CGState& s, int cur, hipStream_t st) {
-  const bool wt = op->weighted != 0;
-#define POLAR_REPL(C)                                                                                              \
-  if (wt)                                                                                                        \
-    hipLaunchKernelGGL((k_replace<T, C, true>), dim3(pl.grid_rows), dim3(kBlock), 0, st, *op, tau0, W, ldw, r, s.R, \
-                       pl.L, s, cur);                                                                            \
-  else                                                                                                           \
-    hipLaunchKernelGGL((k_replace<T, C, false>), dim3(pl.grid_rows), dim3(kBlock), 0, st, *op, tau0, W, ldw, r, s.R, \
-                       pl.L, s, cur);
-  if (cpl == 2) {
-    POLAR_REPL(2)
-  } else if (cpl == 4) {
-    if constexpr (sizeof(T) == 4) { POLAR_REPL(4) }
-  } else {
-    POLAR_REPL(1)
+// Residual replacement on the current residual r (after a window's flush): y = (M + tau0) x0 by the
+// operator kernel in FP64 on the solve's operator values (long rows in chunks, like every product of
+// the solve), then r = w - y on the active columns and their |r|^2.
+template <typename T, int C>
+static int replace_residual_c(const polar_op* op, const Plan& pl, double tau0, const T* W, int64_t ldw, T* r,
+                              double* y64, double* lpart64, const CGState& s, int cur, hipStream_t st) {
+  const int R = s.R;
+  const double* x0 = s.x0;
+  if (op->weighted)
+    hipLaunchKernelGGL((k_opmul<double, C, true, false, T>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, tau0, x0,
+                       (int64_t)s.ld, y64, (int64_t)s.ld, R, pl.L, lpart64, s, cur);
+  else
+    hipLaunchKernelGGL((k_opmul<double, C, false, false, T>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, tau0, x0,
+                       (int64_t)s.ld, y64, (int64_t)s.ld, R, pl.L, lpart64, s, cur);
+  POLAR_LAUNCH_CHECK();
+  if (op->n_long > 0 && pl.grid_l > 0) {
+    hipLaunchKernelGGL((k_longfin<double, C, false, T>), dim3(pl.grid_l), dim3(kBlock), 0, st, *op, tau0, x0,
+                       (int64_t)s.ld, y64, (int64_t)s.ld, R, pl.L, (const double*)lpart64, s, cur, 0);
+    POLAR_LAUNCH_CHECK();
   }
-#undef POLAR_REPL
+  hipLaunchKernelGGL((k_replace_apply<T, C>), dim3(pl.grid_rows), dim3(kBlock), 0, st, op->n, W, ldw,
+                     (const double*)y64, r, R, pl.L, s, cur);
   POLAR_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_replace_fin, dim3(pl.fin), dim3(kBlock), 0, st, s, pl.grid_rows, cur);
   POLAR_LAUNCH_CHECK();
   return 0;
+}
+
+template <typename T>
+static int replace_residual(int cpl, const polar_op* op, const Plan& pl, double tau0, const T* W, int64_t ldw, T* r,
+                            double* y64, double* lpart64, const CGState& s, int cur, hipStream_t st) {
+  if (cpl == 2) return replace_residual_c<T, 2>(op, pl, tau0, W, ldw, r, y64, lpart64, s, cur, st);
+  if constexpr (sizeof(T) == 4) {
+    if (cpl == 4) return replace_residual_c<T, 4>(op, pl, tau0, W, ldw, r, y64, lpart64, s, cur, st);
+  }
+  return replace_residual_c<T, 1>(op, pl, tau0, W, ldw, r, y64, lpart64, s, cur, st);
 }
 
 // End of a window of the blocked update: apply the coefficients to X, P (and, with residual
@@ -1668,15 +1697,15 @@ static int flush_window(int cpl, const Plan& pl, const Ring<T>& ring, const int3
     if (Lf > kFlushLanes || cpl * (int)sizeof(T) > 16) return set_error(POLAR_E_ARG, "flush slice too wide");
     const dim3 grid(pl.nb_s, (s.R + cw - 1) / cw);
     if (cpl == 2) {
-      hipLaunchKernelGGL((k_flush<T, 2>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s);
+      hipLaunchKernelGGL((k_flush<T, 2>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s, d, dsave);
     } else if (cpl == 4) {
-      if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_flush<T, 4>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s);
+      if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_flush<T, 4>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s, d, dsave);
     } else {
-      hipLaunchKernelGGL((k_flush<T, 1>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s);
+      hipLaunchKernelGGL((k_flush<T, 1>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s, d, dsave);
     }
     POLAR_LAUNCH_CHECK();
   }
-  if (dsave) {
+  if (dsave && (nA < n || nA == 0 || pl.nb_s == 0)) {     // rows the flush did not cover
     if (cpl == 2) {
       hipLaunchKernelGGL((k_seed_flush<T, 2>), dim3(pl.nb_d), dim3(kBlock), 0, st, n, s.R, pl.L, ring, d, dsave, s);
     } else if (cpl == 4) {
@@ -1857,6 +1886,14 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   POLAR_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_init_fin, dim3(pl.fin), dim3(kBlock), 0, st, s, pl.grid_rows);
   POLAR_LAUNCH_CHECK();
+  if (s.amask) {
+    if (int rc = zero_async((void*)s.amask, align_up(n, 16), st)) return rc;
+    if (nA > 0) {
+      hipLaunchKernelGGL(k_mark, dim3((int)std::min<int64_t>(4096, (nA + 255) / 256)), dim3(256), 0, st, active, nA,
+                         (uint8_t*)s.amask);
+      POLAR_LAUNCH_CHECK();
+    }
+  }
   if (m.dsave) POLAR_HIP(hipMemcpyAsync(m.dsave, d, sizeof(T) * (size_t)n * R, hipMemcpyDeviceToDevice, st));
   if (nA > 0) {
     const int grid_sh = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (nA + gpb0 - 1) / gpb0));
@@ -1984,7 +2021,8 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
     }
     if (repl && nit == 8 && (it0 + 8) % kReplaceEvery == 0) {
       // the current residual is slot[0] after a full window (Tw = 8) or the only vector (Tw = 1)
-      if (int rc = replace_residual<T>(cpl, op, pl, tau0, W, ldw, slot[0], s, (int)((it0 + 8) & 1), st)) return rc;
+      if (int rc = replace_residual<T>(cpl, op, pl, tau0, W, ldw, slot[0], m.y64, m.lpart64, s, (int)((it0 + 8) & 1), st))
+        return rc;
     }
     if (S > 1) {
       // Re-plan over the segments still iterating: finished segments get no blocks, the others

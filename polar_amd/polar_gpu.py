@@ -293,11 +293,11 @@ def _replacing(replace, block, dtype):
 
 def _col_bytes(n, p, nA, dtype, block=None, replace=False):
     """Device bytes per solved column: the n-row CG vectors (with the residual ring of the blocked
-    update), the FP64 seed iterate and saved direction of residual replacement, and the shifted P
-    block."""
+    update), the FP64 seed iterate, FP64 product and saved direction of residual replacement, and
+    the shifted P block."""
     esz = 8 if dtype == torch.float64 else 4
     nvec = 11 if _blocked(block, dtype) else 4
-    return esz * (nvec * n + p * nA) + ((8 + esz) * n if replace else 0) + 64
+    return esz * (nvec * n + p * nA) + ((16 + esz) * n if replace else 0) + 64
 
 
 def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, freeze=True,
