@@ -86,6 +86,14 @@ GRAPH = False     # direct launches: measured faster than hipGraph replay on the
 REPLACE = None    # FP64 residual replacement: None = the library default (on for FP32-mixed)
 
 
+def step_work(outs):
+    """Column-iterations the step performed: a shared bank solve counts once, not once per s."""
+    tot = sum(o['spmv'] for o in outs)
+    if outs and outs[0].get('shared_bank'):
+        tot -= (len(outs) - 1) * outs[0]['bank_spmv']
+    return int(tot)
+
+
 def step_seconds(outs, union):
     """The step's timed region as the reference defines `seconds` (polar_gpu.py:196, 252-253): from
     after the inputs are resident (sketch H2D, candidate endpoint table) through the final sync;
@@ -97,7 +105,7 @@ def step_seconds(outs, union):
 SCHEDULE = 'hybrid'
 
 
-def run_step(pg, G, inp, sk, precision, union=None, replace=None, block=None):
+def run_step(pg, G, inp, sk, precision, union=None, replace=None, block=None, shared_bank=None):
     """One step: the greedy for every s value.  Schedules (same results):
     'columns'  -- the s values are column groups of one operator (one adjacency pass per phase);
     'segments' -- `union` (a UnionGraph of one copy of G per s) runs them as row segments;
@@ -107,7 +115,7 @@ def run_step(pg, G, inp, sk, precision, union=None, replace=None, block=None):
         from polar_amd import batched
         return batched.polar_columns(G, inp['cand'], inp['k'], list(inp['s_all']), q=inp['q'], B=inp['B'],
                                      tol=inp['tol'], sketch=sk, poles=poles, precision=precision, graph=GRAPH,
-                                     replace=replace, block=block,
+                                     replace=replace, block=block, shared_bank=shared_bank,
                                      bank_union=union if SCHEDULE == 'hybrid' else None)
     if union is not None:
         from polar_amd import batched
@@ -457,11 +465,13 @@ def main():
 
     fp64 = {}
     if not a.skip_fp64:
-        for name, blk in (('fp64', None), ('fp64_blocked', True)):
+        # the plain FP64 path (pinned per-iteration update, per-s seeds: bit-identical to the oracle), and
+        # FP64 on the FP32 leg's schedule (blocked update, shared-seed bank): the precision alone
+        for name, blk, shb in (('fp64', None, None), ('fp64_blocked', True, True)):
             if a.warmup:
-                run_step(pg, G, inp, sk, 'fp64', union, block=blk)
+                run_step(pg, G, inp, sk, 'fp64', union, block=blk, shared_bank=shb)
             torch.cuda.synchronize()
-            o64 = run_step(pg, G, inp, sk, 'fp64', union, block=blk)
+            o64 = run_step(pg, G, inp, sk, 'fp64', union, block=blk, shared_bank=shb)
             torch.cuda.synchronize()
             fp64[name] = dict(seconds=step_seconds(o64, union), outs=o64)
             log(f'{name} step %.3fs' % fp64[name]['seconds'])
@@ -491,7 +501,7 @@ def main():
         del G5, c
         torch.cuda.empty_cache()
 
-    colits = int(sum(o['spmv'] for o in outs32))
+    colits = step_work(outs32)
     dom = max(KERNELS, key=lambda k: prof[k]['ms'])       # k_opmul on every config measured
     if dom != 'k_opmul':
         log(f'note: the dominant kernel class of the profiled step is {dom}; the roofline is k_opmul')
@@ -545,7 +555,9 @@ def main():
                            residual_replacement_note='FP64 residual replacement every 24 iterations (three '
                                                      '8-iteration windows of the blocked update; the c2 '
                                                      'configuration names 25)',
-                           s_schedule={'hybrid': f'{len(inp["s_all"])} s values: bank as row segments, re-rank as column groups',
+                           shared_bank=bool(outs32[0].get('shared_bank')),
+                           s_schedule={'hybrid': f'{len(inp["s_all"])} s values: bank as one shared-seed solve (FP32-mixed; '
+                                                 'row segments for FP64), re-rank as column groups',
                                        'columns': f'{len(inp["s_all"])} s values as column groups of one operator',
                                        'segments': f'{len(inp["s_all"])} s values as segments of one batched solve',
                                        'per-s': 'one greedy run per s value'}[a.schedule],

@@ -391,7 +391,7 @@ def single_segment(G, nA):
 
 def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac', sketch=None, seed=2026,
                   kind='rademacher', poles=None, precision='fp32', graph=None, width=None, block=None,
-                  replace=None, progress=None, bank_union=None):
+                  replace=None, progress=None, bank_union=None, shared_bank=None):
     """POLAR greedy (polar_gpu.py:180-254) for several s values on ONE graph, in lockstep, as column
     groups of one operator: group g is the run for s_list[g], with its own poles, its own bank
     columns [Z | F_g], its own caches and its own selected edges (patch entries tagged g).  Every
@@ -401,11 +401,22 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
     bank_union: a UnionGraph of len(s_list) copies of G; the bank then runs as one segmented solve
     on it (the groups' 2q columns side by side in rows, finished groups leave the grid) and the
     re-rank solves as column groups (same results either way).  replace: FP64 residual replacement
-    in every solve; None = on for FP32, off for FP64."""
+    in every solve; None = on for FP32, off for FP64.
+
+    shared_bank: the groups' bank right-hand sides [Z | F] are the same, so one seed CG can serve all
+    of them: the seed is the smallest pole of all groups and every group's poles are its shifted
+    systems (multi-shift CG), one pass over the adjacency and one set of vector streams per
+    iteration for all s values.  Each pole's solution still meets the tolerance, but the Krylov
+    seed differs from the per-s run's, so results agree with polar_gpu.polar within the CG
+    tolerance rather than bit for bit.  None = on for FP32 (whose results carry FP32 rounding
+    anyway), off for FP64 (bit-identical to the per-s runs and to the oracle).  Per-group `spmv`
+    then counts the shared bank solve's column-iterations in every group."""
     lib = L.require_gpu()
     dev = pg._device()
     dtype = pg._dt(precision)
     replace = pg._replacing(replace, block, dtype)
+    if shared_bank is None:
+        shared_bank = dtype == torch.float32
     dcode = L.dtype_code(dtype)
     stream = L.stream_ptr()
     Gn = len(s_list)
@@ -471,7 +482,31 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
     t0 = time.perf_counter()
 
     terms = [[] for _ in range(Gn)]
-    if bank_union is not None:
+    if shared_bank and Gn > 1:
+        # group-major pole layout, the group holding the smallest pole first (tau[0] is the seed)
+        order = sorted(range(Gn), key=lambda g: (taus[g][0], g))[:1]
+        order += [g for g in range(Gn) if g != order[0]]
+        tau_cat = np.concatenate([taus[g] for g in order])
+        pc = Gn * p
+        XYs = torch.empty((pc, nA, W2), dtype=dtype, device=dev)
+        ws = width or pg.fit_width(n, pc, nA, W2, dtype, block=block, replace=replace)
+        for lo in range(0, W2, ws):
+            hi = min(W2, lo + ws)
+            res = pg.msolve(op, RHS[:, lo:hi], tau_cat, active, tol, trace=want_trace and lo < q,
+                            out=XYs[:, :, lo:hi], graph=graph, block=block, replace=replace)
+            cols = int(res[2].sum())
+            for g in range(Gn):
+                spmv[g] += cols
+                rhs[g] += hi - lo
+            if want_trace and lo < q:
+                te = res[3][:, :min(q, hi) - lo].cpu().numpy()
+                for slot, g in enumerate(order):
+                    terms[g].append((oms[g][:, None] * te[slot * p:(slot + 1) * p]).ravel())
+        Xs, Ys = [None] * Gn, [None] * Gn
+        for slot, g in enumerate(order):
+            Xs[g] = XYs[slot * p:(slot + 1) * p, :, :q]
+            Ys[g] = XYs[slot * p:(slot + 1) * p, :, q:]
+    elif bank_union is not None:
         U = bank_union
         assert len(U.graphs) == Gn and all(x is G for x in U.graphs)
         a0 = np.arange(Gn + 1, dtype=np.int64) * nA
@@ -517,6 +552,7 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
                     terms[g].append(oms[g] * te[:, c - lo])
         Xs = [XY[:, :, g * W2:g * W2 + q] for g in range(Gn)]
         Ys = [XY[:, :, g * W2 + q:(g + 1) * W2] for g in range(Gn)]
+    bank_spmv = spmv.copy()
     hs, ts = [], []
     for g in range(Gn):
         h = torch.empty((p, N), dtype=torch.float64, device=dev)
@@ -614,5 +650,6 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
         out.append(dict(sequence=seqs[g], seconds=seconds, spmv=int(spmv[g]), rhs=int(rhs[g]), p=p,
                         rounds=len(seqs[g]), truncated=len(seqs[g]) < k, marginals=margs[g],
                         trace_estimate=tr[:max(len(seqs[g]), 1)], replace=replace,
+                        shared_bank=bool(shared_bank and Gn > 1), bank_spmv=int(bank_spmv[g]),
                         precision='fp64' if dtype == torch.float64 else 'fp32'))
     return out

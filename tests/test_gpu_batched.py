@@ -183,9 +183,10 @@ def test_polar_columns_equals_per_s(precision):
     s_list = [0.25, 0.5, 0.75]
     poles = [pg.fixed_rule(s, 8, G.sigma, G.Lam(5))[:2] for s in s_list]
     sk = pg.sketches(G, cand, 16, 21, kind='rademacher', on_device='lazy')
-    out = batched.polar_columns(G, cand, 5, s_list, q=16, B=3, sketch=sk, poles=poles, precision=precision)
+    out = batched.polar_columns(G, cand, 5, s_list, q=16, B=3, sketch=sk, poles=poles, precision=precision,
+                                shared_bank=False)
     out_u = batched.polar_columns(G, cand, 5, s_list, q=16, B=3, sketch=sk, poles=poles, precision=precision,
-                                  bank_union=batched.UnionGraph([G] * 3))
+                                  bank_union=batched.UnionGraph([G] * 3), shared_bank=False)
     for a, b in zip(out, out_u):
         for key in ('sequence', 'marginals', 'trace_estimate', 'spmv', 'rhs'):
             assert a[key] == b[key], key
@@ -274,10 +275,35 @@ def test_polar_columns_matches_oracle_per_s(hybrid):
     Go = O.Graph(G.n, G.edges, G.sigma)
     sk = O.sketches(Go, cand, 16, 23, kind='rademacher')
     out = batched.polar_columns(G, cand, 4, s_list, q=16, B=3, sketch=sk, poles=poles, precision='fp64',
-                                bank_union=batched.UnionGraph([G] * 3) if hybrid else None)
+                                bank_union=batched.UnionGraph([G] * 3) if hybrid else None, shared_bank=False)
     for g, s in enumerate(s_list):
         ref = O.polar(Go, cand, 4, s, q=16, B=3, sketch=sk, poles=poles[g], width=32, arith='cr')
         assert out[g]['sequence'] == ref['sequence'], g
         assert (out[g]['spmv'], out[g]['rhs']) == (ref['spmv'], ref['rhs']), g
         assert _rel(out[g]['marginals'], ref['marginals']) <= 1e-13, g
         assert _rel(out[g]['trace_estimate'], ref['trace_estimate']) <= 1e-13, g
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_polar_columns_shared_seed_bank(precision):
+    """One seed CG for the bank of every s value (the smallest pole of all is the seed, every group's
+    poles its shifted systems): each pole still meets the tolerance, so against the per-s runs of
+    the oracle (FP64, pinned arithmetic) the selections are equal, the bank counters are those of a
+    single solve, and marginals and trace estimates agree within 1e-5 -- the CG tolerance (1e-9
+    here) times the condition number, not rounding.  FP32 is the default user of this path."""
+    G = _graphs()[1]
+    cand = graphs.candidates(G.n, G.edges, 350, 15)
+    s_list = [0.25, 0.5, 0.75]
+    poles = [pg.fixed_rule(s, 8, G.sigma, G.Lam(4))[:2] for s in s_list]
+    Go = O.Graph(G.n, G.edges, G.sigma)
+    sk = O.sketches(Go, cand, 16, 25, kind='rademacher')
+    out = batched.polar_columns(G, cand, 4, s_list, q=16, B=3, tol=1e-9, sketch=sk, poles=poles,
+                                precision=precision, shared_bank=True)
+    assert all(o['shared_bank'] for o in out)
+    assert len({o['bank_spmv'] for o in out}) == 1          # one bank solve, counted in every group
+    for g, s in enumerate(s_list):
+        ref = O.polar(Go, cand, 4, s, q=16, B=3, tol=1e-9, sketch=sk, poles=poles[g], width=32, arith='cr')
+        assert out[g]['sequence'] == ref['sequence'], g
+        assert out[g]['rhs'] == ref['rhs'], g
+        assert _rel(out[g]['marginals'], ref['marginals']) <= 1e-5, g
+        assert _rel(out[g]['trace_estimate'], ref['trace_estimate']) <= 1e-5, g
