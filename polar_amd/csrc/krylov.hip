@@ -12,7 +12,7 @@
 //    iteration (Jegerlehner zeta, alpha/beta of every pole, freezing, convergence) in FP64.
 //  * Per iteration: opmul (+dot)  [-> longfin]  -> resid (+|r|^2, scalars)  -> update (d and the
 //    shifted x/p of every live pole on the active rows).  A device flag makes launches after
-//    convergence no-ops, so the host reads it only every 8 iterations.
+//    convergence no-ops, so the host reads it only once per window of 8 iterations.
 #include "common.hpp"
 
 #include <mutex>
@@ -242,7 +242,8 @@ __device__ __forceinline__ void finish_row(const polar_op& op, double shift, con
   finish_row_with<T, CPL>(op, shift, X, ldx, c0, pidx, acc, xv, dgd, yv, cg);
 }
 
-constexpr int kW = 8;            // window length of the blocked update
+constexpr int kW = 8;            // window length of the blocked update (iterations between flushes)
+constexpr int kBatch = kW;       // iterations launched between two convergence checks (one window)
 
 // ------------------------------------------------------------------------------------------------
 // CG state (device), laid out in the workspace.
@@ -1780,7 +1781,7 @@ static hipStream_t capture_stream(int& rc) {
   return cs;
 }
 
-// An instantiated hipGraph of 8 CG iterations (parities 0,1,...,1), rebuilt when the plan, the
+// An instantiated hipGraph of one window of CG iterations (parities 0,1,...,1), rebuilt when the plan, the
 // width or the vector buffers change.
 struct IterGraph {
   hipGraph_t g = nullptr;
@@ -1798,7 +1799,7 @@ struct IterGraph {
 // kernel's LDS), else 1 (per-iteration updates).
 static int window_of(int flags, int p) { return ((flags & POLAR_BLOCK8) && p <= kMaxPolesFlush) ? kW : 1; }
 
-constexpr int kReplaceEvery = 24;   // residual replacement period (iterations; whole windows)
+constexpr int kReplaceEvery = 3 * kW;   // residual replacement period (iterations; whole windows)
 
 template <typename T>
 static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W, int64_t ldw, int64_t R,
@@ -1944,13 +1945,13 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
     return rg;
   };
   bool converged = false;
-  for (int64_t it0 = 0; it0 < maxit; it0 += 8) {
-    const int nit = (int)std::min<int64_t>(8, maxit - it0);
-    if (use_graph && nit == 8) {
+  for (int64_t it0 = 0; it0 < maxit; it0 += kBatch) {
+    const int nit = (int)std::min<int64_t>(kBatch, maxit - it0);
+    if (use_graph && nit == kBatch) {
       if (!graph.x) {
         POLAR_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
         int rc = 0;
-        for (int k = 0; k < 8 && !rc; ++k)
+        for (int k = 0; k < kBatch && !rc; ++k)
           rc = run_iter_any<T>(cpl, op, pl, tau0, slot[k % kW], slot[(k + 1) % kW], d, v, lpart, P, Xo, sj, si, active,
                                nA, p, s, k & 1, k, nullptr, cs);
         if (!rc && Tw > 1) rc = flush_window<T>(cpl, pl, ring_of(), active, nA, p, P, Xo, sj, si, s, n, d, m.dsave, cs);
@@ -1976,8 +1977,8 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
             POLAR_HIP(hipEventCreate(&e));
             ev.push_back(e);
           }
-          // mode 3: one iteration per batch of 8, at a position that rotates from batch to batch
-          const bool take = g_prof_on != 3 || k == (int)((it0 / 8) % 8);
+          // mode 3: one iteration in every 8, at a position that rotates from one group of 8 to the next
+          const bool take = g_prof_on != 3 || (k % 8) == (int)(((it0 + k) / 8) % 8);
           evp = take ? ev.data() + slot : nullptr;
           widths.push_back(s.R);
           timed.push_back(take ? 1 : 0);
@@ -2019,9 +2020,10 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
       converged = true;
       break;
     }
-    if (repl && nit == 8 && (it0 + 8) % kReplaceEvery == 0) {
-      // the current residual is slot[0] after a full window (Tw = 8) or the only vector (Tw = 1)
-      if (int rc = replace_residual<T>(cpl, op, pl, tau0, W, ldw, slot[0], m.y64, m.lpart64, s, (int)((it0 + 8) & 1), st))
+    if (repl && nit == kBatch && (it0 + kBatch) % kReplaceEvery == 0) {
+      // the current residual is slot[0] after a full window (Tw = kW) or the only vector (Tw = 1)
+      if (int rc = replace_residual<T>(cpl, op, pl, tau0, W, ldw, slot[0], m.y64, m.lpart64, s,
+                                       (int)((it0 + kBatch) & 1), st))
         return rc;
     }
     if (S > 1) {

@@ -114,7 +114,7 @@ def test_residual_replacement_after_compaction(precision, block):
     iteration 24, so every replacement reads the right-hand side through the slot -> column map.
     FP64: X with replacement equals X without it within 1e-8 relative (per-iteration and blocked
     seed iterate).  FP32 (blocked): both within 1e-4 of the FP64 solution, the replaced one no
-    worse than twice the plain one."""
+    worse than twice the plain one or than the FP32 rounding floor (1e-6)."""
     G, W, tau, act = _problem(R=16)
     W[:, 0::2] = 0.0
     W[:, 5] *= 1e-3                              # columns of very different scales
@@ -133,7 +133,7 @@ def test_residual_replacement_after_compaction(precision, block):
         X64 = pg.msolve(pg.Op(G), torch.from_numpy(W).cuda(), tau, act, 1e-11)[0]
         for c in cols:
             ep, er = _rel(Xp[:, :, c].double(), X64[:, :, c]), _rel(Xr[:, :, c].double(), X64[:, :, c])
-            assert er <= 1e-4 and er <= 2 * ep + 1e-7, (c, ep, er)
+            assert er <= 1e-4 and er <= max(2 * ep, 1e-6), (c, ep, er)     # 1e-6: FP32 rounding floor
 
 
 def test_residual_replacement_fp64_matches_pinned_within_tolerance():
