@@ -187,6 +187,30 @@ def spmm_lattice_bench(pg, torch, lib, L, n_scale, reps=10):
     return dict(n=n, m=len(e), nnz_offdiag=G.nnz, R=R, graph=f'{side}x{side} lattice', **res)
 
 
+def flush_bench(pg, torch, graphs):
+    """The window flush alone at the n=4M bank's width: an FP32 blocked solve on a Chung-Lu graph
+    (n=200k, mean degree 20), 128 Rademacher columns, 32 poles, every row active and a tolerance
+    nothing reaches, so every (pole, column) pair is changed in each of the 8 windows of the 64
+    iterations it runs (the solve then stops at its iteration cap).  The flush's duration is read
+    from a kernel trace; its bytes per launch are the compulsory ones: P and X of every pole read
+    and written, the 8 window residuals read."""
+    n, R, p, iters = 200_000, 128, 32, 64
+    G = pg.Graph(n, graphs.chung_lu(n, 2.5, 20, 1), 0.05)
+    W = torch.from_numpy(np.random.default_rng(0).choice([-1.0, 1.0], size=(n, R))).float().cuda()
+    tau = np.geomspace(0.05, 400.0, p)
+    for replace in (False, True):
+        try:
+            pg.msolve(pg.Op(G, precision='fp32'), W, tau, np.arange(n), 1e-30, maxit=iters, block=True,
+                      replace=replace)
+        except RuntimeError as ex:
+            if 'iteration cap' not in str(ex):
+                raise
+    torch.cuda.synchronize()
+    return dict(n=n, R=R, poles=p, iterations=iters, windows=iters // 8, runs=['replace off', 'replace on'],
+                flush_bytes=p * n * R * 4 * 4 + 8 * n * R * 4,
+                seed_bytes_with_replace=n * R * (4 + 4 + 4 + 8 + 8))
+
+
 def _rel_max(a, b):
     a, b = np.asarray(a, float), np.asarray(b, float)
     n = min(len(a), len(b))
@@ -376,6 +400,8 @@ def main():
                     help='FP64 residual replacement in the FP32-mixed step (default: on)')
     ap.add_argument('--spmm-only', action='store_true',
                     help='only the n=4M SpMM leg and the 1 GiB copy probe (for counter runs)')
+    ap.add_argument('--flush-only', action='store_true',
+                    help='only the window-flush solve (every pole changed; for kernel traces)')
     a = ap.parse_args()
     global GRAPH, REPLACE, SCHEDULE
     SCHEDULE = a.schedule
@@ -385,7 +411,7 @@ def main():
         print(f'bench.py measures one GPU only (asked for --gpus {a.gpus}); the pole-sharded multi-GPU '
               'layout is a design note, not built.', file=sys.stderr)
         sys.exit(2)
-    cpu_proc = None if (a.skip_cpu or a.spmm_only) else start_cpu_child(a)
+    cpu_proc = None if (a.skip_cpu or a.spmm_only or a.flush_only) else start_cpu_child(a)
 
     import torch
     from polar_amd import _lib as L
@@ -400,6 +426,10 @@ def main():
         G5 = pg.Graph(c['n'], c['edges'], c['sigma'])
         print(json.dumps(dict(spmm_n4m=spmm_bench(pg, torch, lib, L, G5),
                               spmm_lattice_n4m=spmm_lattice_bench(pg, torch, lib, L, 1.0))), flush=True)
+        return
+    if a.flush_only:
+        from polar_amd import graphs
+        print(json.dumps(dict(flush=flush_bench(pg, torch, graphs))), flush=True)
         return
 
     t = time.time()

@@ -892,23 +892,31 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
 
 // Blocked shifted update, once per window of kW iterations (after the window's last iteration):
 // X_j += cX0 P_j + sum_t cXt r_t;  P_j = cP0 P_j + sum_t cPt r_t  on the active rows, for the
-// poles changed in the window.  Each thread keeps its rows' window residuals in registers; the
-// poles' coefficients are staged in LDS and shared by the block's rows, in two buffers: the next
-// pole's are staged right after the current pole is applied, so one barrier per pole suffices.
-// The changed poles come from one flag per pole (pdirty).
-constexpr int kFlushRows = 1;    // rows per thread per tile (measured: 1 < 2 < 4 in step time)
+// poles changed in the window.  Each thread keeps its row's window residuals in registers.  The
+// changed poles come from one flag per pole (pdirty) and are taken kFlushChunk at a time: a chunk's
+// coefficients are staged in LDS (two buffers), all its P and X rows are loaded together, and the
+// next chunk's coefficients are fetched into registers while the current chunk computes, so a
+// chunk costs one memory round trip and one barrier instead of one per pole.
 constexpr int kMaxPolesFlush = 1024;
-constexpr int kFlushLanes = 8;   // lanes per row of a flush tile (column slices of 128 bytes)
+constexpr int kFlushLanes = 16;    // lanes per row of a flush tile (column slices of 128 bytes)
+constexpr int kFlushWaves = 8;     // 512 threads: a tile of 32 rows shares each staged chunk
+constexpr int kFlushBlock = 64 * kFlushWaves;
+constexpr int kFlushChunk = 2;     // poles per staged chunk (2 and 4 measured alike; 2 spills nothing)
+
+// Columns per lane of the flush: half the solve's 16-byte vectors (8 bytes per lane, 16 lanes per
+// 128-byte row piece), so that a lane's window residuals and a chunk's P and X rows fit 4 waves per
+// SIMD, two 512-thread blocks per CU (4 FP32 columns per lane needed 182 registers, 2 waves).
+inline int flush_cpl(int cpl) { return cpl >= 2 ? cpl / 2 : 1; }
 
 template <typename T> struct Ring {
   const T* s[kW];                // r_t is in s[t % kW]
 };
 
-template <typename T, int CPL>
-__global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int L, int cw, Ring<T> ring,
-                                                  const int32_t* __restrict__ active, T* __restrict__ P,
-                                                  T* __restrict__ Xo, int64_t sj, int64_t si, CGState s,
-                                                  const T* __restrict__ dvec, T* __restrict__ dsave) {
+template <typename T, int CPL, bool IDENT>
+__global__ __launch_bounds__(kFlushBlock, 4) void k_flush(int64_t nA, int p, int R, int L, int cw, Ring<T> ring,
+                                                       const int32_t* __restrict__ active, T* __restrict__ P,
+                                                       T* __restrict__ Xo, int64_t sj, int64_t si, CGState s,
+                                                       const T* __restrict__ dvec, T* __restrict__ dsave) {
   // column slice blockIdx.y: columns [cbase, cbase + Rs) (cw = L * CPL columns per slice)
   const int cbase = blockIdx.y * cw;
   const int Rs = min(cw, R - cbase);
@@ -916,124 +924,182 @@ __global__ __launch_bounds__(kBlock) void k_flush(int64_t nA, int p, int R, int 
   int64_t a0, a1;
   seg_range(s.seg_a, s.S, sp.g, nA, a0, a1);
   const int vb = sp.g * s.ld;
+  // a slice is at most kFlushLanes lanes of CPL columns
+  constexpr int kMaxR = kFlushLanes * CPL;
+  constexpr int kNQ = (kW + 1) * kMaxR;            // coefficients per pole and slice (row stride kMaxR)
+  constexpr int kStage = (kFlushChunk * kNQ + kFlushBlock - 1) / kFlushBlock;   // per thread and chunk
   __shared__ int s_jl[kMaxPolesFlush];
   __shared__ int s_nj;
-  // two coefficient buffers: pole jj reads buffer jj & 1, pole jj + 1 is staged into the other;
-  // a slice is at most kFlushLanes lanes of 16-byte vectors (so the LDS allows more blocks per CU)
-  constexpr int kMaxR = kFlushLanes * (16 / (int)sizeof(T));
-  __shared__ T s_cx[2][(kW + 1) * kMaxR], s_cp[2][(kW + 1) * kMaxR];
-  __shared__ int s_dirty[2][kMaxR];
+  __shared__ T s_cx[2][kFlushChunk][kNQ], s_cp[2][kFlushChunk][kNQ];   // [buffer][pole][t * kMaxR + c]
+  __shared__ int s_dirty[2][kFlushChunk][kMaxR];
   // residual replacement: pole 0's window coefficients in FP64, for the seed iterate x0
-  __shared__ double s_c0[(kW + 1) * kMaxR];
+  __shared__ double s_c0[kNQ];
   const bool seed = dsave != nullptr;
   if (threadIdx.x == 0) s_nj = 0;
   __syncthreads();
-  for (int j = threadIdx.x; j < p; j += kBlock)
+  for (int j = threadIdx.x; j < p; j += kFlushBlock)
     if (s.pdirty[j * s.S + sp.g]) s_jl[atomicAdd(&s_nj, 1)] = j;   // poles are independent: any order
   if (seed)
-    for (int q = threadIdx.x; q < (kW + 1) * Rs; q += kBlock) {
-      const int t = q / Rs, c = q - t * Rs;
-      s_c0[q] = s.cX[(int64_t)t * s.sld + vb + cbase + c];
+    for (int q = threadIdx.x; q < kNQ; q += kFlushBlock) {
+      const int t = q / kMaxR, c = q % kMaxR;
+      if (c < Rs) s_c0[q] = s.cX[(int64_t)t * s.sld + vb + cbase + c];
     }
   __syncthreads();
   const int nj = s_nj;
   if (nj == 0) return;                           // uniform over the block
-  const int nq = (kW + 1) * Rs;                  // coefficients per pole and slice (Rs <= kMaxR)
-  auto stage = [&](int j, int b) {               // pole j's coefficients and column flags -> buffer b
-    for (int q = threadIdx.x; q < nq; q += kBlock) {
-      const int t = q / Rs, c = q - t * Rs;
-      const int64_t src = ((int64_t)j * (kW + 1) + t) * s.sld + vb + cbase + c;
-      s_cx[b][q] = (T)s.cX[src];
-      s_cp[b][q] = (T)s.cP[src];
+  const int nch = (nj + kFlushChunk - 1) / kFlushChunk;
+  // chunk ch's coefficients (FP64 in memory) and column flags into registers ...
+  double gx[kStage], gp[kStage];
+  int gd = 0;
+  auto fetch = [&](int ch) {
+    const int j0 = ch * kFlushChunk;
+    const int npc = min(kFlushChunk, nj - j0);
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int q = threadIdx.x + u * kFlushBlock;
+      const int pc = q / kNQ, t = (q % kNQ) / kMaxR, c = q % kMaxR;
+      if (pc < npc && c < Rs) {
+        const int64_t src = ((int64_t)s_jl[j0 + pc] * (kW + 1) + t) * s.sld + vb + cbase + c;
+        gx[u] = s.cX[src];
+        gp[u] = s.cP[src];
+      }
     }
-    for (int c = threadIdx.x; c < Rs; c += kBlock) s_dirty[b][c] = s.dirty[(int64_t)j * s.sld + vb + cbase + c];
+    if (threadIdx.x < npc * Rs) {
+      const int pc = threadIdx.x / Rs, c = threadIdx.x - pc * Rs;
+      gd = s.dirty[(int64_t)s_jl[j0 + pc] * s.sld + vb + cbase + c];
+    }
+  };
+  // ... and from registers into LDS buffer b
+  auto put = [&](int ch, int b) {
+    const int npc = min(kFlushChunk, nj - ch * kFlushChunk);
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int q = threadIdx.x + u * kFlushBlock;
+      const int pc = q / kNQ, c = q % kMaxR;
+      if (pc < npc && c < Rs) {
+        s_cx[b][pc][q % kNQ] = (T)gx[u];
+        s_cp[b][pc][q % kNQ] = (T)gp[u];
+      }
+    }
+    if (threadIdx.x < npc * Rs) {
+      const int pc = threadIdx.x / Rs, c = threadIdx.x - pc * Rs;
+      s_dirty[b][pc][c] = gd;
+    }
   };
   Geo g(L);
   const int cl = g.li * CPL;                     // column within the slice
   const int c0 = cbase + cl;
   const bool colok = cl < Rs;
-  const int gpb = kWaves * g.G;
-  const int64_t tile = (int64_t)gpb * kFlushRows;
+  // compacted columns (!IDENT): slot c0 + e holds original column colmap[c0 + e] of X
+  int xcol[CPL];
+#pragma unroll
+  for (int e = 0; e < CPL; ++e) xcol[e] = IDENT || !colok ? c0 + e : s.colmap[c0 + e];
+  const int64_t tile = (int64_t)kFlushWaves * g.G;
   for (int64_t base = a0 + (int64_t)sp.rank * tile; base < a1; base += (int64_t)sp.nb * tile) {
-    T rr[kFlushRows][kW][CPL];
-    int64_t row[kFlushRows];
-    bool ok[kFlushRows];
+    T rr[kW][CPL];
+    const int64_t row = base + g.wave * g.G + g.gi;
+    const bool ok = colok && row < a1;
+    if (ok) {
+      const int64_t ar = active[row];
 #pragma unroll
-    for (int k = 0; k < kFlushRows; ++k) {
-      row[k] = base + g.wave * g.G + g.gi + (int64_t)k * gpb;
-      ok[k] = colok && row[k] < a1;
-      if (ok[k]) {
-        const int64_t ar = active[row[k]];
+      for (int t = 1; t <= kW; ++t) vload<T, CPL>(ring.s[t % kW] + ar * s.ld + c0, rr[t - 1]);
+      if (seed) {
+        // seed iterate on this active row: x0 += cX0 d(window start) + sum_t cXt r_t with pole 0's
+        // coefficients (the seed pole), the residuals already in registers; then the saved
+        // direction becomes d, the next window's start
+        T b0[CPL], dv[CPL];
+        double xv0[CPL];
+        vload<T, CPL>(dsave + ar * s.ld + c0, b0);
+        vload<T, CPL>(dvec + ar * s.ld + c0, dv);
+        double* xp = s.x0 + ar * s.ld + c0;
+        vload<double, CPL>(xp, xv0);
 #pragma unroll
-        for (int t = 1; t <= kW; ++t) vload<T, CPL>(ring.s[t % kW] + ar * s.ld + c0, rr[k][t - 1]);
-        if (seed) {
-          // seed iterate on this active row: x0 += cX0 d(window start) + sum_t cXt r_t with pole 0's
-          // coefficients (the seed pole), the residuals already in registers; then the saved
-          // direction becomes d, the next window's start
-          T b0[CPL], dv[CPL];
-          double xv0[CPL];
-          vload<T, CPL>(dsave + ar * s.ld + c0, b0);
-          vload<T, CPL>(dvec + ar * s.ld + c0, dv);
-          double* xp = s.x0 + ar * s.ld + c0;
-          vload<double, CPL>(xp, xv0);
+        for (int e = 0; e < CPL; ++e) {
+          __builtin_amdgcn_sched_barrier(0);     // one column's FP64 terms live at a time
+          double inc = s_c0[cl + e] * (double)b0[e];
 #pragma unroll
-          for (int e = 0; e < CPL; ++e) {
-            double inc = s_c0[cl + e] * (double)b0[e];
-#pragma unroll
-            for (int t = 1; t <= kW; ++t) inc = inc + s_c0[t * Rs + cl + e] * (double)rr[k][t - 1][e];
-            xv0[e] = xv0[e] + inc;
-          }
-          vstore<double, CPL>(xp, xv0);
-          vstore<T, CPL>(dsave + ar * s.ld + c0, dv);
+          for (int t = 1; t <= kW; ++t) inc = inc + s_c0[t * kMaxR + cl + e] * (double)rr[t - 1][e];
+          xv0[e] = xv0[e] + inc;
         }
+        vstore<double, CPL>(xp, xv0);
+        vstore<T, CPL>(dsave + ar * s.ld + c0, dv);
       }
     }
     __syncthreads();                             // the previous tile's last buffer is consumed
-    stage(s_jl[0], 0);
+    fetch(0);
+    put(0, 0);
     __syncthreads();
-    for (int jj = 0; jj < nj; ++jj) {
-      const int j = s_jl[jj];
-      const int b = jj & 1;
-      // columns unchanged in the window have identity coefficients: nothing to read or write
-      bool mine = false;
-      if (colok) {
+    for (int ch = 0; ch < nch; ++ch) {
+      const int b = ch & 1;
+      const int j0 = ch * kFlushChunk;
+      const int npc = min(kFlushChunk, nj - j0);
+      if (ch + 1 < nch) fetch(ch + 1);           // in flight while this chunk computes
+      // every P and X row of the chunk is requested before any is used; columns unchanged in the
+      // window have identity coefficients: nothing to read or write
+      bool mine[kFlushChunk];
+      T* prow[kFlushChunk];
+      T* xrow[kFlushChunk];
 #pragma unroll
-        for (int e = 0; e < CPL; ++e) mine |= s_dirty[b][cl + e] != 0;
+      for (int pc = 0; pc < kFlushChunk; ++pc) {
+        // the pole is the same for the whole block: its row bases stay in scalar registers
+        const int j = __builtin_amdgcn_readfirstlane(s_jl[j0 + min(pc, npc - 1)]);
+        prow[pc] = P + ((int64_t)j * nA + row) * s.ld + c0;
+        xrow[pc] = Xo + (int64_t)j * sj + row * si;
+        int any = 0;
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) any |= s_dirty[b][pc][cl + e];
+        mine[pc] = ok && pc < npc && any != 0;
+      }
+      T pv[kFlushChunk][CPL], xv[kFlushChunk][CPL];
+#pragma unroll
+      for (int pc = 0; pc < kFlushChunk; ++pc) {
+        if (mine[pc]) {
+          vload_nt<T, CPL>(prow[pc], pv[pc]);     // P and X rows: touched once per window
+          if constexpr (IDENT) {
+            vload_nt<T, CPL>(xrow[pc] + c0, xv[pc]);
+          } else {
+#pragma unroll
+            for (int e = 0; e < CPL; ++e) xv[pc][e] = xrow[pc][xcol[e]];
+          }
+        }
       }
 #pragma unroll
-      for (int k = 0; k < kFlushRows; ++k) {
-        if (!ok[k] || !mine) continue;
-        T* pp = P + ((int64_t)j * nA + row[k]) * s.ld + c0;
-        T* xrow = Xo + (int64_t)j * sj + row[k] * si;
-        T pv[CPL], xv[CPL];
-        vload_nt<T, CPL>(pp, pv);                 // P and X rows: touched once per window
-        if (s.ident) {
-          vload_nt<T, CPL>(xrow + c0, xv);
-        } else {
+      for (int pc = 0; pc < kFlushChunk; ++pc) {
+        // one pole's coefficients live at a time (in two halves), so that the chunk's P and X rows
+        // and the next chunk's coefficients fit beside them at 4 waves per SIMD
+        __builtin_amdgcn_sched_barrier(0);
+        if (!mine[pc]) continue;
+        const T* cx = s_cx[b][pc];
+        const T* cp = s_cp[b][pc];
+        T ax[CPL], ap[CPL];
 #pragma unroll
-          for (int e = 0; e < CPL; ++e) xv[e] = xrow[s.colmap[c0 + e]];
+        for (int e = 0; e < CPL; ++e) {
+          ax[e] = cx[cl + e] * pv[pc][e];
+          ap[e] = cp[cl + e] * pv[pc][e];
+        }
+#pragma unroll
+        for (int t = 1; t <= kW; ++t) {
+          if (t == kW / 2 + 1) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int e = 0; e < CPL; ++e) {
+            ax[e] = ax[e] + cx[t * kMaxR + cl + e] * rr[t - 1][e];
+            ap[e] = ap[e] + cp[t * kMaxR + cl + e] * rr[t - 1][e];
+          }
         }
 #pragma unroll
         for (int e = 0; e < CPL; ++e) {
-          T ax = s_cx[b][cl + e] * pv[e];
-          T ap = s_cp[b][cl + e] * pv[e];
-#pragma unroll
-          for (int t = 1; t <= kW; ++t) {
-            ax = ax + s_cx[b][t * Rs + cl + e] * rr[k][t - 1][e];
-            ap = ap + s_cp[b][t * Rs + cl + e] * rr[k][t - 1][e];
-          }
-          xv[e] = xv[e] + ax;
-          pv[e] = ap;
+          xv[pc][e] = xv[pc][e] + ax[e];
+          pv[pc][e] = ap[e];
         }
-        if (s.ident) {
-          vstore_nt<T, CPL>(xrow + c0, xv);
+        if constexpr (IDENT) {
+          vstore_nt<T, CPL>(xrow[pc] + c0, xv[pc]);
         } else {
 #pragma unroll
-          for (int e = 0; e < CPL; ++e) xrow[s.colmap[c0 + e]] = xv[e];
+          for (int e = 0; e < CPL; ++e) xrow[pc][xcol[e]] = xv[pc][e];
         }
-        vstore_nt<T, CPL>(pp, pv);
+        vstore_nt<T, CPL>(prow[pc], pv[pc]);
       }
-      if (jj + 1 < nj) stage(s_jl[jj + 1], b ^ 1);   // b ^ 1 was last read before the previous barrier
+      if (ch + 1 < nch) put(ch + 1, b ^ 1);      // b ^ 1 was last read before the previous barrier
       __syncthreads();
     }
   }
@@ -1299,8 +1365,8 @@ static Plan make_plan(const polar_op* op, int64_t R, int cpl, const Segs* sg = n
   const int64_t gpb = (int64_t)kWaves * pl.G;
   // blocked update: the shifted part is the window flush, whose tiles are column slices of at
   // most kFlushLanes lanes per row (more rows per tile); otherwise k_update's full-width rows
-  pl.Lf = Tw > 1 ? std::min(pl.L, kFlushLanes) : pl.L;
-  const int64_t gpb_s = (int64_t)kWaves * (64 / pl.Lf);
+  pl.Lf = Tw > 1 ? std::min(lanes_for(R, flush_cpl(cpl)), kFlushLanes) : pl.L;
+  const int64_t gpb_s = (int64_t)(Tw > 1 ? kFlushWaves : kWaves) * (64 / pl.Lf);
   const int S = sg ? sg->S : 1;
   pl.fin = (int)(S * R);
   if (S == 1) {
@@ -1691,19 +1757,23 @@ static int flush_window(int cpl, const Plan& pl, const Ring<T>& ring, const int3
                         T* P, T* Xo, int64_t sj, int64_t si, const CGState& s, int64_t n, const T* d, T* dsave,
                         hipStream_t st) {
   if (nA > 0 && pl.nb_s > 0) {
-    // column slices of pl.Lf <= 8 lanes (128-byte row pieces): a block's tile has 32 rows, and
-    // each staged pole serves 4x more rows than a full-width tile at 32 lanes
+    // column slices of pl.Lf <= 16 lanes (128-byte row pieces): a block's tile has 32 rows, and
+    // each staged chunk of poles serves all of them
     const int Lf = pl.Lf;
-    const int cw = Lf * cpl;
-    if (Lf > kFlushLanes || cpl * (int)sizeof(T) > 16) return set_error(POLAR_E_ARG, "flush slice too wide");
+    const int fc = flush_cpl(cpl);
+    const int cw = Lf * fc;
+    if (Lf > kFlushLanes || fc * (int)sizeof(T) > 8) return set_error(POLAR_E_ARG, "flush slice too wide");
     const dim3 grid(pl.nb_s, (s.R + cw - 1) / cw);
-    if (cpl == 2) {
-      hipLaunchKernelGGL((k_flush<T, 2>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s, d, dsave);
-    } else if (cpl == 4) {
-      if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_flush<T, 4>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s, d, dsave);
+#define POLAR_FLUSH(C, I) \
+  hipLaunchKernelGGL((k_flush<T, C, I>), grid, dim3(kFlushBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s, d, dsave)
+    if (fc == 2) {
+      if constexpr (sizeof(T) == 4) {
+        if (s.ident) POLAR_FLUSH(2, true); else POLAR_FLUSH(2, false);
+      }
     } else {
-      hipLaunchKernelGGL((k_flush<T, 1>), grid, dim3(kBlock), 0, st, nA, p, s.R, Lf, cw, ring, active, P, Xo, sj, si, s, d, dsave);
+      if (s.ident) POLAR_FLUSH(1, true); else POLAR_FLUSH(1, false);
     }
+#undef POLAR_FLUSH
     POLAR_LAUNCH_CHECK();
   }
   if (dsave && (nA < n || nA == 0 || pl.nb_s == 0)) {     // rows the flush did not cover
