@@ -90,7 +90,8 @@ typedef struct polar_op {
   const int32_t* patch_cols;  /* [patch_ptr[n_patch]] */
   int32_t grid_max;           /* tuning: block budget of the kernels with partial sums (0 = default:
                                  1024 blocks, 16384 for the operator kernel on >= 2^20 rows) */
-  int32_t pad2;
+  int32_t run_rows;           /* tuning: consecutive rows per work unit of the operator kernel (their
+                                 gathers share batches); 0 = default, 1 = one row per unit */
   const int32_t* patch_grp;   /* [patch_ptr[n_patch]] column group of each patch entry, or NULL
                                  (every entry applies to every column) */
 } polar_op;

@@ -42,7 +42,7 @@ class PolarOp(ctypes.Structure):
         ('long_rows', vp), ('long_base', vp), ('extra', vp),
         ('n_patch', i32), ('cpl_max', i32),
         ('patch_rows', vp), ('patch_ptr', vp), ('patch_cols', vp),
-        ('grid_max', i32), ('pad2', i32), ('patch_grp', vp),
+        ('grid_max', i32), ('run_rows', i32), ('patch_grp', vp),
     ]
 
 

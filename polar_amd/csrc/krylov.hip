@@ -540,14 +540,192 @@ __global__ __launch_bounds__(kBlock) void k_init_fin(CGState s, int nparts) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// A run of nr consecutive rows [ra, ra + nr) of the operator (nr < L), processed by one row group:
+// lane li of the group holds ext = indptr[ra + li] (li <= nr).  The run's short rows are walked as
+// one entry sequence, so a batch of 8 gathers spans row boundaries (a row of 4 entries shares its
+// batch with the next row) and one index round serves several rows; each row still sums its
+// entries in adjacency order, so the row values are those of the one-row-per-unit walk.  Long rows
+// of the run are skipped (their chunks are units of the chunk list).  Every shuffle reads lanes of
+// the same group under conditions that are uniform within the group.
+template <typename T, int CPL, bool WEIGHTED, bool DOT, typename TW>
+__device__ __forceinline__ void run_rows_walk(const polar_op& op, double shift, const T* __restrict__ X,
+                                              int64_t ldx, T* __restrict__ Y, int64_t ldy, const Geo& g,
+                                              bool colok, int c0, int ra, int nr, int ext, DD (&dacc)[CPL],
+                                              int& pc, const ColGroups& cg) {
+  const int gbase = g.gi * g.L;
+  const int CH = op.chunk;
+  // long rows of the run: bit i of lmask (i < nr)
+  const int ext_next = __shfl(ext, gbase + min(g.li + 1, g.L - 1), 64);
+  const bool isl = g.li < nr && (ext_next - ext) > CH;
+  const uint64_t bal = __ballot(isl);
+  const uint64_t lmask = g.L == 64 ? bal : ((bal >> gbase) & ((1ull << g.L) - 1));
+  auto finish = [&](int rowabs, const T (&acc)[CPL], const T (&xv)[CPL], double dgd) {
+    int pidx = -1;
+    if (op.n_patch > 0) {
+      if (pc < 0) pc = lower_bound_i32(op.patch_rows, op.n_patch, rowabs);
+      while (pc < op.n_patch && op.patch_rows[pc] < rowabs) ++pc;
+      if (pc < op.n_patch && op.patch_rows[pc] == rowabs) pidx = pc;
+    }
+    if (colok) {
+      T yv[CPL];
+      finish_row_with<T, CPL>(op, shift, X, ldx, c0, pidx, acc, xv, dgd, yv, cg);
+      vstore<T, CPL>(Y + (int64_t)rowabs * ldy + c0, yv);
+      if (DOT) {
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) dd_add(dacc[e], (double)xv[e] * (double)yv[e]);
+      }
+    }
+  };
+  int cr = 0;
+  while (cr < nr) {
+    const uint64_t rem = lmask >> cr;
+    if (rem & 1ull) {               // skip the long rows at cr
+      cr += (int)__builtin_ctzll(~rem);
+      continue;
+    }
+    const int ce = rem == 0 ? nr : min(nr, cr + (int)__builtin_ctzll(rem));
+    // rows [cr, ce) are short: entries [ext(cr), ext(ce)) in one sequence
+    const int E0 = __shfl(ext, gbase + cr, 64);
+    const int E1 = __shfl(ext, gbase + ce, 64);
+    int row = cr;
+    int rend = __shfl(ext, gbase + cr + 1, 64);
+    T acc[CPL], xv[CPL];
+    double dgd = 0.0;
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) acc[e] = T(0);
+    if (colok) {
+      vload<T, CPL>(X + (int64_t)(ra + row) * ldx + c0, xv);
+      dgd = (double)((const TW*)op.diag)[ra + row];
+    }
+    for (int k0 = E0; k0 < E1; k0 += 8 * g.L) {
+      const int cnt = min(8 * g.L, E1 - k0);
+      int myj[8];
+      T myw[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int t = g.li * 8 + u;
+        myj[u] = (t < cnt) ? op.indices[k0 + t] : 0;
+        if (WEIGHTED) myw[u] = (t < cnt) ? (T)((const TW*)op.weights)[k0 + t] : T(0);
+      }
+      for (int b = 0; 8 * b < cnt; ++b) {
+        int j[8];
+        T w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          j[u] = __shfl(myj[u], gbase + b, 64);
+          if (WEIGHTED) w[u] = __shfl(myw[u], gbase + b, 64);
+        }
+        const int nb = min(8, cnt - 8 * b);
+        T x[8][CPL];
+        if (colok) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (u < nb) vload<T, CPL>(X + (int64_t)j[u] * ldx + c0, x[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if (u < nb) {
+            const int k = k0 + 8 * b + u;
+            while (k >= rend) {     // rows ending before entry k (empty rows included)
+              finish(ra + row, acc, xv, dgd);
+              ++row;
+              rend = __shfl(ext, gbase + row + 1, 64);
+#pragma unroll
+              for (int e = 0; e < CPL; ++e) acc[e] = T(0);
+              if (colok) {
+                vload<T, CPL>(X + (int64_t)(ra + row) * ldx + c0, xv);
+                dgd = (double)((const TW*)op.diag)[ra + row];
+              }
+            }
+            if (colok) {
+#pragma unroll
+              for (int e = 0; e < CPL; ++e) acc[e] += (WEIGHTED ? w[u] * x[u][e] : x[u][e]);
+            }
+          }
+        }
+      }
+    }
+    // the last row with entries and any empty rows after it
+    for (;;) {
+      finish(ra + row, acc, xv, dgd);
+      if (++row >= ce) break;
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) acc[e] = T(0);
+      if (colok) {
+        vload<T, CPL>(X + (int64_t)(ra + row) * ldx + c0, xv);
+        dgd = (double)((const TW*)op.diag)[ra + row];
+      }
+    }
+    cr = ce;
+  }
+}
+
 // Kernel A: Y = Op(X) for complete rows; long-row chunk partials to lpart; optional d.v partials.
 // T: vectors; TW: the operator's stored values (diagonal, weights) -- T except for the residual
 // replacement product, which applies the FP32 operator to the FP64 seed iterate in FP64.
-template <typename T, int CPL, bool WEIGHTED, bool DOT, typename TW = T>
+// run > 1: the row units are runs of `run` consecutive rows (run < L; see run_rows_walk).
+template <typename T, int CPL, bool WEIGHTED, bool DOT, typename TW = T, bool RUNS = false>
 __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, const T* __restrict__ X,
                                                   int64_t ldx, T* __restrict__ Y, int64_t ldy, int R,
-                                                  int L, T* __restrict__ lpart, CGState s, int cur) {
+                                                  int L, T* __restrict__ lpart, CGState s, int cur, int run) {
   if (DOT && s.flag[cur] == 0) return;
+  if constexpr (RUNS) {
+    const SegPart sp = seg_part(s.bA, s.S, blockIdx.x, gridDim.x);
+    if (DOT && !seg_live(s, cur, sp.g)) return;
+    if (s.S > 1) shift = s.tau[sp.g * s.p];
+    int64_t r0, r1, x0, x1;
+    seg_range(s.seg_row, s.S, sp.g, op.n, r0, r1);
+    seg_range(s.seg_x, s.S, sp.g, op.n_extra, x0, x1);
+    Geo g(L);
+    const int c0 = g.li * CPL;
+    const bool colok = c0 < R;
+    const int64_t nx = x1 - x0;
+    const int64_t nunits = nx + (r1 - r0 + run - 1) / run;
+    const int gpb = kWaves * g.G;
+    int64_t u = (int64_t)sp.rank * gpb + g.wave * g.G + g.gi;
+    const int64_t stride = (int64_t)sp.nb * gpb;
+    const int CH = op.chunk;
+    const ColGroups cg{s.G > 1 ? s.cgrp : nullptr, s.tau, s.p};
+    DD dacc[CPL];
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) dacc[e] = DD{0.0, 0.0};
+    int pc = -1;
+    // the next run's row extents, loaded one unit ahead
+    int next_ext = 0;
+    auto load_ext = [&](int64_t un) {
+      if (un >= nx && un < nunits) {
+        const int64_t ra = r0 + (un - nx) * run;
+        const int nr = (int)min((int64_t)run, r1 - ra);
+        if (g.li <= nr) next_ext = op.indptr[ra + g.li];
+      }
+    };
+    load_ext(u);
+    for (; u < nunits; u += stride) {
+      const int64_t un = u + stride;
+      if (u >= nx) {
+        const int ext = next_ext;
+        load_ext(un);
+        const int ra = (int)(r0 + (u - nx) * run);
+        const int nr = (int)min((int64_t)run, r1 - ra);
+        run_rows_walk<T, CPL, WEIGHTED, DOT, TW>(op, shift, X, ldx, Y, ldy, g, colok, c0, ra, nr, ext, dacc, pc, cg);
+      } else {
+        load_ext(un);
+        const int2 ex = reinterpret_cast<const int2*>(op.extra)[x0 + u];
+        const int lidx = ex.x, chunk = ex.y;
+        const int row = op.long_rows[lidx];
+        const int rb = op.indptr[row], re = op.indptr[row + 1];
+        const int beg = rb + chunk * CH;
+        const int end = min(re, beg + CH);
+        T acc[CPL];
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) acc[e] = T(0);
+        gather_acc<T, CPL, WEIGHTED, TW>(op.indices, (const TW*)op.weights, X, ldx, beg, end, g, colok, c0, acc);
+        if (colok) vstore<T, CPL>(lpart + (int64_t)(op.long_base[lidx] + chunk) * R + c0, acc);
+      }
+    }
+    if (DOT) block_colsum<CPL>(g, dacc, R, s.parts + (int64_t)blockIdx.x * 2 * R);
+    return;
+  }
   const SegPart sp = seg_part(s.bA, s.S, blockIdx.x, gridDim.x);
   if (DOT && !seg_live(s, cur, sp.g)) return;
   if (s.S > 1) shift = s.tau[sp.g * s.p];
@@ -1310,6 +1488,7 @@ static int g_prof_on = 0;     // 0 off; 1 every class; 2 k_opmul, every iteratio
 
 struct Plan {
   int cpl, L, G;
+  int run;                         // rows per operator work unit (1: one row per unit)
   int grid_a, grid_l, grid_rows;   // blocks of the kernels that write partial sums
   int nb_d, nb_s;                  // k_update: direction part, shifted part (or the window flush)
   int Lf;                          // lanes per row of the shifted part (flush: column slices)
@@ -1356,12 +1535,22 @@ static int64_t op_grid_cap(const polar_op* op) {
   return op->n + op->n_extra >= kLargeRows ? kOpGridLarge : kMaxGrid;
 }
 
+// Rows per operator work unit: op->run_rows (0 = one row per unit).  A run's row extents live in
+// the lanes of its row group, so a run has fewer rows than the group has lanes.
+static int run_for(const polar_op* op, int L) {
+  return std::max(1, std::min(std::max(op->run_rows, 1), L - 1));
+}
+
+// Work units of rows [0, rows) at `run` rows per unit.
+static int64_t row_units(int64_t rows, int run) { return (rows + run - 1) / run; }
+
 static Plan make_plan(const polar_op* op, int64_t R, int cpl, const Segs* sg = nullptr,
                       const std::vector<char>* live = nullptr, int Tw = 1) {
   Plan pl;
   pl.cpl = cpl;
   pl.L = lanes_for(R, cpl);
   pl.G = 64 / pl.L;
+  pl.run = run_for(op, pl.L);
   const int64_t gpb = (int64_t)kWaves * pl.G;
   // blocked update: the shifted part is the window flush, whose tiles are column slices of at
   // most kFlushLanes lanes per row (more rows per tile); otherwise k_update's full-width rows
@@ -1370,7 +1559,7 @@ static Plan make_plan(const polar_op* op, int64_t R, int cpl, const Segs* sg = n
   const int S = sg ? sg->S : 1;
   pl.fin = (int)(S * R);
   if (S == 1) {
-    const int64_t units = op->n + op->n_extra;
+    const int64_t units = row_units(op->n, pl.run) + op->n_extra;
     const int64_t nA = sg ? sg->a[1] : 0;
     const int64_t cap = op->grid_max > 0 ? op->grid_max : kMaxGrid;
     pl.grid_a = (int)std::min<int64_t>(op_grid_cap(op), std::max<int64_t>(1, (units + gpb - 1) / gpb));
@@ -1385,7 +1574,7 @@ static Plan make_plan(const polar_op* op, int64_t R, int cpl, const Segs* sg = n
   for (int g = 0; g < S; ++g) {
     const bool on = !live || (*live)[g];
     ur[g] = on ? sg->row[g + 1] - sg->row[g] : 0;
-    ua[g] = on ? ur[g] + (sg->x[g + 1] - sg->x[g]) : 0;
+    ua[g] = on ? row_units(ur[g], pl.run) + (sg->x[g + 1] - sg->x[g]) : 0;
     ul[g] = on ? sg->l[g + 1] - sg->l[g] : 0;
     us[g] = on ? sg->a[g + 1] - sg->a[g] : 0;
   }
@@ -1439,6 +1628,7 @@ static int check_op(const polar_op* op) {
   if (op->cpl_max < 0 || op->cpl_max > 4 || (op->cpl_max & (op->cpl_max - 1)))
     return set_error(POLAR_E_ARG, "cpl_max must be 0, 1, 2 or 4");
   if (op->grid_max < 0 || op->grid_max > 65536) return set_error(POLAR_E_ARG, "grid_max out of range");
+  if (op->run_rows < 0 || op->run_rows > 63) return set_error(POLAR_E_ARG, "run_rows must be in [0, 63]");
   return 0;
 }
 
@@ -1519,19 +1709,31 @@ static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int
   return cv.off + 256;
 }
 
+// Launch of the operator kernel in the plan's unit shape (one row per unit, or runs of rows).
+template <typename T, int CPL, bool WEIGHTED, bool DOT, typename TW = T>
+static void opmul_go(const polar_op* op, const Plan& pl, double shift, const T* X, int64_t ldx, T* Y, int64_t ldy,
+                     int R, T* lpart, const CGState& s, int cur, hipStream_t st) {
+  if (pl.run > 1)
+    hipLaunchKernelGGL((k_opmul<T, CPL, WEIGHTED, DOT, TW, true>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X,
+                       ldx, Y, ldy, R, pl.L, lpart, s, cur, pl.run);
+  else
+    hipLaunchKernelGGL((k_opmul<T, CPL, WEIGHTED, DOT, TW, false>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X,
+                       ldx, Y, ldy, R, pl.L, lpart, s, cur, 1);
+}
+
 template <typename T, int CPL>
 static int launch_opmul(const polar_op* op, const Plan& pl, double shift, const T* X, int64_t ldx, T* Y,
                         int64_t ldy, int R, T* lpart, const CGState& s, int cur, bool dot, hipStream_t st) {
   if (dot) {
     if (op->weighted)
-      hipLaunchKernelGGL((k_opmul<T, CPL, true, true>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X, ldx, Y, ldy, R, pl.L, lpart, s, cur);
+      opmul_go<T, CPL, true, true>(op, pl, shift, X, ldx, Y, ldy, R, lpart, s, cur, st);
     else
-      hipLaunchKernelGGL((k_opmul<T, CPL, false, true>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X, ldx, Y, ldy, R, pl.L, lpart, s, cur);
+      opmul_go<T, CPL, false, true>(op, pl, shift, X, ldx, Y, ldy, R, lpart, s, cur, st);
   } else {
     if (op->weighted)
-      hipLaunchKernelGGL((k_opmul<T, CPL, true, false>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X, ldx, Y, ldy, R, pl.L, lpart, s, cur);
+      opmul_go<T, CPL, true, false>(op, pl, shift, X, ldx, Y, ldy, R, lpart, s, cur, st);
     else
-      hipLaunchKernelGGL((k_opmul<T, CPL, false, false>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, shift, X, ldx, Y, ldy, R, pl.L, lpart, s, cur);
+      opmul_go<T, CPL, false, false>(op, pl, shift, X, ldx, Y, ldy, R, lpart, s, cur, st);
   }
   POLAR_LAUNCH_CHECK();
   if (op->n_long > 0 && pl.grid_l > 0) {
@@ -1668,9 +1870,9 @@ static int run_iter(const polar_op* op, const Plan& pl, double tau0, const T* ri
   const bool longs = op->n_long > 0 && pl.grid_l > 0;
   if (ev) POLAR_HIP(hipEventRecord(ev[0], st));
   if (op->weighted)
-    hipLaunchKernelGGL((k_opmul<T, CPL, true, true>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, tau0, (const T*)d, (int64_t)s.ld, v, (int64_t)s.ld, R, pl.L, lpart, s, cur);
+    opmul_go<T, CPL, true, true>(op, pl, tau0, (const T*)d, (int64_t)s.ld, v, (int64_t)s.ld, R, lpart, s, cur, st);
   else
-    hipLaunchKernelGGL((k_opmul<T, CPL, false, true>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, tau0, (const T*)d, (int64_t)s.ld, v, (int64_t)s.ld, R, pl.L, lpart, s, cur);
+    opmul_go<T, CPL, false, true>(op, pl, tau0, (const T*)d, (int64_t)s.ld, v, (int64_t)s.ld, R, lpart, s, cur, st);
   POLAR_LAUNCH_CHECK();
   if (ev) POLAR_HIP(hipEventRecord(ev[1], st));
   if (longs) {
@@ -1721,11 +1923,9 @@ static int replace_residual_c(const polar_op* op, const Plan& pl, double tau0, c
   const int R = s.R;
   const double* x0 = s.x0;
   if (op->weighted)
-    hipLaunchKernelGGL((k_opmul<double, C, true, false, T>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, tau0, x0,
-                       (int64_t)s.ld, y64, (int64_t)s.ld, R, pl.L, lpart64, s, cur);
+    opmul_go<double, C, true, false, T>(op, pl, tau0, x0, (int64_t)s.ld, y64, (int64_t)s.ld, R, lpart64, s, cur, st);
   else
-    hipLaunchKernelGGL((k_opmul<double, C, false, false, T>), dim3(pl.grid_a), dim3(kBlock), 0, st, *op, tau0, x0,
-                       (int64_t)s.ld, y64, (int64_t)s.ld, R, pl.L, lpart64, s, cur);
+    opmul_go<double, C, false, false, T>(op, pl, tau0, x0, (int64_t)s.ld, y64, (int64_t)s.ld, R, lpart64, s, cur, st);
   POLAR_LAUNCH_CHECK();
   if (op->n_long > 0 && pl.grid_l > 0) {
     hipLaunchKernelGGL((k_longfin<double, C, false, T>), dim3(pl.grid_l), dim3(kBlock), 0, st, *op, tau0, x0,

@@ -130,6 +130,14 @@ class Graph:
     def nnz(self):
         return int(self._indptr[-1])
 
+    def run_rows(self):
+        """Rows per work unit of the operator kernel: runs of 16 consecutive rows on graphs of 2^20
+        rows and more whose mean degree is at most 8 (lattices, road networks), where a row's few
+        gathers would otherwise half-fill a batch and each row pays its own index round; one row per
+        unit otherwise (the denser and the smaller graphs are faster so).  Results are the same bits
+        either way."""
+        return 16 if self.n >= (1 << 20) and self.nnz <= 8 * self.n else 0
+
     def schedule(self):
         """Long-row split (rows with more than POLAR_CHUNK neighbours), host arrays."""
         if self._sched is None:
@@ -181,6 +189,7 @@ class Op:
         self._struct = None
         self.cpl_max = 0          # launch-shape tuning (polar_amd.autotune); 0 = library default
         self.grid_max = 0
+        self.run_rows = 0         # rows per operator work unit; 0 = Graph.run_rows()
         self.use_graph = False
 
     def add(self, a, b, group=None):
@@ -235,6 +244,7 @@ class Op:
         s.chunk = L.CHUNK
         s.cpl_max = self.cpl_max
         s.grid_max = self.grid_max
+        s.run_rows = self.run_rows if self.run_rows else self.G.run_rows()
         s.long_rows = L.ptr(idx['long_rows']) if s.n_long else None
         s.long_base = L.ptr(idx['long_base']) if s.n_long else None
         s.extra = L.ptr(idx['extra']) if s.n_extra else None
