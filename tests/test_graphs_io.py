@@ -50,3 +50,32 @@ def test_generators_match_their_specs():
     c = configs.make('c1')
     assert c['n'] == 2000 and c['N'] == 5000 and len(c['tau']) == 16 and c['q'] == 32
     assert c['m'] == len(c['edges']) and c['B'] == 1 and c['k'] == 10
+
+
+def test_candidates_fast_is_uniform_over_nonedges():
+    """candidates_fast draws a uniform N-subset of the non-edge pairs, like candidates
+    (realworld/graphs.py:61-69): over 3000 seeds every non-edge is included with probability N/M
+    (M non-edges; chi-square of the inclusion counts at the 0.1% level), no edge or loop is ever
+    drawn, and the first row of the permuted pool is uniform over the non-edges too."""
+    from scipy.stats import chi2
+    n = 16
+    e = graphs.barabasi_albert(n, 2, 3)
+    ekeys = set((e[:, 0] * n + e[:, 1]).tolist())
+    nonedges = [a * n + b for a in range(n) for b in range(a + 1, n) if a * n + b not in ekeys]
+    M, N, trials = len(nonedges), 40, 3000
+    col = {k: i for i, k in enumerate(nonedges)}
+    for fn in (graphs.candidates, graphs.candidates_fast):
+        inc = np.zeros(M)
+        first = np.zeros(M)
+        for seed in range(trials):
+            c = fn(n, e, N, seed)
+            k = c[:, 0] * n + c[:, 1]
+            idx = [col[int(x)] for x in k]           # KeyError if an edge or a loop was drawn
+            inc[idx] += 1
+            first[idx[0]] += 1
+        p = N / M
+        stat = np.sum((inc - trials * p) ** 2 / (trials * p * (1 - p)))
+        assert stat < chi2.ppf(0.999, M - 1), (fn.__name__, stat)
+        exp = trials / M
+        stat1 = np.sum((first - exp) ** 2 / exp)
+        assert stat1 < chi2.ppf(0.999, M - 1), (fn.__name__, stat1)
