@@ -7,9 +7,10 @@
  * convergence checks and (with POLAR_GRAPH) a capture stream; polar_release() frees them.
  * Return value: 0 on success, a positive hipError_t, or a negative POLAR_E_* code; the text of
  * the last error is returned by polar_last_error().  No entry point synchronises the stream
- * except polar_msolve (a 4-byte convergence-flag read every 8 iterations, as the reference's
- * host check at realworld/polar_gpu.py:126) and polar_select_topb / polar_rerank_pick, whose
- * results are the host-side shortlist/winner the reference copies back (polar_gpu.py:221, :236).
+ * except polar_msolve, which reads the convergence flags of each batch of 8 iterations (as the
+ * reference's host check at realworld/polar_gpu.py:126) while the next batch is already queued,
+ * and polar_select_topb / polar_rerank_pick when asked for host copies of their results (the
+ * shortlist and winner the reference copies back, polar_gpu.py:221, :236).
  * Memory that must start at zero is cleared by a fill kernel, never by hipMemsetAsync, so a
  * caller that captures an entry point without a host sync into a HIP graph records no memset
  * node.  Sizes, pointers and dtype codes are checked on the host before anything is launched
@@ -91,7 +92,8 @@ typedef struct polar_op {
   int32_t grid_max;           /* tuning: block budget of the kernels with partial sums (0 = default:
                                  1024 blocks, 16384 for the operator kernel on >= 2^20 rows) */
   int32_t run_rows;           /* tuning: consecutive rows per work unit of the operator kernel (their
-                                 gathers share batches); 0 = default, 1 = one row per unit */
+                                 gathers share batches; capped below the lanes per row); 0 or 1 =
+                                 one row per unit */
   const int32_t* patch_grp;   /* [patch_ptr[n_patch]] column group of each patch entry, or NULL
                                  (every entry applies to every column) */
 } polar_op;
@@ -214,8 +216,9 @@ int polar_proxy(int32_t obj, const double* h, const double* t, const double* tau
                 int32_t p, int64_t N, double sigma, const uint8_t* dead, double* score, void* stream);
 
 /* First Br indices of the stable descending order of score (ties: lower index first), i.e.
- * argsort(-score, stable)[:Br] (polar_gpu.py:221).  Writes device int32[Br] to out and copies them
- * to host_out (synchronising the stream).  ws: polar_select_ws_bytes(N) bytes. */
+ * argsort(-score, stable)[:Br] (polar_gpu.py:221).  Writes device int32[Br] to out and, when
+ * host_out is not NULL, copies them to host_out (synchronising the stream).
+ * ws: polar_select_ws_bytes(N) bytes. */
 int64_t polar_select_ws_bytes(int64_t N);
 int polar_select_topb(const double* score, int64_t N, int32_t Br, int32_t* out, int32_t* host_out,
                       void* ws, int64_t ws_bytes, void* stream);
@@ -226,7 +229,10 @@ int polar_pair_rhs(int32_t dtype, int64_t n, int32_t Br, const int32_t* a, const
 
 /* Re-rank (polar_gpu.py:229-236): bu[j,c] = U[j, sa[c], c] - U[j, sb[c], c];
  * g[c] = sum_j om_j hh[j,c]/(1+bu[j,c]) (obj 0) or log1p(bu[0,c]) (obj 2);
- * winner = first argmax.  Writes g (device double[Br]), copies (winner, g[winner]) to host. */
+ * winner = first argmax.  Writes g (device double[Br + 2]: the scores, then the winner as a double
+ * and its score) and copies (winner, g[winner]) to host_winner / host_value, synchronising the
+ * stream; with both NULL nothing is copied and the stream is not synchronised (several picks can
+ * then share one read-back). */
 int polar_rerank_pick(int32_t obj, int32_t dtype, const void* U, int64_t u_sj, int64_t u_si,
                       int32_t p, int32_t Br, const int32_t* sa, const int32_t* sb, const double* hh,
                       const double* om, double* g, int32_t* host_winner, double* host_value,

@@ -467,12 +467,15 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
     dead = [torch.zeros(N, dtype=torch.uint8, device=dev) for _ in range(Gn)]
     score = torch.empty(N, dtype=torch.float64, device=dev)
     sel_ws = torch.empty(lib.polar_select_ws_bytes(N), dtype=torch.uint8, device=dev)
-    sel_out = torch.empty(64, dtype=torch.int32, device=dev)
+    sel_out = torch.empty((Gn, 64), dtype=torch.int32, device=dev)
     coef = torch.empty((p, 2 * q + 3), dtype=torch.float64, device=dev)
     wX = torch.empty((p, nA), dtype=torch.float64, device=dev)
     wY = torch.empty_like(wX)
     wU = torch.empty_like(wX)
-    gvec = torch.empty(max(B, 1) + 2, dtype=torch.float64, device=dev)
+    gvec = torch.empty((Gn, max(B, 1) + 2), dtype=torch.float64, device=dev)
+    # candidate sketch rows of the shortlists, generated on the GPU (counter-hash kind)
+    gc_dev = torch.empty((Gn, max(B, 1), q), dtype=torch.float64, device=dev) \
+        if isinstance(Gc, pg.RademacherRows) else None
     gc_row = torch.empty(q, dtype=torch.float64, device=dev)
     seqs = [[] for _ in range(Gn)]
     margs = [[] for _ in range(Gn)]
@@ -568,22 +571,28 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
         progress('bank', dict(seconds=time.perf_counter() - t0, spmv=int(spmv.sum())))
 
     for rnd in range(k):
-        shorts = []
+        # every group's proxy and top-B on the device, then one read-back of all shortlists
+        Brs = [min(B, int(live[g].sum())) for g in range(Gn)]
         for g in range(Gn):
-            Br = min(B, int(live[g].sum()))
-            short = np.zeros(Br, dtype=np.int32)
-            if Br:
+            if Brs[g]:
                 L.check(lib.polar_proxy(objc, L.ptr(hs[g]), L.ptr(ts[g]), L.ptr(tau_all[g]), L.ptr(om_d[g]), p, N,
                                         G.sigma, L.ptr(dead[g]), L.ptr(score), stream), 'proxy')
-                L.check(lib.polar_select_topb(L.ptr(score), N, Br, L.ptr(sel_out), short.ctypes.data_as(L.vp),
+                L.check(lib.polar_select_topb(L.ptr(score), N, Brs[g], L.ptr(sel_out[g]), None,
                                               L.ptr(sel_ws), sel_ws.numel(), stream), 'select_topb')
-            shorts.append(short)
-        off = np.r_[0, np.cumsum([len(sh) for sh in shorts])]
+        if gc_dev is not None and any(Brs):
+            keys = sel_out[:, :max(B, 1)].to(torch.int64).reshape(-1)
+            Gc.fill_device(keys, gc_dev.view(-1, q))
+        sel_h = sel_out.cpu().numpy()
+        shorts = [np.ascontiguousarray(sel_h[g, :Brs[g]]) for g in range(Gn)]
+        off = np.r_[0, np.cumsum(Brs)]
         Rt = int(off[-1])
         if Rt == 0:
             break
-        ia = torch.from_numpy(np.concatenate([cand[sh, 0] for sh in shorts])).to(dev)
-        ib = torch.from_numpy(np.concatenate([cand[sh, 1] for sh in shorts])).to(dev)
+        allsh = np.concatenate(shorts)
+        # re-rank right-hand sides and the shortlists' endpoint slots, uploaded together
+        idx = torch.from_numpy(np.stack([cand[allsh, 0], cand[allsh, 1], es_h[allsh, 0], es_h[allsh, 1]])).to(dev)
+        ia, ib = idx[0], idx[1]
+        sab = idx[2:].to(torch.int32)
         ic = torch.arange(Rt, device=dev)
         Wb = torch.zeros((n, Rt), dtype=dtype, device=dev)
         Wb.index_put_((ia, ic), torch.ones(Rt, dtype=dtype, device=dev))
@@ -591,39 +600,43 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
         cg = np.concatenate([np.full(len(sh), g, np.int32) for g, sh in enumerate(shorts)])
         Ub, hh, it = msolve_batched(op, segs, Wb, tau_all, active, tol, track=obj != 'logdet', graph=graph,
                                     block=block, replace=replace, col_group=cg)[:3]
+        # every group's re-rank pick on the device, then one read-back of all winners
         for g in range(Gn):
-            Br = len(shorts[g])
+            if Brs[g]:
+                Ug = Ub[:, :, off[g]:]
+                hg = hh[:, 0, off[g]:off[g + 1]].contiguous() if hh is not None else None
+                L.check(lib.polar_rerank_pick(objc, dcode, L.ptr(Ug), Ug.stride(0), Ug.stride(1), p, Brs[g],
+                                              L.ptr(sab[0, off[g]:]), L.ptr(sab[1, off[g]:]), L.ptr(hg), L.ptr(om_d[g]),
+                                              L.ptr(gvec[g]), None, None, stream), 'rerank_pick')
+        gv_h = gvec.cpu().numpy()
+        for g in range(Gn):
+            Br = Brs[g]
             if Br == 0:
                 continue
             spmv[g] += int(it[0, off[g]:off[g + 1]].sum())
             rhs[g] += Br
             short = shorts[g]
-            Ug = Ub[:, :, off[g]:]
-            sa = torch.from_numpy(es_h[short, 0].astype(np.int32)).to(dev)
-            sb = torch.from_numpy(es_h[short, 1].astype(np.int32)).to(dev)
-            hg = hh[:, 0, off[g]:off[g + 1]].contiguous() if hh is not None else None
-            win = np.zeros(1, dtype=np.int32)
-            val = np.zeros(1, dtype=np.float64)
-            L.check(lib.polar_rerank_pick(objc, dcode, L.ptr(Ug), Ug.stride(0), Ug.stride(1), p, Br, L.ptr(sa),
-                                          L.ptr(sb), L.ptr(hg), L.ptr(om_d[g]), L.ptr(gvec),
-                                          win.ctypes.data_as(L.vp), val.ctypes.data_as(L.vp), stream), 'rerank_pick')
-            j = 0 if Br == 1 else int(win[0])
+            j = 0 if Br == 1 else int(gv_h[g, Br])
             e = int(short[j])
-            u = Ug[:, :, j]
-            margs[g].append(float(val[0]))
+            u = Ub[:, :, off[g] + j]
+            margs[g].append(float(gv_h[g, Br + 1]))
             seqs[g].append(e)
             live[g][e] = False
             dead[g][e] = 1
             aa, bb = (int(x) for x in es_h[e])
             a, b = (int(x) for x in cand[e])
-            if isinstance(Gc, torch.Tensor):
-                gc_row.copy_(Gc[e])
+            if gc_dev is not None:
+                gcr = gc_dev[g, j]
             else:
-                gc_row.copy_(torch.from_numpy(np.ascontiguousarray(Gc[e], dtype=np.float64)))
+                gcr = gc_row
+                if isinstance(Gc, torch.Tensor):
+                    gc_row.copy_(Gc[e])
+                else:
+                    gc_row.copy_(torch.from_numpy(np.ascontiguousarray(Gc[e], dtype=np.float64)))
             if rnd + 1 < k:
                 Xg, Yg = Xs[g], Ys[g]
                 L.check(lib.polar_maint_prep(dcode, L.ptr(Xg), L.ptr(Yg), Xg.stride(0), Xg.stride(1), p, q, L.ptr(u),
-                                             u.stride(0), u.stride(1), aa, bb, L.ptr(gc_row), L.ptr(coef), stream),
+                                             u.stride(0), u.stride(1), aa, bb, L.ptr(gcr), L.ptr(coef), stream),
                         'maint_prep')
                 L.check(lib.polar_maint_rank1(dcode, L.ptr(Xg), L.ptr(Yg), Xg.stride(0), Xg.stride(1), p, nA, q,
                                               L.ptr(u), u.stride(0), u.stride(1), L.ptr(coef), L.ptr(wX), L.ptr(wY),
@@ -633,7 +646,7 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
                 if want_trace:
                     tr_hist[g].append(coef[:, 2 * q:2 * q + 2].clone())
             Fv = RHS[:, g * W2 + q:(g + 1) * W2]
-            L.check(lib.polar_rows_axpy(dcode, L.ptr(Fv), Fv.stride(0), q, a, b, L.ptr(gc_row), stream), 'rows_axpy')
+            L.check(lib.polar_rows_axpy(dcode, L.ptr(Fv), Fv.stride(0), q, a, b, L.ptr(gcr), stream), 'rows_axpy')
             op.add(a, b, group=g)
         if progress is not None:
             progress('round', dict(round=rnd, edges=[sq[-1] if sq else None for sq in seqs],

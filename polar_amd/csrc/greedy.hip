@@ -742,6 +742,7 @@ int polar_rerank_pick(int32_t obj, int32_t dtype, const void* U, int64_t u_sj, i
   else
     hipLaunchKernelGGL(k_rerank<float>, dim3(1), dim3(kBlock), 0, st, obj, (const float*)U, u_sj, u_si, p, Br, sa, sb, hh, om, g, res);
   POLAR_LAUNCH_CHECK();
+  if (!host_winner && !host_value) return 0;     // the caller reads g[Br], g[Br + 1] itself
   double hr[2];
   POLAR_HIP(hipMemcpyAsync(hr, res, sizeof(hr), hipMemcpyDeviceToHost, st));
   POLAR_HIP(hipStreamSynchronize(st));

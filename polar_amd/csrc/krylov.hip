@@ -305,6 +305,9 @@ struct CGState {
   // Column groups (one segment): G groups, cgrp[slot] = group of the column in that slot.
   int G;
   int* cgrp;
+  // The operator kernels take the seed shift from tau[g * p] in device memory (segment g, 0 for one
+  // system) instead of their `shift` argument: a solve never reads tau back to the host.
+  int tau_dev;
 };
 
 // Segment of block b of a kernel whose blocks are partitioned by the prefix table `pref`.
@@ -672,7 +675,7 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
   if constexpr (RUNS) {
     const SegPart sp = seg_part(s.bA, s.S, blockIdx.x, gridDim.x);
     if (DOT && !seg_live(s, cur, sp.g)) return;
-    if (s.S > 1) shift = s.tau[sp.g * s.p];
+    if (s.S > 1 || s.tau_dev) shift = s.tau[sp.g * s.p];
     int64_t r0, r1, x0, x1;
     seg_range(s.seg_row, s.S, sp.g, op.n, r0, r1);
     seg_range(s.seg_x, s.S, sp.g, op.n_extra, x0, x1);
@@ -728,7 +731,7 @@ __global__ __launch_bounds__(kBlock) void k_opmul(polar_op op, double shift, con
   }
   const SegPart sp = seg_part(s.bA, s.S, blockIdx.x, gridDim.x);
   if (DOT && !seg_live(s, cur, sp.g)) return;
-  if (s.S > 1) shift = s.tau[sp.g * s.p];
+  if (s.S > 1 || s.tau_dev) shift = s.tau[sp.g * s.p];
   int64_t r0, r1, x0, x1;
   seg_range(s.seg_row, s.S, sp.g, op.n, r0, r1);
   seg_range(s.seg_x, s.S, sp.g, op.n_extra, x0, x1);
@@ -824,7 +827,7 @@ __global__ __launch_bounds__(kBlock) void k_longfin(polar_op op, double shift, c
   if (DOT && s.flag[cur] == 0) return;
   const SegPart sp = seg_part(s.bL, s.S, blockIdx.x, gridDim.x);
   if (DOT && !seg_live(s, cur, sp.g)) return;
-  if (s.S > 1) shift = s.tau[sp.g * s.p];
+  if (s.S > 1 || s.tau_dev) shift = s.tau[sp.g * s.p];
   int64_t l0, l1;
   seg_range(s.seg_l, s.S, sp.g, op.n_long, l0, l1);
   Geo g(L);
@@ -2101,6 +2104,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   s.S = S;
   s.sld = (int)(S * R);
   s.tau = tau;
+  s.tau_dev = 1;
   s.G = sg.G;
   if (sg.G > 1)
     POLAR_HIP(hipMemcpyAsync(s.cgrp, sg.col_group.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
@@ -2178,11 +2182,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
     POLAR_LAUNCH_CHECK();
   }
 
-  double tau0 = 0.0;          // one system: the seed shift is a kernel argument (else read from tau)
-  if (S == 1 && sg.G == 1) {
-    POLAR_HIP(hipMemcpyAsync(&tau0, tau, sizeof(double), hipMemcpyDeviceToHost, st));
-    POLAR_HIP(hipStreamSynchronize(st));
-  }
+  const double tau0 = 0.0;    // unused: the operator kernels read the seed shift from tau (tau_dev)
 
   const bool prof = g_prof_on != 0;
   const bool use_graph = (flags & POLAR_GRAPH) && !prof;
@@ -2193,30 +2193,53 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
     cs = capture_stream(rc);
     if (rc) return hip_fail((hipError_t)rc, "capture stream", __LINE__);
   }
-  std::vector<hipEvent_t> ev;
+  std::vector<hipEvent_t> ev;    // profiling: kEv events per iteration, for the two batches in flight
   std::vector<float> kms;        // per iteration: per-kernel milliseconds
   std::vector<int> widths;       // per iteration: processed width
   std::vector<char> timed;       // per iteration: its events were recorded (mode 3 samples 1 in 8)
   const int esz = (int)sizeof(T);
   const int align = cpl0;        // compacted widths stay multiples of the initial vector width
   std::vector<int> permh(R);
-  int* stat = nullptr;               // pinned: [flag parity 0, flag parity 1, act...]
+  // Two batches of kBatch iterations are in flight: batch b+1 is queued before batch b's convergence
+  // status is read, so the GPU does not wait for the host at every check.  Launches after
+  // convergence do no work (every kernel of an iteration tests the any-active flag first), so the
+  // extra batch costs only its launches.  Each batch copies its status into its own half of a
+  // pinned buffer: [flag parity 0, flag parity 1, per-column active flags in the batch's slot order].
+  const int64_t stat_n = 2 + (int64_t)S * R;
+  int* statbuf = nullptr;
   {
     int rc = 0;
-    stat = status_buffer(2 + (int64_t)S * R, rc);
+    statbuf = status_buffer(2 * stat_n, rc);
     if (rc) return hip_fail((hipError_t)rc, "hipHostMalloc", __LINE__);
   }
-  const int* acth = stat + 2;
-  hipEvent_t evf[2] = {nullptr, nullptr};   // profiling of the window flushes
+  struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};       // a batch's status copy has landed
+    hipEvent_t f[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // profiling of the window flushes
+    ~Events() {
+      for (auto x : e)
+        if (x) (void)hipEventDestroy(x);
+      for (auto& y : f)
+        for (auto x : y)
+          if (x) (void)hipEventDestroy(x);
+    }
+  } evs;
+  for (auto& x : evs.e) POLAR_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
   float flush_ms = 0.f;
   auto ring_of = [&]() {
     Ring<T> rg;
     for (int t = 0; t < kW; ++t) rg.s[t] = slot[t];
     return rg;
   };
-  bool converged = false;
-  for (int64_t it0 = 0; it0 < maxit; it0 += kBatch) {
-    const int nit = (int)std::min<int64_t>(kBatch, maxit - it0);
+  struct Batch {
+    int64_t it0;      // first iteration
+    int nit;          // iterations (kBatch, fewer at the cap)
+    int buf;          // status half and event set
+    int R_at;         // slot width when queued (the layout of its status flags)
+    size_t tbase;     // profiling: index of its first iteration in `timed`
+  };
+  auto launch_batch = [&](const Batch& b) -> int {
+    const int64_t it0 = b.it0;
+    const int nit = b.nit;
     if (use_graph && nit == kBatch) {
       if (!graph.x) {
         POLAR_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
@@ -2241,15 +2264,15 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
         const int64_t it = it0 + k;
         hipEvent_t* evp = nullptr;
         if (prof) {
-          const int slot = k * kEv;
-          while ((int)ev.size() < slot + kEv) {
+          const int eslot = (b.buf * kBatch + k) * kEv;
+          while ((int)ev.size() < eslot + kEv) {
             hipEvent_t e;
             POLAR_HIP(hipEventCreate(&e));
             ev.push_back(e);
           }
           // mode 3: one iteration in every 8, at a position that rotates from one group of 8 to the next
           const bool take = g_prof_on != 3 || (k % 8) == (int)(((it0 + k) / 8) % 8);
-          evp = take ? ev.data() + slot : nullptr;
+          evp = take ? ev.data() + eslot : nullptr;
           widths.push_back(s.R);
           timed.push_back(take ? 1 : 0);
         }
@@ -2260,113 +2283,147 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
       if (Tw > 1) {
         const bool fprof = prof && g_prof_on == 1;
         if (fprof) {
-          for (auto& e : evf)
+          for (auto& e : evs.f[b.buf])
             if (!e) POLAR_HIP(hipEventCreate(&e));
-          POLAR_HIP(hipEventRecord(evf[0], st));
+          POLAR_HIP(hipEventRecord(evs.f[b.buf][0], st));
         }
         if (int rc = flush_window<T>(cpl, pl, ring_of(), active, nA, p, P, Xo, sj, si, s, n, d, m.dsave, st)) return rc;
-        if (fprof) POLAR_HIP(hipEventRecord(evf[1], st));
+        if (fprof) POLAR_HIP(hipEventRecord(evs.f[b.buf][1], st));
       }
-    }
-    POLAR_HIP(hipMemcpyAsync(stat, s.flag, sizeof(int) * (2 + (S > 1 ? S * R : s.R)), hipMemcpyDeviceToHost, st));
-    POLAR_HIP(hipStreamSynchronize(st));
-    const int flag = stat[(it0 + nit) & 1];
-    if (prof) {
-      const size_t t0 = timed.size() - (size_t)nit;
-      for (int k = 0; k < nit; ++k)
-        for (int c = 0; c < kEv - 1; ++c) {
-          float ms = 0.f;
-          if (timed[t0 + k] && (c == 0 || g_prof_on == 1))
-            POLAR_HIP(hipEventElapsedTime(&ms, ev[k * kEv + c], ev[k * kEv + c + 1]));
-          kms.push_back(ms);
-        }
-      if (Tw > 1 && g_prof_on == 1) {
-        float ms = 0.f;
-        POLAR_HIP(hipEventElapsedTime(&ms, evf[0], evf[1]));
-        flush_ms += ms;
-      }
-    }
-    if (!flag) {
-      converged = true;
-      break;
     }
     if (repl && nit == kBatch && (it0 + kBatch) % kReplaceEvery == 0) {
-      // the current residual is slot[0] after a full window (Tw = kW) or the only vector (Tw = 1)
+      // the current residual is slot[0] after a full window (Tw = kW) or the only vector (Tw = 1);
+      // after convergence the replacement kernels return at once, like the iterations
       if (int rc = replace_residual<T>(cpl, op, pl, tau0, W, ldw, slot[0], m.y64, m.lpart64, s,
                                        (int)((it0 + kBatch) & 1), st))
         return rc;
     }
-    if (S > 1) {
-      // Re-plan over the segments still iterating: finished segments get no blocks, the others
-      // share the whole budget (their kernels skip finished segments either way).
-      bool changed = false;
-      for (int g = 0; g < S; ++g) {
-        char on = 0;
-        for (int c = 0; c < R; ++c) on |= acth[g * R + c] != 0;
-        changed |= on != live[g];
-        live[g] = on;
+    POLAR_HIP(hipMemcpyAsync(statbuf + b.buf * stat_n, s.flag, sizeof(int) * (2 + (S > 1 ? S * R : s.R)),
+                             hipMemcpyDeviceToHost, st));
+    POLAR_HIP(hipEventRecord(evs.e[b.buf], st));
+    return 0;
+  };
+  bool converged = false;
+  bool inflight = false;       // a queued batch whose status has not been read
+  int inflight_buf = 0;
+  if (maxit > 0) {
+    Batch cur{0, (int)std::min<int64_t>(kBatch, maxit), 0, s.R, timed.size()};
+    if (int rc = launch_batch(cur)) return rc;
+    for (;;) {
+      const int64_t it1 = cur.it0 + cur.nit;
+      const bool have_next = it1 < maxit;
+      Batch nxt{it1, (int)std::min<int64_t>(kBatch, maxit - it1), cur.buf ^ 1, s.R, timed.size()};
+      if (have_next) {
+        if (int rc = launch_batch(nxt)) return rc;
       }
-      if (changed) {
-        pl = make_plan(op, R, cpl, &sg, &live, Tw);
-        s.nb_a = pl.grid_a;
-        POLAR_HIP(hipMemcpyAsync(m.dtab, pl.tab.data(), sizeof(int) * pl.tab.size(), hipMemcpyHostToDevice, st));
-        graph.drop();
+      inflight = have_next;
+      inflight_buf = nxt.buf;
+      POLAR_HIP(hipEventSynchronize(evs.e[cur.buf]));
+      const int* stat = statbuf + cur.buf * stat_n;
+      const int flag = stat[(cur.it0 + cur.nit) & 1];
+      if (prof) {
+        for (int k = 0; k < cur.nit; ++k)
+          for (int c = 0; c < kEv - 1; ++c) {
+            float ms = 0.f;
+            const int eslot = (cur.buf * kBatch + k) * kEv;
+            if (timed[cur.tbase + k] && (c == 0 || g_prof_on == 1))
+              POLAR_HIP(hipEventElapsedTime(&ms, ev[eslot + c], ev[eslot + c + 1]));
+            kms.push_back(ms);
+          }
+        if (Tw > 1 && g_prof_on == 1 && evs.f[cur.buf][0]) {
+          float ms = 0.f;
+          POLAR_HIP(hipEventElapsedTime(&ms, evs.f[cur.buf][0], evs.f[cur.buf][1]));
+          flush_ms += ms;
+        }
       }
-      continue;
-    }
-    // Active-column compaction (one segment): when at most half of the current slots are still
-    // active, move them (padded to the vector width with finished slots) to the front.  Columns
-    // are independent, so every column's arithmetic is unchanged.
-    int count = 0;
-    for (int c = 0; c < s.R; ++c) count += acth[c] != 0;
-    const int Rn = std::min(s.R, (int)align_up(std::max(count, 1), align));
-    if (Rn <= s.R / 2) {
-      int k = 0;
-      for (int c = 0; c < s.R; ++c)
-        if (acth[c]) permh[k++] = c;
-      for (int c = 0; c < s.R && k < Rn; ++c)
-        if (!acth[c]) permh[k++] = c;
-      POLAR_HIP(hipMemcpyAsync(perm, permh.data(), sizeof(int) * Rn, hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL(k_writeback, dim3((p * s.R + 255) / 256), dim3(256), 0, st, s, (int)R, h_out, te_out, iters_out);
-      POLAR_LAUNCH_CHECK();
-      // the current residual is slot[0] (a full window ends with r_8 there; Tw = 1: the residual)
-      const int gv = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (n * Rn + kBlock - 1) / kBlock));
-      hipLaunchKernelGGL(k_compact_vec<T>, dim3(gv), dim3(kBlock), 0, st, n, (int64_t)R, Rn, (const int*)perm, (const T*)slot[0], v);
-      POLAR_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_compact_vec<T>, dim3(gv), dim3(kBlock), 0, st, n, (int64_t)R, Rn, (const int*)perm, (const T*)d, t2);
-      POLAR_LAUNCH_CHECK();
-      {
-        T* old0 = slot[0];
-        for (int t = 0; t < kW; ++t)
-          if (slot[t] == old0) slot[t] = v;
-        v = old0;
+      if (!flag) {
+        converged = true;
+        break;
       }
-      std::swap(d, t2);
-      if (repl)
-        if (int e = compact_rows<double>(n, R, Rn, perm, s.x0, st)) return e;
-      if (m.dsave)
-        if (int e = compact_rows<T>(n, R, Rn, perm, m.dsave, st)) return e;
-      if (int e = compact_rows<T>((int64_t)p * nA, R, Rn, perm, P, st)) return e;
-      for (double* a : {s.rr, s.stop, s.a, s.be, s.ap, s.bp})
-        if (int e = compact_rows<double>(1, R, Rn, perm, a, st)) return e;
-      for (int* a : {s.act, s.dmask})
-        if (int e = compact_rows<int>(1, R, Rn, perm, a, st)) return e;
-      if (int e = compact_rows<int64_t>(1, R, Rn, perm, s.iters, st)) return e;
-      for (double* a : {s.z, s.zp, s.zn, s.aj, s.bj, s.h, s.ip, s.pn, s.te, s.tg})
-        if (int e = compact_rows<double>(p, R, Rn, perm, a, st)) return e;
-      for (int* a : {s.live, s.m})
-        if (int e = compact_rows<int>(p, R, Rn, perm, a, st)) return e;
-      if (int e = compact_rows<int>(1, R, Rn, perm, s.colmap, st)) return e;
-      if (sg.G > 1)
-        if (int e = compact_rows<int>(1, R, Rn, perm, s.cgrp, st)) return e;
-      s.R = Rn;
-      s.ident = 0;
-      cpl = cpl_for<T>(Rn, R, cpl0);
-      pl = make_plan(op, Rn, cpl, &sg, &live, Tw);
-      s.nb_a = pl.grid_a;
-      graph.drop();
+      if (!have_next) break;
+      // Re-planning and compaction follow the per-column flags of `cur` (a superset of the columns
+      // still active after `nxt`), in the slot order `cur` was queued with; they are queued after
+      // `nxt`, which must be a whole batch (the current residual is then slot[0]).
+      const int* acth = stat + 2;
+      const bool may_change = cur.R_at == s.R && nxt.nit == kBatch;
+      if (may_change && S > 1) {
+        // Re-plan over the segments still iterating: finished segments get no blocks, the others
+        // share the whole budget (their kernels skip finished segments either way).
+        bool changed = false;
+        for (int g = 0; g < S; ++g) {
+          char on = 0;
+          for (int c = 0; c < R; ++c) on |= acth[g * R + c] != 0;
+          changed |= on != live[g];
+          live[g] = on;
+        }
+        if (changed) {
+          if (use_graph) POLAR_HIP(hipStreamSynchronize(st));   // the batch in flight may be a graph launch
+          pl = make_plan(op, R, cpl, &sg, &live, Tw);
+          s.nb_a = pl.grid_a;
+          POLAR_HIP(hipMemcpyAsync(m.dtab, pl.tab.data(), sizeof(int) * pl.tab.size(), hipMemcpyHostToDevice, st));
+          graph.drop();
+        }
+      } else if (may_change) {
+        // Active-column compaction (one segment): when at most half of the current slots are still
+        // active, move them (padded to the vector width with finished slots) to the front.  Columns
+        // are independent, so every column's arithmetic is unchanged.
+        int count = 0;
+        for (int c = 0; c < s.R; ++c) count += acth[c] != 0;
+        const int Rn = std::min(s.R, (int)align_up(std::max(count, 1), align));
+        if (Rn <= s.R / 2) {
+          if (use_graph) POLAR_HIP(hipStreamSynchronize(st));   // the batch in flight may be a graph launch
+          int k = 0;
+          for (int c = 0; c < s.R; ++c)
+            if (acth[c]) permh[k++] = c;
+          for (int c = 0; c < s.R && k < Rn; ++c)
+            if (!acth[c]) permh[k++] = c;
+          POLAR_HIP(hipMemcpyAsync(perm, permh.data(), sizeof(int) * Rn, hipMemcpyHostToDevice, st));
+          hipLaunchKernelGGL(k_writeback, dim3((p * s.R + 255) / 256), dim3(256), 0, st, s, (int)R, h_out, te_out, iters_out);
+          POLAR_LAUNCH_CHECK();
+          // the current residual is slot[0] (a full window ends with r_8 there; Tw = 1: the residual)
+          const int gv = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (n * Rn + kBlock - 1) / kBlock));
+          hipLaunchKernelGGL(k_compact_vec<T>, dim3(gv), dim3(kBlock), 0, st, n, (int64_t)R, Rn, (const int*)perm, (const T*)slot[0], v);
+          POLAR_LAUNCH_CHECK();
+          hipLaunchKernelGGL(k_compact_vec<T>, dim3(gv), dim3(kBlock), 0, st, n, (int64_t)R, Rn, (const int*)perm, (const T*)d, t2);
+          POLAR_LAUNCH_CHECK();
+          {
+            T* old0 = slot[0];
+            for (int t = 0; t < kW; ++t)
+              if (slot[t] == old0) slot[t] = v;
+            v = old0;
+          }
+          std::swap(d, t2);
+          if (repl)
+            if (int e = compact_rows<double>(n, R, Rn, perm, s.x0, st)) return e;
+          if (m.dsave)
+            if (int e = compact_rows<T>(n, R, Rn, perm, m.dsave, st)) return e;
+          if (int e = compact_rows<T>((int64_t)p * nA, R, Rn, perm, P, st)) return e;
+          for (double* a : {s.rr, s.stop, s.a, s.be, s.ap, s.bp})
+            if (int e = compact_rows<double>(1, R, Rn, perm, a, st)) return e;
+          for (int* a : {s.act, s.dmask})
+            if (int e = compact_rows<int>(1, R, Rn, perm, a, st)) return e;
+          if (int e = compact_rows<int64_t>(1, R, Rn, perm, s.iters, st)) return e;
+          for (double* a : {s.z, s.zp, s.zn, s.aj, s.bj, s.h, s.ip, s.pn, s.te, s.tg})
+            if (int e = compact_rows<double>(p, R, Rn, perm, a, st)) return e;
+          for (int* a : {s.live, s.m})
+            if (int e = compact_rows<int>(p, R, Rn, perm, a, st)) return e;
+          if (int e = compact_rows<int>(1, R, Rn, perm, s.colmap, st)) return e;
+          if (sg.G > 1)
+            if (int e = compact_rows<int>(1, R, Rn, perm, s.cgrp, st)) return e;
+          s.R = Rn;
+          s.ident = 0;
+          cpl = cpl_for<T>(Rn, R, cpl0);
+          pl = make_plan(op, Rn, cpl, &sg, &live, Tw);
+          s.nb_a = pl.grid_a;
+          graph.drop();
+        }
+      }
+      cur = nxt;
     }
   }
+  // the batch queued after the last check (it does no work after convergence) has landed its
+  // status before the pinned buffer is used again
+  if (inflight) POLAR_HIP(hipEventSynchronize(evs.e[inflight_buf]));
   if (prof) {
     // count only the iterations that did work (launches after convergence return at once)
     std::vector<int64_t> ih((size_t)S * s.R);
@@ -2405,7 +2462,8 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
       // direction update per active column-iteration; per window the flush reads the T residuals
       // of every column slot and reads and writes X, P of the (pole, column) pairs it changed
       double slots = 0.0;
-      for (int w : widths) slots += (double)w * S;
+      for (int64_t k = 0; k < (int64_t)(kms.size() / (kEv - 1)) && k < (int64_t)widths.size(); ++k)
+        slots += (double)widths[k] * S;     // iterations whose status was read
       g_prof[5].bytes += esz * 3.0 * nrows * (double)st2[0] + esz * narows * slots + 4.0 * esz * narows * (double)st2[2];
       g_prof[5].ms += flush_ms;
     } else {
@@ -2415,8 +2473,6 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
     g_prof[6].bytes += (double)st2[1];
     for (auto e : ev) (void)hipEventDestroy(e);
   }
-  for (auto e : evf)
-    if (e) (void)hipEventDestroy(e);
   hipLaunchKernelGGL(k_writeback, dim3((p * S * s.R + 255) / 256), dim3(256), 0, st, s, (int)R, h_out, te_out, iters_out);
   POLAR_LAUNCH_CHECK();
   if (!converged) return set_error(POLAR_E_ITERCAP, "multi-shift CG iteration cap");

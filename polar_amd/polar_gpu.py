@@ -429,6 +429,14 @@ class RademacherRows:
     def full(self):
         return _rad_rows(self.seed, 2, np.arange(self.N), self.q) / math.sqrt(self.q)
 
+    def fill_device(self, keys, out):
+        """out[i] = Gc[keys[i]] on the GPU (FP64; keys: device int64 candidate indices, out: device
+        (len(keys), q) with unit column stride), the same values as __getitem__."""
+        lib = L.require_gpu()
+        L.check(lib.polar_rademacher(L.POLAR_F64, self.seed, 2, L.ptr(keys), len(keys), self.q,
+                                     1.0 / math.sqrt(self.q), L.ptr(out), out.stride(0), L.stream_ptr()),
+                'polar_rademacher')
+
 
 class DeviceSketch:
     """A counter-hash Rademacher sketch (Z, F, Gc) that is generated on the GPU where polar() needs
