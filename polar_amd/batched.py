@@ -239,12 +239,16 @@ def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac'
     dead = [torch.zeros(N, dtype=torch.uint8, device=dev) for N in Ns]
     score = [torch.empty(N, dtype=torch.float64, device=dev) for N in Ns]
     sel_ws = torch.empty(lib.polar_select_ws_bytes(max(Ns)), dtype=torch.uint8, device=dev)
-    sel_out = torch.empty(64, dtype=torch.int32, device=dev)
+    sel_out = torch.empty((S, 64), dtype=torch.int32, device=dev)
     coef = torch.empty((p, 2 * q + 3), dtype=torch.float64, device=dev)
     wX = torch.empty((p, max(len(a) for a in actives)), dtype=torch.float64, device=dev)
     wY = torch.empty_like(wX)
     wU = torch.empty_like(wX)
-    gvec = torch.empty(max(B, 1) + 2, dtype=torch.float64, device=dev)
+    gvec = torch.empty((S, max(B, 1) + 2), dtype=torch.float64, device=dev)
+    # candidate sketch rows of the shortlists of the graphs whose sketch is the counter-hash kind,
+    # generated on the GPU
+    gc_dev = torch.empty((S, max(B, 1), q), dtype=torch.float64, device=dev) \
+        if any(isinstance(Gc, pg.RademacherRows) for Gc in Gcs) else None
     gc_row = torch.empty(q, dtype=torch.float64, device=dev)
     seqs = [[] for _ in range(S)]
     margs = [[] for _ in range(S)]
@@ -285,71 +289,78 @@ def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac'
         progress('bank', dict(seconds=time.perf_counter() - t0, spmv=int(spmv.sum())))
 
     for rnd in range(k):
-        shorts = []
+        # every graph's proxy and top-B on the device, then one read-back of all shortlists
+        Brs = [min(B, int(live[g].sum())) for g in range(S)]
         for g in range(S):
+            if Brs[g] == 0:                   # candidates exhausted: this graph stops selecting
+                continue
             L.check(lib.polar_proxy(objc, L.ptr(hs[g]), L.ptr(ts[g]), L.ptr(tau_all[g]), L.ptr(om_d[g]), p, Ns[g],
                                     graphs[g].sigma, L.ptr(dead[g]), L.ptr(score[g]), stream), 'proxy')
-            Br = min(B, int(live[g].sum()))
-            short = np.zeros(Br, dtype=np.int32)
-            if Br == 0:                       # candidates exhausted: this graph stops selecting
-                shorts.append(short)
-                continue
-            L.check(lib.polar_select_topb(L.ptr(score[g]), Ns[g], Br, L.ptr(sel_out), short.ctypes.data_as(L.vp),
+            L.check(lib.polar_select_topb(L.ptr(score[g]), Ns[g], Brs[g], L.ptr(sel_out[g]), None,
                                           L.ptr(sel_ws), sel_ws.numel(), stream), 'select_topb')
-            shorts.append(short)
-        Bm = max(len(sh) for sh in shorts)
+            if isinstance(Gcs[g], pg.RademacherRows):
+                Gcs[g].fill_device(sel_out[g, :Brs[g]].to(torch.int64), gc_dev[g])
+        Bm = max(Brs)
         if Bm == 0:
             break
-        rows_a, rows_b, cols = [], [], []
+        sel_h = sel_out.cpu().numpy()
+        shorts = [np.ascontiguousarray(sel_h[g, :Brs[g]]) for g in range(S)]
+        # re-rank right-hand sides (union rows, column = position in the graph's shortlist) and the
+        # shortlists' endpoint slots, uploaded together
+        up = []
         for g in range(S):
             c = cands[g][shorts[g]]
-            rows_a.append(c[:, 0] + row0[g])
-            rows_b.append(c[:, 1] + row0[g])
-            cols.append(np.arange(len(shorts[g])))
-        ia = torch.from_numpy(np.concatenate(rows_a)).to(dev)
-        ib = torch.from_numpy(np.concatenate(rows_b)).to(dev)
-        ic = torch.from_numpy(np.concatenate(cols)).to(dev)
+            up.append(np.stack([c[:, 0] + row0[g], c[:, 1] + row0[g], np.arange(Brs[g]),
+                                es[g][shorts[g], 0], es[g][shorts[g], 1]]))
+        idx = torch.from_numpy(np.concatenate(up, axis=1).astype(np.int64)).to(dev)
+        ia, ib, ic = idx[0], idx[1], idx[2]
+        sab = idx[3:].to(torch.int32)
+        off = np.r_[0, np.cumsum(Brs)]
         Wb = torch.zeros((n, Bm), dtype=dtype, device=dev)
         Wb.index_put_((ia, ic), torch.ones(len(ic), dtype=dtype, device=dev))
         Wb.index_put_((ib, ic), torch.full((len(ic),), -1.0, dtype=dtype, device=dev))
         Ub, hh, it = msolve_batched(op, segs, Wb, tau_all, active, tol, track=obj != 'logdet', graph=graph,
                                     block=block, replace=replace)[:3]
         spmv += it.sum(axis=1)
+        # every graph's re-rank pick on the device, then one read-back of all winners
         for g in range(S):
-            Br = len(shorts[g])
+            if Brs[g]:
+                Ug = Ub[:, a0[g]:]
+                hg = hh[:, g, :Brs[g]].contiguous() if hh is not None else None
+                L.check(lib.polar_rerank_pick(objc, dcode, L.ptr(Ug), Ug.stride(0), Ug.stride(1), p, Brs[g],
+                                              L.ptr(sab[0, off[g]:]), L.ptr(sab[1, off[g]:]), L.ptr(hg),
+                                              L.ptr(om_d[g]), L.ptr(gvec[g]), None, None, stream), 'rerank_pick')
+        gv_h = gvec.cpu().numpy()
+        for g in range(S):
+            Br = Brs[g]
             if Br == 0:
                 continue
             rhs[g] += Br
             short = shorts[g]
             Ug = Ub[:, a0[g]:]
-            sa = torch.from_numpy(es[g][short, 0].astype(np.int32)).to(dev)
-            sb = torch.from_numpy(es[g][short, 1].astype(np.int32)).to(dev)
-            hg = hh[:, g, :Br].contiguous() if hh is not None else None
-            win = np.zeros(1, dtype=np.int32)
-            val = np.zeros(1, dtype=np.float64)
-            L.check(lib.polar_rerank_pick(objc, dcode, L.ptr(Ug), Ug.stride(0), Ug.stride(1), p, Br, L.ptr(sa),
-                                          L.ptr(sb), L.ptr(hg), L.ptr(om_d[g]), L.ptr(gvec),
-                                          win.ctypes.data_as(L.vp), val.ctypes.data_as(L.vp), stream),
-                    'rerank_pick')
-            j = 0 if Br == 1 else int(win[0])
+            j = 0 if Br == 1 else int(gv_h[g, Br])
             e = int(short[j])
             u = Ug[:, :, j]
-            margs[g].append(float(val[0]))
+            margs[g].append(float(gv_h[g, Br + 1]))
             seqs[g].append(e)
             live[g][e] = False
             dead[g][e] = 1
             aa, bb = (int(x) for x in es[g][e])
             a, b = (int(x) for x in cands[g][e])
             Gc = Gcs[g]
-            if isinstance(Gc, torch.Tensor):
-                gc_row.copy_(Gc[e])
+            if isinstance(Gc, pg.RademacherRows):
+                gcr = gc_dev[g, j]
             else:
-                gc_row.copy_(torch.from_numpy(np.ascontiguousarray(Gc[e], dtype=np.float64)))
+                gcr = gc_row
+                if isinstance(Gc, torch.Tensor):
+                    gc_row.copy_(Gc[e])
+                else:
+                    gc_row.copy_(torch.from_numpy(np.ascontiguousarray(Gc[e], dtype=np.float64)))
             if rnd + 1 < k:
                 Xg, Yg = X[:, a0[g]:], Y[:, a0[g]:]
                 nAg = int(a0[g + 1] - a0[g])
                 L.check(lib.polar_maint_prep(dcode, L.ptr(Xg), L.ptr(Yg), Xg.stride(0), Xg.stride(1), p, q, L.ptr(u),
-                                             u.stride(0), u.stride(1), aa, bb, L.ptr(gc_row), L.ptr(coef), stream),
+                                             u.stride(0), u.stride(1), aa, bb, L.ptr(gcr), L.ptr(coef), stream),
                         'maint_prep')
                 L.check(lib.polar_maint_rank1(dcode, L.ptr(Xg), L.ptr(Yg), Xg.stride(0), Xg.stride(1), p, nAg, q,
                                               L.ptr(u), u.stride(0), u.stride(1), L.ptr(coef), L.ptr(wX), L.ptr(wY),
@@ -361,7 +372,7 @@ def polar_batched(graphs, cands, k, s, q=32, B=4, eps=1e-2, tol=1e-7, obj='frac'
                     tr_hist[g].append(coef[:, 2 * q:2 * q + 2].clone())
             Fv = RHS[:, q:]
             L.check(lib.polar_rows_axpy(dcode, L.ptr(Fv), Fv.stride(0), q, a + int(row0[g]), b + int(row0[g]),
-                                        L.ptr(gc_row), stream), 'rows_axpy')
+                                        L.ptr(gcr), stream), 'rows_axpy')
             op.add(a + int(row0[g]), b + int(row0[g]))
         if progress is not None:
             progress('round', dict(round=rnd, edges=[sq[-1] if sq else None for sq in seqs], seconds=time.perf_counter() - t0,
