@@ -61,6 +61,16 @@ enum polar_status {
                              per-iteration updates, about 8x fewer passes over the shifted blocks */
 #define POLAR_REPLACE 32  /* residual replacement every 24 iterations: r = w - (M + tau0) x0 with the
                              seed iterate x0 kept in FP64 and the product in FP64 (for FP32 vectors) */
+#define POLAR_HISTORY 64  /* the shifted blocks are not formed: the residual of every iteration and
+                             every pole's per-iteration coefficients stay in the workspace (maxit
+                             iterations at most, see polar_msolve_history_ws_bytes), and
+                             polar_history_rows / polar_history_cols form X_j afterwards on chosen
+                             rows of every column, or on every active row of chosen columns.  One
+                             segment (column groups allowed); no column compaction; implies
+                             POLAR_BLOCK8's window coefficients; Xout may be NULL; POLAR_GRAPH is
+                             ignored.  No counterpart in the reference: its re-rank solve forms
+                             every shortlisted column's X (polar_gpu.py:228) and then uses the
+                             winner's (:237) and two rows of the others (:230). */
 
 /* Operator y = (diag + shift) x - A_w x + patch(x): the graph Laplacian M0 = diag(deg+sigma) - A
  * of realworld/polar_gpu.py:47-55 stored as diagonal + off-diagonal adjacency (value-free when
@@ -193,6 +203,26 @@ int polar_msolve_batched(const polar_op* op, const polar_segs* segs, const void*
                          const double* tau, int32_t p, const int32_t* active, double tol, int64_t maxit,
                          int32_t flags, void* Xout, int64_t x_sj, int64_t x_si, double* h_out,
                          double* te_out, int64_t* iters_out, void* ws, int64_t ws_bytes, void* stream);
+
+/* Workspace of a POLAR_HISTORY solve of R columns (polar_msolve, or polar_msolve_batched with one
+ * segment) that keeps up to tcap iterations; pass maxit = tcap to the solve (a solve that needs
+ * more returns POLAR_E_ITERCAP and can be repeated without POLAR_HISTORY). */
+int64_t polar_msolve_history_ws_bytes(const polar_op* op, int64_t R, int32_t p, int64_t nA, int32_t flags,
+                                      int64_t tcap);
+
+/* After a POLAR_HISTORY solve with this workspace (same op, R, p, nA, flags, tcap): the shifted
+ * solutions on K rows of every column, E[j][k][c] = X_j[rows[k*R + c], c] (rows: device int32
+ * absolute row ids; E: device, the op's dtype, p*K*R values).  For the re-rank this is
+ * U[j, a_c, c] and U[j, b_c, c] of realworld/polar_gpu.py:230 (K = 2, rows = the endpoints). */
+int polar_history_rows(const polar_op* op, int64_t R, int32_t p, int64_t nA, int32_t flags, int64_t tcap,
+                       void* ws, int64_t ws_bytes, int32_t K, const int32_t* rows, void* E, void* stream);
+
+/* After a POLAR_HISTORY solve: X_j on every active row for the C columns cols[] (device int32),
+ * X[(i*p + j)*nA + a] = X_j[active[a], cols[i]] (device, the op's dtype): the winner's u of
+ * realworld/polar_gpu.py:237.  At most 4 columns per call. */
+int polar_history_cols(const polar_op* op, int64_t R, int32_t p, int64_t nA, int32_t flags, int64_t tcap,
+                       void* ws, int64_t ws_bytes, int32_t C, const int32_t* cols, const int32_t* active,
+                       void* X, void* stream);
 
 /* out[j*N + e] = sum_c (X[j, ea[e], c] - X[j, eb[e], c])^2, double (polar_gpu.py:205). */
 int polar_pair_sqnorm(int32_t dtype, const void* X, int64_t x_sj, int64_t x_si, int32_t p,

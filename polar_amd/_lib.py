@@ -13,6 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / 'libpolar_amd.so'
 
 POLAR_F64, POLAR_F32 = 0, 1
 POLAR_TRACK, POLAR_TRACE, POLAR_NOFREEZE, POLAR_GRAPH, POLAR_BLOCK8, POLAR_REPLACE = 1, 2, 4, 8, 16, 32
+POLAR_HISTORY = 64
 ABI_VERSION = 5
 POLAR_E_ITERCAP, POLAR_E_ARG = -2, -3
 CHUNK = 64  # POLAR_CHUNK: default long-row chunk length
@@ -24,7 +25,8 @@ EXPORTS = [
     'polar_maint_prep', 'polar_maint_rank1', 'polar_maint_cache', 'polar_rows_axpy',
     'polar_rademacher', 'polar_edge_sketch', 'polar_prof_set', 'polar_prof_get', 'polar_pair_wsum',
     'polar_msolve_batched_ws_bytes', 'polar_msolve_batched', 'polar_csr_spmm', 'polar_release',
-    'polar_spmm_static', 'polar_spmm_static_max_bytes',
+    'polar_spmm_static', 'polar_spmm_static_max_bytes', 'polar_msolve_history_ws_bytes', 'polar_history_rows',
+    'polar_history_cols',
 ]
 
 vp = ctypes.c_void_p
@@ -84,6 +86,9 @@ _SIGS = {
     'polar_msolve_batched_ws_bytes': (i64, [OPP, SEGP, i64, i32, i32]),
     'polar_msolve_batched': (i32, [OPP, SEGP, vp, i64, i64, vp, i32, vp, f64, i64, i32, vp, i64, i64, vp, vp,
                                    vp, vp, i64, vp]),
+    'polar_msolve_history_ws_bytes': (i64, [OPP, i64, i32, i64, i32, i64]),
+    'polar_history_rows': (i32, [OPP, i64, i32, i64, i32, i64, vp, i64, i32, vp, vp, vp]),
+    'polar_history_cols': (i32, [OPP, i64, i32, i64, i32, i64, vp, i64, i32, vp, vp, vp, vp]),
 }
 
 _lib = None

@@ -72,7 +72,7 @@ class Segments:
 
 
 def msolve_batched(op, segs, W, tau, active, tol, maxit=100000, track=False, trace=False, out=None,
-                   width=None, graph=None, freeze=True, block=None, replace=False, col_group=None):
+                   width=None, graph=None, freeze=True, block=None, replace=False, col_group=None, history=0):
     """Batched block multi-shift CG: segment g solves (M_g + tau[g, j]) U = W[rows of g, :R].
 
     op: Op of a UnionGraph; segs: Segments; W: (n, R) device; tau: (S, p) host or device; active:
@@ -81,7 +81,10 @@ def msolve_batched(op, segs, W, tau, active, tol, maxit=100000, track=False, tra
 
     Column groups (one segment): col_group (R,) assigns column c to group col_group[c], which
     solves with the poles tau[col_group[c]] (tau is (G, p)) and the op's added edges tagged with
-    that group; h, te are then (p, 1, R) and iters (1, R)."""
+    that group; h, te are then (p, 1, R) and iters (1, R).
+
+    history=T > 0 (one segment): POLAR_HISTORY with T iterations at most, as polar_gpu.msolve: X is
+    None and a polar_gpu.History is appended to the results."""
     lib = L.require_gpu()
     dev = pg._device()
     dtype = op.dtype
@@ -100,8 +103,12 @@ def msolve_batched(op, segs, W, tau, active, tol, maxit=100000, track=False, tra
         tau_d = torch.as_tensor(tau, dtype=torch.float64, device=dev).reshape(S, -1).contiguous()
     p = tau_d.shape[1]
     nA = int(segs.a0[-1])
-    X = out if out is not None else torch.empty((p, nA, R), dtype=dtype, device=dev)
-    assert X.shape == (p, nA, R) and X.stride(2) == 1
+    if history:
+        assert S == 1 and out is None
+        X = None
+    else:
+        X = out if out is not None else torch.empty((p, nA, R), dtype=dtype, device=dev)
+        assert X.shape == (p, nA, R) and X.stride(2) == 1
     h = torch.empty((p, S, R), dtype=torch.float64, device=dev) if track else None
     te = torch.empty((p, S, R), dtype=torch.float64, device=dev) if trace else None
     iters = torch.empty((S, R), dtype=torch.int64, device=dev)
@@ -112,7 +119,13 @@ def msolve_batched(op, segs, W, tau, active, tol, maxit=100000, track=False, tra
              (L.POLAR_BLOCK8 if pg._blocked(block, dtype) else 0) | (L.POLAR_REPLACE if replace else 0))
     s = op.c_op(dtype)
     step = min(R, pg._max_cols(dtype), width or R)
-    ws_bytes = lib.polar_msolve_batched_ws_bytes(s, segs.c, step, p, flags)
+    if history:
+        assert R <= pg._max_cols(dtype)
+        step, maxit = R, int(history)
+        flags |= L.POLAR_HISTORY
+        ws_bytes = lib.polar_msolve_history_ws_bytes(s, R, p, nA, flags, maxit)
+    else:
+        ws_bytes = lib.polar_msolve_batched_ws_bytes(s, segs.c, step, p, flags)
     L.check(0 if ws_bytes > 0 else L.POLAR_E_ARG, 'polar_msolve_batched_ws_bytes')
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     act = active if nA else torch.zeros(1, dtype=torch.int32, device=dev)
@@ -125,7 +138,8 @@ def msolve_batched(op, segs, W, tau, active, tol, maxit=100000, track=False, tra
             segs.c.G = int(tau_d.shape[0])
             segs.c.col_group = col_group[lo:lo + r].ctypes.data_as(L.vp)
         rc = lib.polar_msolve_batched(s, segs.c, L.ptr(W[:, lo:]), W.stride(0), r, L.ptr(tau_d), p, L.ptr(act),
-                                      float(tol), int(maxit), flags, L.ptr(X[:, :, lo:]), X.stride(0), X.stride(1),
+                                      float(tol), int(maxit), flags, None if X is None else L.ptr(X[:, :, lo:]),
+                                      0 if X is None else X.stride(0), 0 if X is None else X.stride(1),
                                       L.ptr(hb), L.ptr(tb), L.ptr(ib), L.ptr(ws), ws_bytes, L.stream_ptr())
         if col_group is not None:
             segs.c.G, segs.c.col_group = 0, None
@@ -139,6 +153,8 @@ def msolve_batched(op, segs, W, tau, active, tol, maxit=100000, track=False, tra
     res = (X, h, iters.cpu().numpy())
     if trace:
         res = res + (te,)
+    if history:
+        res = res + (pg.History(op, s, R, p, nA, flags, maxit, ws, dtype),)
     return res
 
 
@@ -402,7 +418,7 @@ def single_segment(G, nA):
 
 def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac', sketch=None, seed=2026,
                   kind='rademacher', poles=None, precision='fp32', graph=None, width=None, block=None,
-                  replace=None, progress=None, bank_union=None, shared_bank=None):
+                  replace=None, progress=None, bank_union=None, shared_bank=None, history=None):
     """POLAR greedy (polar_gpu.py:180-254) for several s values on ONE graph, in lockstep, as column
     groups of one operator: group g is the run for s_list[g], with its own poles, its own bank
     columns [Z | F_g], its own caches and its own selected edges (patch entries tagged g).  Every
@@ -413,6 +429,9 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
     on it (the groups' 2q columns side by side in rows, finished groups leave the grid) and the
     re-rank solves as column groups (same results either way).  replace: FP64 residual replacement
     in every solve; None = on for FP32, off for FP64.
+
+    history: the FP32 re-rank solve forms the shifted solutions only on the shortlists' endpoint
+    rows and the winners' columns (polar_gpu.polar's `history`); None = when it fits, False = never.
 
     shared_bank: the groups' bank right-hand sides [Z | F] are the same, so one seed CG can serve all
     of them: the seed is the smallest pole of all groups and every group's poles are its shifted
@@ -488,6 +507,8 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
     gc_dev = torch.empty((Gn, max(B, 1), q), dtype=torch.float64, device=dev) \
         if isinstance(Gc, pg.RademacherRows) else None
     gc_row = torch.empty(q, dtype=torch.float64, device=dev)
+    # endpoint row indices 0 / 1 of a (p, 2, R) block of endpoint values (history re-rank)
+    z01 = torch.arange(2, dtype=torch.int32, device=dev)[:, None].repeat(1, max(B, 1)).contiguous()
     seqs = [[] for _ in range(Gn)]
     margs = [[] for _ in range(Gn)]
     traces = [[] for _ in range(Gn)]
@@ -609,17 +630,37 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
         Wb.index_put_((ia, ic), torch.ones(Rt, dtype=dtype, device=dev))
         Wb.index_put_((ib, ic), torch.full((Rt,), -1.0, dtype=dtype, device=dev))
         cg = np.concatenate([np.full(len(sh), g, np.int32) for g, sh in enumerate(shorts)])
-        Ub, hh, it = msolve_batched(op, segs, Wb, tau_all, active, tol, track=obj != 'logdet', graph=graph,
-                                    block=block, replace=replace, col_group=cg)[:3]
+        H = None
+        hcap = pg.history_cap(n, Rt, dtype) if history is not False else 0
+        if hcap:
+            # FP32: keep the residuals, form X_j only at the shortlisted endpoints and the winners
+            try:
+                _, hh, it, H = msolve_batched(op, segs, Wb, tau_all, active, tol, track=obj != 'logdet',
+                                              graph=graph, block=block, replace=replace, col_group=cg, history=hcap)
+            except RuntimeError as ex:
+                if 'iteration cap' not in str(ex):
+                    raise
+        if H is None:
+            Ub, hh, it = msolve_batched(op, segs, Wb, tau_all, active, tol, track=obj != 'logdet', graph=graph,
+                                        block=block, replace=replace, col_group=cg)[:3]
+        else:
+            Ub = H.rows(idx[:2])                       # (p, 2, Rt): U at the two endpoints of every column
         # every group's re-rank pick on the device, then one read-back of all winners
         for g in range(Gn):
             if Brs[g]:
                 Ug = Ub[:, :, off[g]:]
                 hg = hh[:, 0, off[g]:off[g + 1]].contiguous() if hh is not None else None
+                sa, sb = (sab[0, off[g]:], sab[1, off[g]:]) if H is None else (z01[0], z01[1])
                 L.check(lib.polar_rerank_pick(objc, dcode, L.ptr(Ug), Ug.stride(0), Ug.stride(1), p, Brs[g],
-                                              L.ptr(sab[0, off[g]:]), L.ptr(sab[1, off[g]:]), L.ptr(hg), L.ptr(om_d[g]),
+                                              L.ptr(sa), L.ptr(sb), L.ptr(hg), L.ptr(om_d[g]),
                                               L.ptr(gvec[g]), None, None, stream), 'rerank_pick')
         gv_h = gvec.cpu().numpy()
+        wins = {g: (0 if Brs[g] == 1 else int(gv_h[g, Brs[g]])) for g in range(Gn) if Brs[g]}
+        if H is not None and rnd + 1 < k and wins:
+            # the winners' columns on every active row, for the maintenance
+            wg = sorted(wins)
+            Uw = H.cols(torch.from_numpy(np.array([off[g] + wins[g] for g in wg], dtype=np.int32)).to(dev), active)
+            Uwin = {g: Uw[i] for i, g in enumerate(wg)}
         for g in range(Gn):
             Br = Brs[g]
             if Br == 0:
@@ -627,9 +668,9 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
             spmv[g] += int(it[0, off[g]:off[g + 1]].sum())
             rhs[g] += Br
             short = shorts[g]
-            j = 0 if Br == 1 else int(gv_h[g, Br])
+            j = wins[g]
             e = int(short[j])
-            u = Ub[:, :, off[g] + j]
+            u = Ub[:, :, off[g] + j] if H is None else (Uwin[g] if rnd + 1 < k else None)
             margs[g].append(float(gv_h[g, Br + 1]))
             seqs[g].append(e)
             live[g][e] = False

@@ -308,6 +308,12 @@ struct CGState {
   // The operator kernels take the seed shift from tau[g * p] in device memory (segment g, 0 for one
   // system) instead of their `shift` argument: a solve never reads tau back to the host.
   int tau_dev;
+  // POLAR_HISTORY (one segment): the finisher of iteration t keeps, per (pole, column slot), the
+  // live flag and a_j, b_j, zeta_j of X_j += a_j P_j, P_j <- zeta_j r_{t+1} + b_j P_j at
+  // [t][j][ld]; hist_T[0] = iterations that did work.  NULL otherwise.
+  uint8_t* hm;
+  double *haj, *hbj, *hzn;
+  int* hist_T;
 };
 
 // Segment of block b of a kernel whose blocks are partitioned by the prefix table `pref`.
@@ -407,7 +413,7 @@ __device__ __forceinline__ void pole_load(const CGState& s, int j, int idx, int 
 // Finisher after |rn|^2 (one workgroup per (segment, column), threads over poles): every scalar
 // recurrence of polar_gpu.py:106-125 for the column, freezing, convergence and the any-active
 // flags.  Everything that does not depend on this iteration's |r|^2 is loaded first.
-__global__ __launch_bounds__(kBlock) void k_scalars(CGState s, int nparts, int cur, int wt) {
+__global__ __launch_bounds__(kBlock) void k_scalars(CGState s, int nparts, int cur, int wt, int hit) {
   if (s.flag[cur] == 0) return;
   const int g = blockIdx.x / s.R, c = blockIdx.x - g * s.R;
   if (!seg_live(s, cur, g)) return;
@@ -474,8 +480,16 @@ __global__ __launch_bounds__(kBlock) void k_scalars(CGState s, int nparts, int c
       s.dirty[idx] = 1;
       s.pdirty[j * s.S + g] = 1;
     }
+    if (s.hm && hit >= 0) {
+      const int64_t hx = ((int64_t)hit * s.p + j) * s.ld + c;
+      s.hm[hx] = m ? 1 : 0;
+      s.haj[hx] = aj;
+      s.hbj[hx] = bj;
+      s.hzn[hx] = zn;
+    }
     if (s.freeze && m && fabs(zn) * nr <= stop) s.live[idx] = 0;
   }
+  if (s.hist_T && hit >= 0 && blockIdx.x == 0 && threadIdx.x == 0) s.hist_T[0] = hit + 1;
   if (nlive) atomicAdd(&nlive_s, nlive);
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1392,9 +1406,16 @@ __global__ __launch_bounds__(kBlock) void k_seed_flush(int64_t n, int R, int L, 
       for (int e = 0; e < CPL; ++e) inc[e] = cx[0][e] * (double)b[e];
 #pragma unroll
       for (int t = 1; t <= kW; ++t) {
+        // steps the window did not take (convergence, or the end of a history solve) have zero
+        // weight and a residual that may never have been written: skipped
+        bool nz = false;
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) nz |= cx[t][e] != 0.0;
+        if (!nz) continue;
         vload<T, CPL>(ring.s[t % kW] + i * s.ld + c0, b);
 #pragma unroll
-        for (int e = 0; e < CPL; ++e) inc[e] = inc[e] + cx[t][e] * (double)b[e];
+        for (int e = 0; e < CPL; ++e)
+          if (cx[t][e] != 0.0) inc[e] = inc[e] + cx[t][e] * (double)b[e];
       }
 #pragma unroll
       for (int e = 0; e < CPL; ++e) xv[e] = xv[e] + inc[e];
@@ -1658,12 +1679,18 @@ struct SolveMem {
   double *y64, *lpart64;   // residual replacement: the FP64 product and its long-row partials
   int* dtab;           // 5 (S+1) block prefix tables
   int64_t* dseg;       // 4 (S+1) segment ranges
+  // POLAR_HISTORY: ring holds r_0 .. r_tcap; `cur` is the residual that replaced a stored one (the
+  // next iteration reads it); coef[c][j][t] is the weight of r_t in X_j (formed on demand)
+  int64_t tcap;
+  T* cur;
+  double* coef;
 };
 
 template <typename T>
 static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int p, int64_t nA, int Tw,
-                           bool repl, SolveMem<T>& m) {
+                           bool repl, SolveMem<T>& m, int64_t tcap = 0) {
   Carver cv(base);
+  m.tcap = tcap;
   CGState& s = m.s;
   const int64_t C = (int64_t)S * R;
   s.rr = cv.take<double>(C); s.stop = cv.take<double>(C); s.a = cv.take<double>(C);
@@ -1703,11 +1730,26 @@ static int64_t carve_solve(void* base, const polar_op* op, int S, int64_t R, int
   m.dseg = cv.take<int64_t>(4 * (S + 1));
   m.perm = cv.take<int>(R);
   const int64_t n = op->n;
-  m.ring = cv.take<T>(n * R * Tw);
+  if (tcap > 0) {
+    s.hist_T = cv.take<int>(1);
+    s.hm = cv.take<uint8_t>(tcap * p * R);
+    s.haj = cv.take<double>(tcap * p * R);
+    s.hbj = cv.take<double>(tcap * p * R);
+    s.hzn = cv.take<double>(tcap * p * R);
+    m.coef = cv.take<double>(R * p * (tcap + 1));
+    m.cur = cv.take<T>(n * R);
+  } else {
+    s.hist_T = nullptr;
+    s.hm = nullptr;
+    s.haj = s.hbj = s.hzn = nullptr;
+    m.coef = nullptr;
+    m.cur = nullptr;
+  }
+  m.ring = cv.take<T>(n * R * (tcap > 0 ? align_up(tcap, kW) + 1 : Tw));
   m.d = cv.take<T>(n * R);
   m.v = cv.take<T>(n * R);
   m.t2 = cv.take<T>(n * R);
-  m.P = cv.take<T>((int64_t)p * nA * R);
+  m.P = cv.take<T>(tcap > 0 ? 0 : (int64_t)p * nA * R);
   m.lpart = cv.take<T>((int64_t)op->n_partial * R);
   return cv.off + 256;
 }
@@ -1867,7 +1909,7 @@ __global__ void k_iota(int* __restrict__ a, int n) {
 template <typename T, int CPL>
 static int run_iter(const polar_op* op, const Plan& pl, double tau0, const T* rin, T* rout, T* d, T* v, T* lpart,
                     T* P, T* Xo, int64_t sj, int64_t si, const int32_t* active, int64_t nA, int p, const CGState& s,
-                    int cur, int wt, hipEvent_t* ev, hipStream_t st) {
+                    int cur, int wt, int hit, hipEvent_t* ev, hipStream_t st) {
   const int64_t n = op->n;
   const int R = s.R;
   const bool longs = op->n_long > 0 && pl.grid_l > 0;
@@ -1891,10 +1933,11 @@ static int run_iter(const polar_op* op, const Plan& pl, double tau0, const T* ri
                      (const T*)d, s, cur);
   POLAR_LAUNCH_CHECK();
   if (ev && g_prof_on == 1) POLAR_HIP(hipEventRecord(ev[4], st));
-  hipLaunchKernelGGL(k_scalars, dim3(pl.fin), dim3(kBlock), 0, st, s, pl.grid_rows, cur, wt);
+  hipLaunchKernelGGL(k_scalars, dim3(pl.fin), dim3(kBlock), 0, st, s, pl.grid_rows, cur, wt, hit);
   POLAR_LAUNCH_CHECK();
   if (ev && g_prof_on == 1) POLAR_HIP(hipEventRecord(ev[5], st));
-  const int nb_d = pl.nb_d, nb_s = s.T > 1 ? 0 : pl.nb_s;    // blocked: the window flush updates X, P
+  // blocked: the window flush updates X, P; history: no X, P at all
+  const int nb_d = pl.nb_d, nb_s = (s.T > 1 || s.hm) ? 0 : pl.nb_s;
   if (p * R <= 1024)
     hipLaunchKernelGGL((k_update<T, CPL, true>), dim3(nb_d + nb_s), dim3(kBlock), 0, st, n, nA, p, R, pl.L,
                        (const T*)rout, d, active, P, Xo, sj, si, nb_d, s, cur);
@@ -1909,12 +1952,12 @@ static int run_iter(const polar_op* op, const Plan& pl, double tau0, const T* ri
 template <typename T>
 static int run_iter_any(int cpl, const polar_op* op, const Plan& pl, double tau0, const T* rin, T* rout, T* d, T* v,
                         T* lpart, T* P, T* Xo, int64_t sj, int64_t si, const int32_t* active, int64_t nA, int p,
-                        const CGState& s, int cur, int wt, hipEvent_t* ev, hipStream_t st) {
-  if (cpl == 2) return run_iter<T, 2>(op, pl, tau0, rin, rout, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, wt, ev, st);
+                        const CGState& s, int cur, int wt, int hit, hipEvent_t* ev, hipStream_t st) {
+  if (cpl == 2) return run_iter<T, 2>(op, pl, tau0, rin, rout, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, wt, hit, ev, st);
   if constexpr (sizeof(T) == 4) {
-    if (cpl == 4) return run_iter<T, 4>(op, pl, tau0, rin, rout, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, wt, ev, st);
+    if (cpl == 4) return run_iter<T, 4>(op, pl, tau0, rin, rout, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, wt, hit, ev, st);
   }
-  return run_iter<T, 1>(op, pl, tau0, rin, rout, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, wt, ev, st);
+  return run_iter<T, 1>(op, pl, tau0, rin, rout, d, v, lpart, P, Xo, sj, si, active, nA, p, s, cur, wt, hit, ev, st);
 }
 
 // Residual replacement on the current residual r (after a window's flush): y = (M + tau0) x0 by the
@@ -2070,7 +2113,9 @@ struct IterGraph {
 
 // Window length of the shifted update: kW with POLAR_BLOCK8 (when the pole list fits the flush
 // kernel's LDS), else 1 (per-iteration updates).
-static int window_of(int flags, int p) { return ((flags & POLAR_BLOCK8) && p <= kMaxPolesFlush) ? kW : 1; }
+static int window_of(int flags, int p) {
+  return ((flags & (POLAR_BLOCK8 | POLAR_HISTORY)) && p <= kMaxPolesFlush) ? kW : 1;
+}
 
 constexpr int kReplaceEvery = 3 * kW;   // residual replacement period (iterations; whole windows)
 
@@ -2085,7 +2130,10 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   SolveMem<T> m;
   const int Tw = window_of(flags, p);
   const bool repl = (flags & POLAR_REPLACE) != 0;
-  const int64_t need = carve_solve<T>(ws, op, S, R, p, nA, Tw, repl, m);
+  const bool hist = (flags & POLAR_HISTORY) != 0;
+  if (hist && S != 1) return set_error(POLAR_E_ARG, "POLAR_HISTORY needs one segment");
+  if (hist && maxit <= 0) return set_error(POLAR_E_ARG, "POLAR_HISTORY needs maxit >= 1 (the history capacity)");
+  const int64_t need = carve_solve<T>(ws, op, S, R, p, nA, Tw, repl, m, hist ? maxit : 0);
   if (need > ws_bytes) return set_error(POLAR_E_ARG, "msolve workspace too small (%lld < %lld)", (long long)ws_bytes, (long long)need);
   CGState s = m.s;
   T *d = m.d, *v = m.v, *t2 = m.t2, *P = m.P, *lpart = m.lpart;
@@ -2163,14 +2211,17 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   POLAR_LAUNCH_CHECK();
   if (s.amask) {
     if (int rc = zero_async((void*)s.amask, align_up(n, 16), st)) return rc;
-    if (nA > 0) {
+    if (nA > 0 && !hist) {     // history: the seed iterate advances on every row in k_seed_flush
       hipLaunchKernelGGL(k_mark, dim3((int)std::min<int64_t>(4096, (nA + 255) / 256)), dim3(256), 0, st, active, nA,
                          (uint8_t*)s.amask);
       POLAR_LAUNCH_CHECK();
     }
   }
   if (m.dsave) POLAR_HIP(hipMemcpyAsync(m.dsave, d, sizeof(T) * (size_t)n * R, hipMemcpyDeviceToDevice, st));
-  if (nA > 0) {
+  if (hist) {
+    if (int rc = zero_async(s.hist_T, sizeof(int), st)) return rc;
+  }
+  if (nA > 0 && !hist) {
     const int grid_sh = (int)std::min<int64_t>(65536, std::max<int64_t>(1, (nA + gpb0 - 1) / gpb0));
     if (cpl == 2) {
       hipLaunchKernelGGL((k_init_shift<T, 2>), dim3(grid_sh), dim3(kBlock), 0, st, nA, p, (int)R, pl.L, W, ldw, active, P, Xo, sj, si);
@@ -2185,7 +2236,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
   const double tau0 = 0.0;    // unused: the operator kernels read the seed shift from tau (tau_dev)
 
   const bool prof = g_prof_on != 0;
-  const bool use_graph = (flags & POLAR_GRAPH) && !prof;
+  const bool use_graph = (flags & POLAR_GRAPH) && !prof && !hist;
   IterGraph graph;
   hipStream_t cs = nullptr;
   if (use_graph) {
@@ -2230,6 +2281,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
     for (int t = 0; t < kW; ++t) rg.s[t] = slot[t];
     return rg;
   };
+  int64_t cur_it = -1;         // history: the iteration whose input residual is m.cur (a replacement)
   struct Batch {
     int64_t it0;      // first iteration
     int nit;          // iterations (kBatch, fewer at the cap)
@@ -2246,7 +2298,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
         int rc = 0;
         for (int k = 0; k < kBatch && !rc; ++k)
           rc = run_iter_any<T>(cpl, op, pl, tau0, slot[k % kW], slot[(k + 1) % kW], d, v, lpart, P, Xo, sj, si, active,
-                               nA, p, s, k & 1, k, nullptr, cs);
+                               nA, p, s, k & 1, k, -1, nullptr, cs);
         if (!rc && Tw > 1) rc = flush_window<T>(cpl, pl, ring_of(), active, nA, p, P, Xo, sj, si, s, n, d, m.dsave, cs);
         hipGraph_t gr = nullptr;
         const hipError_t ec = hipStreamEndCapture(cs, &gr);
@@ -2276,8 +2328,14 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
           widths.push_back(s.R);
           timed.push_back(take ? 1 : 0);
         }
-        int rc = run_iter_any<T>(cpl, op, pl, tau0, slot[k % kW], slot[(k + 1) % kW], d, v, lpart, P, Xo, sj, si,
-                                 active, nA, p, s, (int)(it & 1), k, evp, st);
+        const T* rin = slot[k % kW];
+        T* rout = slot[(k + 1) % kW];
+        if (hist) {       // r_t in ring slot t (or the residual that replaced it), r_{t+1} into slot t+1
+          rin = it == cur_it ? m.cur : m.ring + it * n * R;
+          rout = m.ring + (it + 1) * n * R;
+        }
+        int rc = run_iter_any<T>(cpl, op, pl, tau0, rin, rout, d, v, lpart, P, Xo, sj, si, active, nA, p, s,
+                                 (int)(it & 1), k, hist ? (int)it : -1, evp, st);
         if (rc) return rc;
       }
       if (Tw > 1) {
@@ -2287,14 +2345,29 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
             if (!e) POLAR_HIP(hipEventCreate(&e));
           POLAR_HIP(hipEventRecord(evs.f[b.buf][0], st));
         }
-        if (int rc = flush_window<T>(cpl, pl, ring_of(), active, nA, p, P, Xo, sj, si, s, n, d, m.dsave, st)) return rc;
+        if (hist) {
+          // no shifted blocks: only the seed iterate (on every row) and the next window's start
+          Ring<T> rg;
+          for (int t = 1; t <= kW; ++t) rg.s[t % kW] = m.ring + (it0 + t) * n * R;
+          if (int rc = flush_window<T>(cpl, pl, rg, active, 0, p, P, Xo, sj, si, s, n, d, m.dsave, st)) return rc;
+        } else {
+          if (int rc = flush_window<T>(cpl, pl, ring_of(), active, nA, p, P, Xo, sj, si, s, n, d, m.dsave, st)) return rc;
+        }
         if (fprof) POLAR_HIP(hipEventRecord(evs.f[b.buf][1], st));
       }
     }
     if (repl && nit == kBatch && (it0 + kBatch) % kReplaceEvery == 0) {
       // the current residual is slot[0] after a full window (Tw = kW) or the only vector (Tw = 1);
-      // after convergence the replacement kernels return at once, like the iterations
-      if (int rc = replace_residual<T>(cpl, op, pl, tau0, W, ldw, slot[0], m.y64, m.lpart64, s,
+      // after convergence the replacement kernels return at once, like the iterations.  History:
+      // ring slot t keeps the residual the shifted recurrences used, the replaced one goes to `cur`
+      T* rcur = slot[0];
+      if (hist) {
+        const int64_t t1 = it0 + kBatch;
+        POLAR_HIP(hipMemcpyAsync(m.cur, m.ring + t1 * n * R, sizeof(T) * (size_t)n * R, hipMemcpyDeviceToDevice, st));
+        rcur = m.cur;
+        cur_it = t1;
+      }
+      if (int rc = replace_residual<T>(cpl, op, pl, tau0, W, ldw, rcur, m.y64, m.lpart64, s,
                                        (int)((it0 + kBatch) & 1), st))
         return rc;
     }
@@ -2363,7 +2436,7 @@ static int msolve_impl(int cpl0, const polar_op* op, const Segs& sg, const T* W,
           POLAR_HIP(hipMemcpyAsync(m.dtab, pl.tab.data(), sizeof(int) * pl.tab.size(), hipMemcpyHostToDevice, st));
           graph.drop();
         }
-      } else if (may_change) {
+      } else if (may_change && !hist) {
         // Active-column compaction (one segment): when at most half of the current slots are still
         // active, move them (padded to the vector width with finished slots) to the front.  Columns
         // are independent, so every column's arithmetic is unchanged.
@@ -2560,6 +2633,186 @@ static int scratch_address(void** out) {
   return 0;
 }
 
+// ------------------------------------------------------------------------------------------------
+// POLAR_HISTORY: X_j formed after the solve from the stored residuals r_0 .. r_T and the per-step
+// pole coefficients.  With P^(0) = r_0 and, at every step t where the pole is live,
+// X += a_t P^(t), P^(t+1) = z_t r_{t+1} + b_t P^(t) (no change otherwise), X_j = sum_t w_t r_t with
+// weights from the backward recursion G_T = 0; at a live step G_t = a_t + b_t G_{t+1} and
+// w_{t+1} = z_t G_{t+1}, at any other step G_t = G_{t+1} and w_{t+1} = 0; w_0 = G_0.  FP64 weights
+// and sums, whatever the vector type.  Loops over steps load kHistBatch steps before using them,
+// so that their latencies overlap.
+constexpr int kHistBatch = 16;
+
+__global__ __launch_bounds__(256) void k_hist_coef(const uint8_t* __restrict__ hm, const double* __restrict__ haj,
+                                                   const double* __restrict__ hbj, const double* __restrict__ hzn,
+                                                   const int* __restrict__ hist_T, int p, int ld, int64_t tcap,
+                                                   const int* __restrict__ cols, int C, double* __restrict__ coef) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;   // (selected column i, pole j)
+  if (idx >= C * p) return;
+  const int i = idx / p, j = idx - i * p;
+  const int c = cols ? cols[i] : i;
+  const int Tn = hist_T[0];
+  double* w = coef + ((int64_t)c * p + j) * (tcap + 1);
+  double G = 0.0;
+  for (int t = Tn - 1; t >= 0; t -= kHistBatch) {
+    const int nb = min(kHistBatch, t + 1);
+    uint8_t mm[kHistBatch];
+    double aa[kHistBatch], bb[kHistBatch], zz[kHistBatch];
+#pragma unroll
+    for (int u = 0; u < kHistBatch; ++u)
+      if (u < nb) {
+        const int64_t hx = ((int64_t)(t - u) * p + j) * ld + c;
+        mm[u] = hm[hx];
+        aa[u] = haj[hx];
+        bb[u] = hbj[hx];
+        zz[u] = hzn[hx];
+      }
+#pragma unroll
+    for (int u = 0; u < kHistBatch; ++u)
+      if (u < nb) {
+        if (mm[u]) {
+          w[t - u + 1] = zz[u] * G;
+          G = aa[u] + bb[u] * G;
+        } else {
+          w[t - u + 1] = 0.0;
+        }
+      }
+  }
+  w[0] = G;
+}
+
+// E[(j*K + k)*R + c] = X_j[rows[k*R + c], c]: one thread per value, steps in increasing order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_hist_rows(const T* __restrict__ ring, int64_t n, int ld, int p, int64_t tcap,
+                                                   const int* __restrict__ hist_T, const double* __restrict__ coef,
+                                                   int R, int K, const int32_t* __restrict__ rows, T* __restrict__ E) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= p * K * R) return;
+  const int c = idx % R, k = (idx / R) % K, j = idx / (R * K);
+  const int64_t row = rows[k * R + c];
+  const int Tn = hist_T[0];
+  const double* w = coef + ((int64_t)c * p + j) * (tcap + 1);
+  double acc = 0.0;
+  for (int t0 = 0; t0 <= Tn; t0 += kHistBatch) {
+    const int nb = min(kHistBatch, Tn + 1 - t0);
+    double wt[kHistBatch], v[kHistBatch];
+#pragma unroll
+    for (int u = 0; u < kHistBatch; ++u)
+      if (u < nb) {
+        wt[u] = w[t0 + u];
+        v[u] = (double)ring[(int64_t)(t0 + u) * n * ld + row * ld + c];
+      }
+#pragma unroll
+    for (int u = 0; u < kHistBatch; ++u)
+      if (u < nb && wt[u] != 0.0) acc = acc + wt[u] * v[u];
+  }
+  E[idx] = (T)acc;
+}
+
+// X[(i*p + j)*nA + a] = X_j[active[a], cols[i]]: one thread per (active row, selected column), the C
+// columns of a row on neighbouring lanes (one residual row serves them all); the weights are staged
+// in LDS kHistStage steps at a time; PM >= p accumulators per thread.
+constexpr int kHistStage = 32;
+constexpr int kHistMaxC = 4;
+template <typename T, int PM>
+__global__ __launch_bounds__(256) void k_hist_cols(const T* __restrict__ ring, int64_t n, int ld, int p, int64_t tcap,
+                                                   const int* __restrict__ hist_T, const double* __restrict__ coef,
+                                                   const int* __restrict__ cols, int C,
+                                                   const int32_t* __restrict__ active, int64_t nA,
+                                                   T* __restrict__ X) {
+  __shared__ double sw[kHistMaxC][PM][kHistStage];
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(q % C);
+  const int64_t a = q / C;
+  const int c = cols[i];
+  const int Tn = hist_T[0];
+  double acc[PM];
+#pragma unroll
+  for (int j = 0; j < PM; ++j) acc[j] = 0.0;
+  const bool ok = a < nA;
+  const int64_t row = ok ? active[a] : 0;
+  for (int t0 = 0; t0 <= Tn; t0 += kHistStage) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < C * p * kHistStage; e += blockDim.x) {
+      const int ci = e / (p * kHistStage), j = (e / kHistStage) % p, tt = e % kHistStage;
+      sw[ci][j][tt] = t0 + tt <= Tn ? coef[((int64_t)cols[ci] * p + j) * (tcap + 1) + t0 + tt] : 0.0;
+    }
+    __syncthreads();
+    if (ok) {
+      const int nt = min(kHistStage, Tn + 1 - t0);
+      for (int u0 = 0; u0 < nt; u0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (u0 + u < nt) v[u] = (double)ring[(int64_t)(t0 + u0 + u) * n * ld + row * ld + c];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (u0 + u < nt) {
+#pragma unroll
+            for (int j = 0; j < PM; ++j)
+              if (j < p) acc[j] = acc[j] + sw[i][j][u0 + u] * v[u];
+          }
+      }
+    }
+  }
+  if (ok) {
+#pragma unroll
+    for (int j = 0; j < PM; ++j)
+      if (j < p) X[((int64_t)i * p + j) * nA + a] = (T)acc[j];
+  }
+}
+
+template <typename T>
+static int history_mem(const polar_op* op, int64_t R, int p, int64_t nA, int flags, int64_t tcap, void* ws,
+                       int64_t ws_bytes, SolveMem<T>& m) {
+  if (!(flags & POLAR_HISTORY)) return set_error(POLAR_E_ARG, "history query without POLAR_HISTORY");
+  if (R <= 0 || p <= 0 || nA <= 0 || tcap <= 0 || !ws) return set_error(POLAR_E_ARG, "bad history arguments");
+  const int64_t need = carve_solve<T>(ws, op, 1, R, p, nA, window_of(flags, p), (flags & POLAR_REPLACE) != 0, m, tcap);
+  if (need > ws_bytes) return set_error(POLAR_E_ARG, "history workspace too small (%lld < %lld)", (long long)ws_bytes, (long long)need);
+  return 0;
+}
+
+template <typename T>
+static int history_rows_impl(const polar_op* op, int64_t R, int p, int64_t nA, int flags, int64_t tcap, void* ws,
+                             int64_t ws_bytes, int K, const int32_t* rows, T* E, hipStream_t st) {
+  SolveMem<T> m;
+  if (int rc = history_mem<T>(op, R, p, nA, flags, tcap, ws, ws_bytes, m)) return rc;
+  const CGState& s = m.s;
+  hipLaunchKernelGGL(k_hist_coef, dim3((int)((R * p + 255) / 256)), dim3(256), 0, st, s.hm, s.haj, s.hbj, s.hzn,
+                     s.hist_T, p, (int)R, tcap, (const int*)nullptr, (int)R, m.coef);
+  POLAR_LAUNCH_CHECK();
+  const int64_t tot = (int64_t)p * K * R;
+  hipLaunchKernelGGL((k_hist_rows<T>), dim3((int)((tot + 255) / 256)), dim3(256), 0, st, (const T*)m.ring, op->n,
+                     (int)R, p, tcap, s.hist_T, (const double*)m.coef, (int)R, K, rows, E);
+  POLAR_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
+static int history_cols_impl(const polar_op* op, int64_t R, int p, int64_t nA, int flags, int64_t tcap, void* ws,
+                             int64_t ws_bytes, int C, const int32_t* cols, const int32_t* active, T* X,
+                             hipStream_t st) {
+  SolveMem<T> m;
+  if (int rc = history_mem<T>(op, R, p, nA, flags, tcap, ws, ws_bytes, m)) return rc;
+  if (p > 64) return set_error(POLAR_E_ARG, "polar_history_cols supports at most 64 poles (p=%d)", p);
+  const CGState& s = m.s;
+  hipLaunchKernelGGL(k_hist_coef, dim3((C * p + 255) / 256), dim3(256), 0, st, s.hm, s.haj, s.hbj, s.hzn, s.hist_T,
+                     p, (int)R, tcap, cols, C, m.coef);
+  POLAR_LAUNCH_CHECK();
+  if (C > kHistMaxC) return set_error(POLAR_E_ARG, "polar_history_cols takes at most %d columns per call", kHistMaxC);
+  const int64_t blocks = (nA * C + 255) / 256;
+  if (blocks > INT32_MAX) return set_error(POLAR_E_ARG, "too many rows x columns");
+  const dim3 grid((unsigned)blocks);
+  if (p <= 32)
+    hipLaunchKernelGGL((k_hist_cols<T, 32>), grid, dim3(256), 0, st, (const T*)m.ring, op->n, (int)R, p, tcap,
+                       s.hist_T, (const double*)m.coef, cols, C, active, nA, X);
+  else
+    hipLaunchKernelGGL((k_hist_cols<T, 64>), grid, dim3(256), 0, st, (const T*)m.ring, op->n, (int)R, p, tcap,
+                       s.hist_T, (const double*)m.coef, cols, C, active, nA, X);
+  POLAR_LAUNCH_CHECK();
+  return 0;
+}
+
 }  // namespace polar
 
 using namespace polar;
@@ -2673,7 +2926,7 @@ int polar_msolve(const polar_op* op, const void* W, int64_t ldw, int64_t R, cons
                  int32_t flags, void* Xout, int64_t x_sj, int64_t x_si, double* h_out,
                  double* te_out, int64_t* iters_out, void* ws, int64_t ws_bytes, void* stream) {
   if (int rc = check_op(op)) return rc;
-  if (R <= 0 || ldw < R || p <= 0 || nA <= 0 || !W || !tau || !active || !Xout || !ws)
+  if (R <= 0 || ldw < R || p <= 0 || nA <= 0 || !W || !tau || !active || (!Xout && !(flags & POLAR_HISTORY)) || !ws)
     return set_error(POLAR_E_ARG, "bad msolve arguments");
   if (R > 512) return set_error(POLAR_E_ARG, "R=%lld exceeds 512 columns per block solve", (long long)R);
   const Segs sg = one_seg(op, nA);
@@ -2700,7 +2953,7 @@ int polar_msolve_batched(const polar_op* op, const polar_segs* segs, const void*
   Segs sg;
   if (int rc = segs_from(op, segs, R, sg)) return rc;
   const int64_t nA = sg.a[sg.S];
-  if (R <= 0 || ldw < R || p <= 0 || !W || !tau || (nA > 0 && !active) || !Xout || !ws)
+  if (R <= 0 || ldw < R || p <= 0 || !W || !tau || (nA > 0 && !active) || (!Xout && !(flags & POLAR_HISTORY)) || !ws)
     return set_error(POLAR_E_ARG, "bad msolve arguments");
   if (R > 512) return set_error(POLAR_E_ARG, "R=%lld exceeds 512 columns per block solve", (long long)R);
   if ((int64_t)sg.S * R > (int64_t)INT32_MAX / 64 || (int64_t)sg.S * R * p > (int64_t)INT32_MAX / 2)
@@ -2709,6 +2962,40 @@ int polar_msolve_batched(const polar_op* op, const polar_segs* segs, const void*
   if (op->dtype == POLAR_F64)
     return msolve_dispatch<double>(op, sg, W, ldw, R, tau, p, active, tol, maxit, flags, Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
   return msolve_dispatch<float>(op, sg, W, ldw, R, tau, p, active, tol, maxit, flags, Xout, x_sj, x_si, h_out, te_out, iters_out, ws, ws_bytes, st);
+}
+
+int64_t polar_msolve_history_ws_bytes(const polar_op* op, int64_t R, int32_t p, int64_t nA, int32_t flags,
+                                      int64_t tcap) {
+  if (check_op(op) || R <= 0 || p <= 0 || nA <= 0 || tcap <= 0) return -1;
+  flags |= POLAR_HISTORY;
+  const bool repl = (flags & POLAR_REPLACE) != 0;
+  if (op->dtype == POLAR_F64) {
+    SolveMem<double> m;
+    return carve_solve<double>(nullptr, op, 1, R, p, nA, window_of(flags, p), repl, m, tcap);
+  }
+  SolveMem<float> m;
+  return carve_solve<float>(nullptr, op, 1, R, p, nA, window_of(flags, p), repl, m, tcap);
+}
+
+int polar_history_rows(const polar_op* op, int64_t R, int32_t p, int64_t nA, int32_t flags, int64_t tcap,
+                       void* ws, int64_t ws_bytes, int32_t K, const int32_t* rows, void* E, void* stream) {
+  if (int rc = check_op(op)) return rc;
+  if (K <= 0 || !rows || !E) return set_error(POLAR_E_ARG, "bad history_rows arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (op->dtype == POLAR_F64)
+    return history_rows_impl<double>(op, R, p, nA, flags, tcap, ws, ws_bytes, K, rows, (double*)E, st);
+  return history_rows_impl<float>(op, R, p, nA, flags, tcap, ws, ws_bytes, K, rows, (float*)E, st);
+}
+
+int polar_history_cols(const polar_op* op, int64_t R, int32_t p, int64_t nA, int32_t flags, int64_t tcap,
+                       void* ws, int64_t ws_bytes, int32_t C, const int32_t* cols, const int32_t* active,
+                       void* X, void* stream) {
+  if (int rc = check_op(op)) return rc;
+  if (C <= 0 || !cols || !active || !X) return set_error(POLAR_E_ARG, "bad history_cols arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (op->dtype == POLAR_F64)
+    return history_cols_impl<double>(op, R, p, nA, flags, tcap, ws, ws_bytes, C, cols, active, (double*)X, st);
+  return history_cols_impl<float>(op, R, p, nA, flags, tcap, ws, ws_bytes, C, cols, active, (float*)X, st);
 }
 
 }  // extern "C"

@@ -310,8 +310,55 @@ def _col_bytes(n, p, nA, dtype, block=None, replace=False):
     return esz * (nvec * n + p * nA) + ((16 + esz) * n if replace else 0) + 64
 
 
+HISTORY_BYTES = 16 << 30      # residual history budget of one re-rank solve
+HISTORY_STEPS = 1024          # iterations a history solve keeps at most
+
+
+def history_cap(n, R, dtype):
+    """Iterations a FP32 re-rank solve of R columns on n rows keeps in history mode (0: it would keep
+    too few to be worth it, and the solve forms its shifted blocks as usual)."""
+    if dtype != torch.float32:
+        return 0
+    cap = min(HISTORY_STEPS, HISTORY_BYTES // max(1, n * R * 4))
+    return int(cap) if cap >= 256 else 0
+
+
+class History:
+    """A solve run with POLAR_HISTORY: every residual and every pole's per-step coefficients are kept
+    in its workspace, and X_j is formed only where it is needed -- on two rows of every column (the
+    re-rank's endpoint values, polar_gpu.py:230) and on every active row of the winner's column
+    (:237), instead of on every active row of every column during the solve."""
+
+    def __init__(self, op, s, R, p, nA, flags, tcap, ws, dtype):
+        self.op, self.s, self.R, self.p, self.nA = op, s, R, p, nA
+        self.flags, self.tcap, self.ws, self.dtype = flags, tcap, ws, dtype
+
+    def _args(self):
+        return (self.s, self.R, self.p, self.nA, self.flags, self.tcap, L.ptr(self.ws), self.ws.numel())
+
+    def rows(self, rows):
+        """E[j, k, c] = X_j[rows[k, c], c]; rows: device int32 (K, R) absolute row ids."""
+        lib = L.require_gpu()
+        rows = rows.to(torch.int32).contiguous()
+        K = rows.shape[0]
+        E = torch.empty((self.p, K, self.R), dtype=self.dtype, device=self.ws.device)
+        L.check(lib.polar_history_rows(*self._args(), K, L.ptr(rows), L.ptr(E), L.stream_ptr()), 'polar_history_rows')
+        return E
+
+    def cols(self, cols, active):
+        """X[i, j, a] = X_j[active[a], cols[i]]; cols: device int32 (C,), active: device int32 (nA,)."""
+        lib = L.require_gpu()
+        cols = cols.to(torch.int32).contiguous()
+        X = torch.empty((cols.numel(), self.p, self.nA), dtype=self.dtype, device=self.ws.device)
+        for lo in range(0, cols.numel(), 4):        # at most 4 columns per call
+            C = min(4, cols.numel() - lo)
+            L.check(lib.polar_history_cols(*self._args(), C, L.ptr(cols[lo:]), L.ptr(active), L.ptr(X[lo:]),
+                                           L.stream_ptr()), 'polar_history_cols')
+        return X
+
+
 def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, freeze=True,
-           trace=False, out=None, width=None, graph=None, block=None, replace=False):
+           trace=False, out=None, width=None, graph=None, block=None, replace=False, history=0):
     """Block multi-shift CG for (M + tau_j I) U = W on all poles (polar_gpu.py:88-133).
 
     Returns (X[p, nA, R], h[p, R] or None, iters[R] host int64) and, with trace=True, a 4th value
@@ -324,7 +371,9 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
     scalars, counts and tracked norms; X within rounding; ~8x fewer passes over the shifted
     blocks); None = on for FP32, off for FP64 (whose default path is the pinned one).
     replace=True: residual replacement every 24 iterations (r = w - (M + tau0) x0 with the seed
-    iterate x0 accumulated in FP64 and the product in FP64)."""
+    iterate x0 accumulated in FP64 and the product in FP64).  history=T > 0: POLAR_HISTORY with T
+    iterations at most (maxit is then T; more raises the iteration-cap error): X is not formed, the
+    first value returned is None and a History is appended to the results."""
     lib = L.require_gpu()
     dev = _device()
     dtype = op.dtype
@@ -344,8 +393,11 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
         act_d = torch.as_tensor(np.asarray(active.cpu() if torch.is_tensor(active) else active),
                                 dtype=torch.int32).to(dev)
     nA = act_d.numel()
-    X = out if out is not None else torch.empty((p, nA, R), dtype=dtype, device=dev)
-    assert X.shape == (p, nA, R) and X.stride(2) == 1
+    if history:
+        X = None
+    else:
+        X = out if out is not None else torch.empty((p, nA, R), dtype=dtype, device=dev)
+        assert X.shape == (p, nA, R) and X.stride(2) == 1
     h = torch.empty((p, R), dtype=torch.float64, device=dev) if track else None
     te = torch.empty((p, R), dtype=torch.float64, device=dev) if trace else None
     iters = torch.empty(R, dtype=torch.int64, device=dev)
@@ -356,7 +408,14 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
              (L.POLAR_BLOCK8 if _blocked(block, dtype) else 0) | (L.POLAR_REPLACE if replace else 0))
     s = op.c_op(dtype)
     step = min(R, _max_cols(dtype), width or R)
-    ws_bytes = lib.polar_msolve_ws_bytes(s, step, p, nA, flags)
+    if history:
+        assert R <= _max_cols(dtype)
+        step, maxit = R, int(history)
+        flags |= L.POLAR_HISTORY
+        ws_bytes = lib.polar_msolve_history_ws_bytes(s, R, p, nA, flags, maxit)
+    else:
+        ws_bytes = lib.polar_msolve_ws_bytes(s, step, p, nA, flags)
+    L.check(0 if ws_bytes > 0 else L.POLAR_E_ARG, 'msolve workspace size')
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     for lo in range(0, R, step):
         r = min(step, R - lo)
@@ -364,9 +423,9 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
         hb = (h if r == R else torch.empty((p, r), dtype=torch.float64, device=dev)) if track else None
         tb = (te if r == R else torch.empty((p, r), dtype=torch.float64, device=dev)) if trace else None
         rc = lib.polar_msolve(s, L.ptr(W[:, lo:]), W.stride(0), r, L.ptr(tau_d), p, L.ptr(act_d), nA,
-                              float(tol), int(maxit), flags, L.ptr(X[:, :, lo:]), X.stride(0),
-                              X.stride(1), L.ptr(hb), L.ptr(tb), L.ptr(iters[lo:]), L.ptr(ws),
-                              ws_bytes, L.stream_ptr())
+                              float(tol), int(maxit), flags, None if X is None else L.ptr(X[:, :, lo:]),
+                              0 if X is None else X.stride(0), 0 if X is None else X.stride(1), L.ptr(hb),
+                              L.ptr(tb), L.ptr(iters[lo:]), L.ptr(ws), ws_bytes, L.stream_ptr())
         L.check(rc, 'polar_msolve')
         if track and r != R:
             h[:, lo:lo + r] = hb
@@ -380,6 +439,8 @@ def msolve(op, W, tau, active, tol, maxit=100000, track=False, counter=None, fre
     res = (X, h, it_h)
     if trace:
         res = res + (te,)
+    if history:
+        res = res + (History(op, s, R, p, nA, flags, maxit, ws, dtype),)
     return res
 
 
@@ -578,7 +639,7 @@ class _Phases:
 def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=None, naive=False,
           seed=2026, kind='block', trace=None, time_limit=None, poles=None, width=None,
           freeze=True, precision='fp64', profile=False, progress=None, graph=None, tune=None, block=None,
-          shortlist='proxy', replace=None):
+          shortlist='proxy', replace=None, history=None):
     """POLAR greedy, cache backend (polar_gpu.py:180-254), on the GPU.
 
     profile=True adds `phases`: seconds per phase (bank, select, rerank_solve, rerank_pick,
@@ -590,6 +651,11 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     (minus the sketched trace's derivative, see polar_amd.adjoint) instead of the reference's
     proxy; the re-rank by exact marginals is unchanged.  replace: FP64 residual replacement every
     24 iterations in every solve (see msolve); None = on for FP32 (the FP32-mixed form), off for FP64.
+    history: the FP32 re-rank solve keeps its residuals and forms the shifted solutions only on
+    the shortlist's endpoint rows and the winner's column (msolve history=, History); None = when
+    its residual history fits (history_cap), False = never (the re-rank forms every column's
+    shifted blocks, as the FP64 path and polar_amd.batched.polar_batched always do).  Both agree
+    within FP32 rounding.
     trace: a list that receives per round dict(shortlist, chosen) as in the reference, plus `proxy`
     (the shortlist's proxy scores) and `scores` (its re-rank scores; empty for naive).
     progress: optional callable(stage, info) called after the bank ('bank', {seconds, spmv}) and
@@ -678,6 +744,8 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     coef = torch.empty((p, 2 * q + 3), dtype=torch.float64, device=dev)
     wX = wY = wU = None                 # maintenance buffers, allocated after the bank solves
     gvec = torch.empty(max(B, 1) + 2, dtype=torch.float64, device=dev)
+    # endpoint row indices 0 / 1 of a (p, 2, Br) block of endpoint values (history re-rank)
+    z01 = torch.arange(2, dtype=torch.int32, device=dev)[:, None].repeat(1, max(B, 1)).contiguous()
     gc_row = torch.empty(q, dtype=torch.float64, device=dev)
     tr_hist = []
     _sync()
@@ -748,15 +816,29 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             L.check(lib.polar_pair_rhs(dcode, n, Br, L.ptr(a_idx), L.ptr(b_idx), L.ptr(Wb), stream), 'pair_rhs')
             need_track = obj != 'logdet'
             ph.mark('select')
-            if Ubuf is None:                # one re-rank block for all rounds
-                Ubuf = torch.empty((p, nA, max(B, 1)), dtype=dtype, device=dev)
-            U = Ubuf[:, :, :Br]
-            rw = fit_width(n, p, nA, Br, dtype, block=block)
-            U, hh, _ = msolve(op, Wb, tau, active, tol, track=need_track, counter=cnt, freeze=freeze, out=U,
-                              width=rw, graph=graph, block=block, replace=replace)[:3]
+            H = None
+            hcap = history_cap(n, Br, dtype) if history is not False else 0
+            if hcap:
+                # FP32: keep the residuals, form X_j only at the shortlisted endpoints and the winner
+                try:
+                    _, hh, _, H = msolve(op, Wb, tau, active, tol, track=need_track, counter=cnt, freeze=freeze,
+                                         graph=graph, block=block, replace=replace, history=hcap)
+                except RuntimeError as ex:
+                    if 'iteration cap' not in str(ex):
+                        raise
+            if H is None:
+                if Ubuf is None:                # one re-rank block for all rounds
+                    Ubuf = torch.empty((p, nA, max(B, 1)), dtype=dtype, device=dev)
+                U = Ubuf[:, :, :Br]
+                rw = fit_width(n, p, nA, Br, dtype, block=block)
+                U, hh, _ = msolve(op, Wb, tau, active, tol, track=need_track, counter=cnt, freeze=freeze, out=U,
+                                  width=rw, graph=graph, block=block, replace=replace)[:3]
+                sa = torch.from_numpy(es_h[short, 0].astype(np.int32)).to(dev)
+                sb = torch.from_numpy(es_h[short, 1].astype(np.int32)).to(dev)
+            else:
+                U = H.rows(torch.stack([a_idx, b_idx]))        # (p, 2, Br): U at the two endpoints
+                sa, sb = z01[0], z01[1]
             ph.mark('rerank_solve')
-            sa = torch.from_numpy(es_h[short, 0].astype(np.int32)).to(dev)
-            sb = torch.from_numpy(es_h[short, 1].astype(np.int32)).to(dev)
             win = np.zeros(1, dtype=np.int32)
             val = np.zeros(1, dtype=np.float64)
             L.check(lib.polar_rerank_pick(objc, dcode, L.ptr(U), U.stride(0), U.stride(1), p, Br, L.ptr(sa), L.ptr(sb),
@@ -764,7 +846,10 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
                                           val.ctypes.data_as(L.vp), stream), 'rerank_pick')
             j = 0 if Br == 1 else int(win[0])
             e = int(short[j])
-            u = U[:, :, j]
+            if H is None:
+                u = U[:, :, j]
+            elif rnd + 1 < k:                   # the winner's column, for the maintenance
+                u = H.cols(torch.full((1,), j, dtype=torch.int32, device=dev), active)[0]
             marg.append(float(val[0]))
             if Br >= 2:
                 gaps_d[rnd, 2:4] = torch.topk(gvec[:Br], 2).values

@@ -96,7 +96,8 @@ def test_polar_batched_equals_per_graph(precision):
     kk = [min(k, len(c)) for c in cands]
     out = batched.polar_batched(Gs, cands, k, 0.5, q=q, B=B, sketch=sk, poles=poles, precision=precision)
     for g, G in enumerate(Gs):
-        ref = pg.polar(G, cands[g], kk[g], 0.5, q=q, B=B, sketch=sk[g], poles=poles[g], precision=precision)
+        ref = pg.polar(G, cands[g], kk[g], 0.5, q=q, B=B, sketch=sk[g], poles=poles[g], precision=precision,
+                       history=False)
         got = out[g]
         assert got['sequence'][:kk[g]] == ref['sequence'], g
         assert got['marginals'][:kk[g]] == ref['marginals'], g
@@ -210,7 +211,7 @@ def test_polar_batched_single_pole_objectives_equal_per_graph(obj):
           for g, G in enumerate(Gs)]
     out = batched.polar_batched(Gs, cands, k, 0.5, q=q, B=B, sketch=sk, obj=obj, precision='fp64')
     for g, G in enumerate(Gs):
-        ref = pg.polar(G, cands[g], k, 0.5, q=q, B=B, sketch=sk[g], obj=obj, precision='fp64')
+        ref = pg.polar(G, cands[g], k, 0.5, q=q, B=B, sketch=sk[g], obj=obj, precision='fp64', history=False)
         assert out[g]['sequence'] == ref['sequence'], g
         assert out[g]['marginals'] == ref['marginals'], g
         assert (out[g]['spmv'], out[g]['rhs'], out[g]['p']) == (ref['spmv'], ref['rhs'], 1), g

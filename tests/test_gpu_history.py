@@ -1,0 +1,144 @@
+"""History re-rank solves (POLAR_HISTORY): the solve keeps every residual and every pole's per-step
+coefficients, and the shifted solutions X_j are formed afterwards only where the re-rank needs
+them: on the two endpoint rows of every shortlisted column and on every active row of the
+winner's column.
+
+* The CG itself is the blocked FP32 solve's: iteration counts, tracked norms and trace terms are
+  equal bit for bit (the finisher only records more).
+* X_j formed from the history equals the window-flushed X_j within the FP32 rounding of the
+  window flush (relative to the column's largest entry, 1e-5), with and without residual
+  replacement, with column groups; in FP64 the two agree to 1e-12.
+* polar(), polar_columns(): the history re-rank selects the same edges as the flushed re-rank,
+  with marginals and trace estimates within 1e-6, and a history too short for the solve falls
+  back to the flushed solve with the same results.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip('torch')
+pytestmark = pytest.mark.gpu
+
+from polar_amd import batched, configs, graphs  # noqa: E402
+from polar_amd import polar_gpu as pg  # noqa: E402
+
+
+def _graph(n=3000, seed=5):
+    e = graphs.chung_lu(n, 2.3, 8.0, seed)
+    hubs = np.array([[0, j] for j in range(1, 400)])
+    return pg.Graph(n, graphs.simple_edges(np.concatenate([e, hubs]), n), 0.2)
+
+
+def _rhs(G, R, seed):
+    """Re-rank-like right-hand sides e_a - e_b."""
+    rng = np.random.default_rng(seed)
+    W = np.zeros((G.n, R))
+    for c in range(R):
+        a, b = rng.choice(G.n, 2, replace=False)
+        W[a, c], W[b, c] = 1.0, -1.0
+    return W
+
+
+@pytest.mark.parametrize('precision,replace', [('fp32', False), ('fp32', True), ('fp64', False)])
+def test_history_solve_matches_flushed_solve(precision, replace):
+    G = _graph()
+    R, tol = 12, 1e-9
+    dt = torch.float32 if precision == 'fp32' else torch.float64
+    W = torch.from_numpy(_rhs(G, R, 1)).to('cuda', dt)
+    tau = np.array([0.0, 0.03, 0.3, 3.0, 30.0, 300.0])
+    act_h = np.arange(0, G.n, 2)
+    act = torch.from_numpy(act_h.astype(np.int32)).cuda()
+    op = pg.Op(G, precision=precision)
+    X, h, it, te = pg.msolve(op, W, tau, act, tol, track=True, trace=True, block=True, replace=replace)
+    _, h2, it2, te2, H = pg.msolve(op, W, tau, act, tol, track=True, trace=True, block=True, replace=replace,
+                                   history=512)
+    assert np.array_equal(it, it2) and it.max() > 40
+    assert torch.equal(h, h2) and torch.equal(te, te2)
+    bound = 1e-5 if precision == 'fp32' else 1e-12
+    Xd = X.double()
+    scale = Xd.abs().amax(dim=1, keepdim=True)                    # per (pole, column)
+    # every column on every active row
+    Xh = H.cols(torch.arange(R, dtype=torch.int32, device='cuda'), act).double()   # (R, p, nA)
+    err = (Xh.permute(1, 2, 0) - Xd).abs() / scale
+    assert float(err.max()) < bound
+    # two chosen rows of every column (absolute row ids among the active rows)
+    rng = np.random.default_rng(2)
+    slots = rng.integers(0, len(act_h), size=(2, R))
+    rows = torch.from_numpy(act_h[slots].astype(np.int32)).cuda()
+    E = H.rows(rows).double()                                     # (p, 2, R)
+    want = torch.stack([Xd[:, torch.from_numpy(slots[k]).cuda(), torch.arange(R, device='cuda')] for k in range(2)], 1)
+    assert float(((E - want).abs() / scale.squeeze(1)[:, None, :]).max()) < bound
+
+
+def test_history_with_column_groups_matches_flushed_solve():
+    G = _graph()
+    R, p = 12, 6
+    cg = np.repeat(np.arange(3), 4).astype(np.int32)
+    tau = np.stack([np.geomspace(1e-3, 100.0, p) * f for f in (1.0, 1.5, 2.0)])
+    segs = batched.single_segment(G, G.n)
+    act = torch.arange(G.n, dtype=torch.int32, device='cuda')
+    W = torch.from_numpy(_rhs(G, R, 3)).float().cuda()
+    op = pg.Op(G, precision='fp32')
+    X, h, it = batched.msolve_batched(op, segs, W, tau, act, 1e-8, track=True, block=True, replace=True,
+                                      col_group=cg)[:3]
+    _, h2, it2, H = batched.msolve_batched(op, segs, W, tau, act, 1e-8, track=True, block=True, replace=True,
+                                           col_group=cg, history=512)
+    assert np.array_equal(it, it2) and torch.equal(h, h2)
+    Xh = H.cols(torch.arange(R, dtype=torch.int32, device='cuda'), act).double().permute(1, 2, 0)
+    Xd = X.double()
+    assert float(((Xh - Xd).abs() / Xd.abs().amax(dim=1, keepdim=True)).max()) < 1e-5
+
+
+def test_history_cap_reached_raises_iteration_cap():
+    G = _graph()
+    W = torch.from_numpy(_rhs(G, 4, 4)).float().cuda()
+    act = torch.arange(G.n, dtype=torch.int32, device='cuda')
+    with pytest.raises(RuntimeError, match='iteration cap'):
+        pg.msolve(pg.Op(G, precision='fp32'), W, np.array([0.0, 1.0]), act, 1e-12, block=True, history=16)
+
+
+def _greedy_pair(fn, monkeypatch):
+    """fn() with history re-rank solves, with flushed ones, and with a history cap too short."""
+    out = [fn()]
+    monkeypatch.setattr(pg, 'history_cap', lambda n, R, dtype: 0)
+    out.append(fn())
+    monkeypatch.setattr(pg, 'history_cap', lambda n, R, dtype: 8 if dtype == torch.float32 else 0)
+    out.append(fn())
+    return out
+
+
+def _same(a, b):
+    assert a['sequence'] == b['sequence']
+    assert (a['spmv'], a['rhs']) == (b['spmv'], b['rhs'])
+    m_a, m_b = np.array(a['marginals']), np.array(b['marginals'])
+    assert np.max(np.abs(m_a - m_b) / np.abs(m_b)) < 1e-6
+    t_a, t_b = np.array(a['trace_estimate']), np.array(b['trace_estimate'])
+    assert np.max(np.abs(t_a - t_b) / np.abs(t_b)) < 1e-6
+
+
+def test_polar_history_rerank_equals_flushed_rerank(monkeypatch):
+    c = configs.make('c2', 0.02)
+    G = pg.Graph(c['n'], c['edges'], c['sigma'])
+    sk = pg.sketches(G, c['cand'], 16, c['seed'], kind='rademacher', on_device='lazy')
+    assert pg.history_cap(G.n, 4, torch.float32) > 0
+
+    def run():
+        return pg.polar(G, c['cand'], 6, c['s'], q=16, B=4, tol=c['tol'], sketch=sk, poles=(c['tau'], c['om']),
+                        precision='fp32')
+    hist, flushed, capped = _greedy_pair(run, monkeypatch)
+    _same(hist, flushed)
+    _same(capped, flushed)
+
+
+def test_polar_columns_history_rerank_equals_flushed_rerank(monkeypatch):
+    c = configs.make('c2', 0.02)
+    G = pg.Graph(c['n'], c['edges'], c['sigma'])
+    s_list = [0.25, 0.5, 0.75]
+    poles = [pg.fixed_rule(s, 8, G.sigma, G.Lam(5))[:2] for s in s_list]
+
+    def run():
+        sk = pg.sketches(G, c['cand'], 16, c['seed'], kind='rademacher', on_device='lazy')
+        return batched.polar_columns(G, c['cand'], 5, s_list, q=16, B=3, sketch=sk, poles=poles, precision='fp32')
+    hist, flushed, capped = _greedy_pair(run, monkeypatch)
+    for a, b, d in zip(hist, flushed, capped):
+        _same(a, b)
+        _same(d, b)
