@@ -2795,13 +2795,13 @@ static int history_cols_impl(const polar_op* op, int64_t R, int p, int64_t nA, i
   SolveMem<T> m;
   if (int rc = history_mem<T>(op, R, p, nA, flags, tcap, ws, ws_bytes, m)) return rc;
   if (p > 64) return set_error(POLAR_E_ARG, "polar_history_cols supports at most 64 poles (p=%d)", p);
+  if (C > kHistMaxC) return set_error(POLAR_E_ARG, "polar_history_cols takes at most %d columns per call", kHistMaxC);
+  const int64_t blocks = (nA * C + 255) / 256;
+  if (blocks > INT32_MAX) return set_error(POLAR_E_ARG, "too many rows x columns");
   const CGState& s = m.s;
   hipLaunchKernelGGL(k_hist_coef, dim3((C * p + 255) / 256), dim3(256), 0, st, s.hm, s.haj, s.hbj, s.hzn, s.hist_T,
                      p, (int)R, tcap, cols, C, m.coef);
   POLAR_LAUNCH_CHECK();
-  if (C > kHistMaxC) return set_error(POLAR_E_ARG, "polar_history_cols takes at most %d columns per call", kHistMaxC);
-  const int64_t blocks = (nA * C + 255) / 256;
-  if (blocks > INT32_MAX) return set_error(POLAR_E_ARG, "too many rows x columns");
   const dim3 grid((unsigned)blocks);
   if (p <= 32)
     hipLaunchKernelGGL((k_hist_cols<T, 32>), grid, dim3(256), 0, st, (const T*)m.ring, op->n, (int)R, p, tcap,

@@ -168,3 +168,30 @@ def test_edgeless_graph_greedy_matches_oracle_and_closed_form():
         if u not in touched and v not in touched:
             assert abs(rg['marginals'][r] - closed) <= 1e-13 * closed
         touched |= {u, v}
+
+
+def test_history_entry_points_check_arguments():
+    """polar_history_rows / polar_history_cols: no POLAR_HISTORY flag, a workspace smaller than the
+    history's, no iterations kept, more than 4 columns per call, no rows."""
+    lib = L.require_gpu()
+    op, s, dt, n = _small_op('fp32')
+    R, p, nA, tcap = 4, 3, n, 32
+    fl = L.POLAR_HISTORY | L.POLAR_BLOCK8
+    need = lib.polar_msolve_history_ws_bytes(s, R, p, nA, fl, tcap)
+    assert need > 0
+    ws = torch.zeros(need, dtype=torch.uint8, device='cuda')
+    rows = torch.zeros((2, R), dtype=torch.int32, device='cuda')
+    E = torch.empty((p, 2, R), dtype=dt, device='cuda')
+    cols = torch.zeros(5, dtype=torch.int32, device='cuda')
+    act = torch.arange(n, dtype=torch.int32, device='cuda')
+    X = torch.empty((5, p, nA), dtype=dt, device='cuda')
+    sp = L.stream_ptr()
+    rows_call = lambda f, nb, t, K: lib.polar_history_rows(s, R, p, nA, f, t, L.ptr(ws), nb, K, L.ptr(rows), L.ptr(E), sp)
+    _expect_arg(rows_call(L.POLAR_BLOCK8, need, tcap, 2), 'without POLAR_HISTORY')
+    _expect_arg(rows_call(fl, need - 256, tcap, 2), 'too small')
+    _expect_arg(rows_call(fl, need, 0, 2), 'bad history arguments')
+    _expect_arg(rows_call(fl, need, tcap, 0), 'bad history_rows arguments')
+    _expect_arg(lib.polar_history_cols(s, R, p, nA, fl, tcap, L.ptr(ws), need, 5, L.ptr(cols), L.ptr(act), L.ptr(X), sp),
+                'at most 4 columns')
+    _expect_arg(lib.polar_history_cols(s, R, p, nA, fl, tcap, L.ptr(ws), need, 2, None, L.ptr(act), L.ptr(X), sp),
+                'bad history_cols arguments')
