@@ -430,8 +430,9 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
     re-rank solves as column groups (same results either way).  replace: FP64 residual replacement
     in every solve; None = on for FP32, off for FP64.
 
-    history: the FP32 re-rank solve forms the shifted solutions only on the shortlists' endpoint
-    rows and the winners' columns (polar_gpu.polar's `history`); None = when it fits, False = never.
+    history: the re-rank solve with the blocked update forms the shifted solutions only on the
+    shortlists' endpoint rows and the winners' columns (polar_gpu.polar's `history`); None = when it
+    fits, False = never.
 
     shared_bank: the groups' bank right-hand sides [Z | F] are the same, so one seed CG can serve all
     of them: the seed is the smallest pole of all groups and every group's poles are its shifted
@@ -631,7 +632,7 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
         Wb.index_put_((ib, ic), torch.full((Rt,), -1.0, dtype=dtype, device=dev))
         cg = np.concatenate([np.full(len(sh), g, np.int32) for g, sh in enumerate(shorts)])
         H = None
-        hcap = pg.history_cap(n, Rt, dtype) if history is not False else 0
+        hcap = pg.history_cap(n, Rt, dtype, block) if history is not False else 0
         if hcap:
             # FP32: keep the residuals, form X_j only at the shortlisted endpoints and the winners
             try:

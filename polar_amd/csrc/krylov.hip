@@ -2643,25 +2643,62 @@ static int scratch_address(void** out) {
 // so that their latencies overlap.
 constexpr int kHistBatch = 16;
 
+// One wave per (selected column i, pole j).  Lane l takes the steps [t0, t1) of the l-th of 64 equal
+// segments: pass 1 composes the segment's map G_{t0} = A + B G_{t1} (the identity at frozen steps),
+// a backward scan over the lanes composes the maps of all later segments, which gives every
+// segment's G_{t1} (G_T = 0 after the last), and pass 2 runs the recursion inside the segment from
+// there and writes the segment's weights.
 __global__ __launch_bounds__(256) void k_hist_coef(const uint8_t* __restrict__ hm, const double* __restrict__ haj,
                                                    const double* __restrict__ hbj, const double* __restrict__ hzn,
                                                    const int* __restrict__ hist_T, int p, int ld, int64_t tcap,
                                                    const int* __restrict__ cols, int C, double* __restrict__ coef) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;   // (selected column i, pole j)
-  if (idx >= C * p) return;
-  const int i = idx / p, j = idx - i * p;
+  const int lane = threadIdx.x & 63;
+  const int pair = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (pair >= C * p) return;               // the whole wave
+  const int i = pair / p, j = pair - i * p;
   const int c = cols ? cols[i] : i;
   const int Tn = hist_T[0];
   double* w = coef + ((int64_t)c * p + j) * (tcap + 1);
-  double G = 0.0;
-  for (int t = Tn - 1; t >= 0; t -= kHistBatch) {
-    const int nb = min(kHistBatch, t + 1);
+  const int seg = (Tn + 63) / 64;
+  const int t0 = min(Tn, lane * seg), t1 = min(Tn, t0 + seg);
+  double A = 0.0, B = 1.0;
+  for (int tb = t1 - 1; tb >= t0; tb -= kHistBatch) {
+    const int nb = min(kHistBatch, tb - t0 + 1);
+    uint8_t mm[kHistBatch];
+    double aa[kHistBatch], bb[kHistBatch];
+#pragma unroll
+    for (int u = 0; u < kHistBatch; ++u)
+      if (u < nb) {
+        const int64_t hx = ((int64_t)(tb - u) * p + j) * ld + c;
+        mm[u] = hm[hx];
+        aa[u] = haj[hx];
+        bb[u] = hbj[hx];
+      }
+#pragma unroll
+    for (int u = 0; u < kHistBatch; ++u)
+      if (u < nb && mm[u]) {
+        A = aa[u] + bb[u] * A;
+        B = bb[u] * B;
+      }
+  }
+  // lane l: f_l o f_{l+1} o ... o f_63
+  for (int off = 1; off < 64; off <<= 1) {
+    const double A2 = __shfl_down(A, off, 64), B2 = __shfl_down(B, off, 64);
+    if (lane + off < 64) {
+      A = A + B * A2;
+      B = B * B2;
+    }
+  }
+  double G = __shfl_down(A, 1, 64);
+  if (lane == 63) G = 0.0;
+  for (int tb = t1 - 1; tb >= t0; tb -= kHistBatch) {
+    const int nb = min(kHistBatch, tb - t0 + 1);
     uint8_t mm[kHistBatch];
     double aa[kHistBatch], bb[kHistBatch], zz[kHistBatch];
 #pragma unroll
     for (int u = 0; u < kHistBatch; ++u)
       if (u < nb) {
-        const int64_t hx = ((int64_t)(t - u) * p + j) * ld + c;
+        const int64_t hx = ((int64_t)(tb - u) * p + j) * ld + c;
         mm[u] = hm[hx];
         aa[u] = haj[hx];
         bb[u] = hbj[hx];
@@ -2671,42 +2708,49 @@ __global__ __launch_bounds__(256) void k_hist_coef(const uint8_t* __restrict__ h
     for (int u = 0; u < kHistBatch; ++u)
       if (u < nb) {
         if (mm[u]) {
-          w[t - u + 1] = zz[u] * G;
+          w[tb - u + 1] = zz[u] * G;
           G = aa[u] + bb[u] * G;
         } else {
-          w[t - u + 1] = 0.0;
+          w[tb - u + 1] = 0.0;
         }
       }
   }
-  w[0] = G;
+  if (lane == 0) w[0] = G;
 }
 
-// E[(j*K + k)*R + c] = X_j[rows[k*R + c], c]: one thread per value, steps in increasing order.
+// E[(j*K + k)*R + c] = X_j[rows[k*R + c], c]: one wave per value, lane l summing the steps
+// t = l (mod 64), the lanes' FP64 sums added across the wave.
 template <typename T>
 __global__ __launch_bounds__(256) void k_hist_rows(const T* __restrict__ ring, int64_t n, int ld, int p, int64_t tcap,
                                                    const int* __restrict__ hist_T, const double* __restrict__ coef,
                                                    int R, int K, const int32_t* __restrict__ rows, T* __restrict__ E) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= p * K * R) return;
+  constexpr int kU = 4;
+  const int lane = threadIdx.x & 63;
+  const int idx = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (idx >= p * K * R) return;            // the whole wave
   const int c = idx % R, k = (idx / R) % K, j = idx / (R * K);
   const int64_t row = rows[k * R + c];
   const int Tn = hist_T[0];
   const double* w = coef + ((int64_t)c * p + j) * (tcap + 1);
   double acc = 0.0;
-  for (int t0 = 0; t0 <= Tn; t0 += kHistBatch) {
-    const int nb = min(kHistBatch, Tn + 1 - t0);
-    double wt[kHistBatch], v[kHistBatch];
+  for (int tb = lane; tb <= Tn; tb += 64 * kU) {
+    double wt[kU], v[kU];
 #pragma unroll
-    for (int u = 0; u < kHistBatch; ++u)
-      if (u < nb) {
-        wt[u] = w[t0 + u];
-        v[u] = (double)ring[(int64_t)(t0 + u) * n * ld + row * ld + c];
+    for (int u = 0; u < kU; ++u) {
+      const int t = tb + 64 * u;
+      wt[u] = 0.0;
+      v[u] = 0.0;
+      if (t <= Tn) {
+        wt[u] = w[t];
+        v[u] = (double)ring[(int64_t)t * n * ld + row * ld + c];
       }
+    }
 #pragma unroll
-    for (int u = 0; u < kHistBatch; ++u)
-      if (u < nb && wt[u] != 0.0) acc = acc + wt[u] * v[u];
+    for (int u = 0; u < kU; ++u)
+      if (wt[u] != 0.0) acc = acc + wt[u] * v[u];
   }
-  E[idx] = (T)acc;
+  for (int off = 32; off > 0; off >>= 1) acc = acc + __shfl_xor(acc, off, 64);
+  if (lane == 0) E[idx] = (T)acc;
 }
 
 // X[(i*p + j)*nA + a] = X_j[active[a], cols[i]]: one thread per (active row, selected column), the C
@@ -2777,12 +2821,14 @@ static int history_rows_impl(const polar_op* op, int64_t R, int p, int64_t nA, i
                              int64_t ws_bytes, int K, const int32_t* rows, T* E, hipStream_t st) {
   SolveMem<T> m;
   if (int rc = history_mem<T>(op, R, p, nA, flags, tcap, ws, ws_bytes, m)) return rc;
+  const int64_t tot = (int64_t)p * K * R;
+  if (tot > INT32_MAX || R * p > INT32_MAX) return set_error(POLAR_E_ARG, "too many history values");
   const CGState& s = m.s;
-  hipLaunchKernelGGL(k_hist_coef, dim3((int)((R * p + 255) / 256)), dim3(256), 0, st, s.hm, s.haj, s.hbj, s.hzn,
+  // one wave per (column, pole) pair, then one wave per value: 4 waves per block
+  hipLaunchKernelGGL(k_hist_coef, dim3((int)((R * p + 3) / 4)), dim3(256), 0, st, s.hm, s.haj, s.hbj, s.hzn,
                      s.hist_T, p, (int)R, tcap, (const int*)nullptr, (int)R, m.coef);
   POLAR_LAUNCH_CHECK();
-  const int64_t tot = (int64_t)p * K * R;
-  hipLaunchKernelGGL((k_hist_rows<T>), dim3((int)((tot + 255) / 256)), dim3(256), 0, st, (const T*)m.ring, op->n,
+  hipLaunchKernelGGL((k_hist_rows<T>), dim3((int)((tot + 3) / 4)), dim3(256), 0, st, (const T*)m.ring, op->n,
                      (int)R, p, tcap, s.hist_T, (const double*)m.coef, (int)R, K, rows, E);
   POLAR_LAUNCH_CHECK();
   return 0;
@@ -2799,7 +2845,7 @@ static int history_cols_impl(const polar_op* op, int64_t R, int p, int64_t nA, i
   const int64_t blocks = (nA * C + 255) / 256;
   if (blocks > INT32_MAX) return set_error(POLAR_E_ARG, "too many rows x columns");
   const CGState& s = m.s;
-  hipLaunchKernelGGL(k_hist_coef, dim3((C * p + 255) / 256), dim3(256), 0, st, s.hm, s.haj, s.hbj, s.hzn, s.hist_T,
+  hipLaunchKernelGGL(k_hist_coef, dim3((C * p + 3) / 4), dim3(256), 0, st, s.hm, s.haj, s.hbj, s.hzn, s.hist_T,
                      p, (int)R, tcap, cols, C, m.coef);
   POLAR_LAUNCH_CHECK();
   const dim3 grid((unsigned)blocks);

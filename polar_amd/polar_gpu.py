@@ -314,12 +314,15 @@ HISTORY_BYTES = 16 << 30      # residual history budget of one re-rank solve
 HISTORY_STEPS = 1024          # iterations a history solve keeps at most
 
 
-def history_cap(n, R, dtype):
-    """Iterations a FP32 re-rank solve of R columns on n rows keeps in history mode (0: it would keep
-    too few to be worth it, and the solve forms its shifted blocks as usual)."""
-    if dtype != torch.float32:
+def history_cap(n, R, dtype, block=None):
+    """Iterations a re-rank solve of R columns on n rows keeps in history mode: with the blocked
+    update (the FP32 default, FP64 with block=True) when at least 256 iterations fit in
+    HISTORY_BYTES, else 0 (the solve forms its shifted blocks as usual).  The pinned FP64 path never
+    uses it: its per-iteration update is the oracle's arithmetic."""
+    if not _blocked(block, dtype):
         return 0
-    cap = min(HISTORY_STEPS, HISTORY_BYTES // max(1, n * R * 4))
+    esz = 8 if dtype == torch.float64 else 4
+    cap = min(HISTORY_STEPS, HISTORY_BYTES // max(1, n * R * esz))
     return int(cap) if cap >= 256 else 0
 
 
@@ -651,11 +654,11 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
     (minus the sketched trace's derivative, see polar_amd.adjoint) instead of the reference's
     proxy; the re-rank by exact marginals is unchanged.  replace: FP64 residual replacement every
     24 iterations in every solve (see msolve); None = on for FP32 (the FP32-mixed form), off for FP64.
-    history: the FP32 re-rank solve keeps its residuals and forms the shifted solutions only on
-    the shortlist's endpoint rows and the winner's column (msolve history=, History); None = when
-    its residual history fits (history_cap), False = never (the re-rank forms every column's
-    shifted blocks, as the FP64 path and polar_amd.batched.polar_batched always do).  Both agree
-    within FP32 rounding.
+    history: the re-rank solve with the blocked update keeps its residuals and forms the shifted
+    solutions only on the shortlist's endpoint rows and the winner's column (msolve history=,
+    History); None = when its residual history fits (history_cap), False = never (the re-rank forms
+    every column's shifted blocks, as the pinned FP64 path and polar_amd.batched.polar_batched
+    always do).  Both agree within the rounding of the vectors' precision.
     trace: a list that receives per round dict(shortlist, chosen) as in the reference, plus `proxy`
     (the shortlist's proxy scores) and `scores` (its re-rank scores; empty for naive).
     progress: optional callable(stage, info) called after the bank ('bank', {seconds, spmv}) and
@@ -817,7 +820,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             need_track = obj != 'logdet'
             ph.mark('select')
             H = None
-            hcap = history_cap(n, Br, dtype) if history is not False else 0
+            hcap = history_cap(n, Br, dtype, block) if history is not False else 0
             if hcap:
                 # FP32: keep the residuals, form X_j only at the shortlisted endpoints and the winner
                 try:
