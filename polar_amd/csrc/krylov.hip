@@ -2755,16 +2755,20 @@ __global__ __launch_bounds__(256) void k_hist_rows(const T* __restrict__ ring, i
 
 // X[(i*p + j)*nA + a] = X_j[active[a], cols[i]]: one thread per (active row, selected column), the C
 // columns of a row on neighbouring lanes (one residual row serves them all); the weights are staged
-// in LDS kHistStage steps at a time; PM >= p accumulators per thread.
+// in LDS kHistStage steps at a time; PM >= p accumulators per thread.  Each column's weight block
+// is padded by kHistPad doubles, so that the C lanes of a row, which read the same step and pole of
+// different columns at once, hit different LDS banks.
 constexpr int kHistStage = 32;
 constexpr int kHistMaxC = 4;
+constexpr int kHistPad = 2;
 template <typename T, int PM>
 __global__ __launch_bounds__(256) void k_hist_cols(const T* __restrict__ ring, int64_t n, int ld, int p, int64_t tcap,
                                                    const int* __restrict__ hist_T, const double* __restrict__ coef,
                                                    const int* __restrict__ cols, int C,
                                                    const int32_t* __restrict__ active, int64_t nA,
                                                    T* __restrict__ X) {
-  __shared__ double sw[kHistMaxC][PM][kHistStage];
+  __shared__ double sw_[kHistMaxC][PM * kHistStage + kHistPad];
+  auto sw = [&](int ci, int j, int tt) -> double& { return sw_[ci][j * kHistStage + tt]; };
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int i = (int)(q % C);
   const int64_t a = q / C;
@@ -2779,7 +2783,7 @@ __global__ __launch_bounds__(256) void k_hist_cols(const T* __restrict__ ring, i
     __syncthreads();
     for (int e = threadIdx.x; e < C * p * kHistStage; e += blockDim.x) {
       const int ci = e / (p * kHistStage), j = (e / kHistStage) % p, tt = e % kHistStage;
-      sw[ci][j][tt] = t0 + tt <= Tn ? coef[((int64_t)cols[ci] * p + j) * (tcap + 1) + t0 + tt] : 0.0;
+      sw(ci, j, tt) = t0 + tt <= Tn ? coef[((int64_t)cols[ci] * p + j) * (tcap + 1) + t0 + tt] : 0.0;
     }
     __syncthreads();
     if (ok) {
@@ -2794,7 +2798,7 @@ __global__ __launch_bounds__(256) void k_hist_cols(const T* __restrict__ ring, i
           if (u0 + u < nt) {
 #pragma unroll
             for (int j = 0; j < PM; ++j)
-              if (j < p) acc[j] = acc[j] + sw[i][j][u0 + u] * v[u];
+              if (j < p) acc[j] = acc[j] + sw(i, j, u0 + u) * v[u];
           }
       }
     }
