@@ -1094,13 +1094,13 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
 // chunk costs one memory round trip and one barrier instead of one per pole.
 constexpr int kMaxPolesFlush = 1024;
 constexpr int kFlushLanes = 16;    // lanes per row of a flush tile (column slices of 128 bytes)
-constexpr int kFlushWaves = 8;     // 512 threads: a tile of 32 rows shares each staged chunk
+constexpr int kFlushWaves = 16;    // 1024 threads: a tile of 64 rows shares each staged chunk
 constexpr int kFlushBlock = 64 * kFlushWaves;
 constexpr int kFlushChunk = 2;     // poles per staged chunk (2 and 4 measured alike; 2 spills nothing)
 
 // Columns per lane of the flush: half the solve's 16-byte vectors (8 bytes per lane, 16 lanes per
 // 128-byte row piece), so that a lane's window residuals and a chunk's P and X rows fit 4 waves per
-// SIMD, two 512-thread blocks per CU (4 FP32 columns per lane needed 182 registers, 2 waves).
+// SIMD, one 1024-thread block per CU (4 FP32 columns per lane needed 182 registers, 2 waves).
 inline int flush_cpl(int cpl) { return cpl >= 2 ? cpl / 2 : 1; }
 
 template <typename T> struct Ring {
