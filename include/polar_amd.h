@@ -212,14 +212,16 @@ int64_t polar_msolve_history_ws_bytes(const polar_op* op, int64_t R, int32_t p, 
 
 /* After a POLAR_HISTORY solve with this workspace (same op, R, p, nA, flags, tcap): the shifted
  * solutions on K rows of every column, E[j][k][c] = X_j[rows[k*R + c], c] (rows: device int32
- * absolute row ids; E: device, the op's dtype, p*K*R values).  For the re-rank this is
+ * absolute row ids in [0, n), read on the device; E: device, the op's dtype, p*K*R values).
+ * Asynchronous; reads the history, so call it before the workspace is reused.  For the re-rank this is
  * U[j, a_c, c] and U[j, b_c, c] of realworld/polar_gpu.py:230 (K = 2, rows = the endpoints). */
 int polar_history_rows(const polar_op* op, int64_t R, int32_t p, int64_t nA, int32_t flags, int64_t tcap,
                        void* ws, int64_t ws_bytes, int32_t K, const int32_t* rows, void* E, void* stream);
 
 /* After a POLAR_HISTORY solve: X_j on every active row for the C columns cols[] (device int32),
  * X[(i*p + j)*nA + a] = X_j[active[a], cols[i]] (device, the op's dtype): the winner's u of
- * realworld/polar_gpu.py:237.  At most 4 columns per call. */
+ * realworld/polar_gpu.py:237.  At most 4 columns per call and 64 poles; cols in [0, R) and active
+ * the solve's active rows, both read on the device.  Asynchronous. */
 int polar_history_cols(const polar_op* op, int64_t R, int32_t p, int64_t nA, int32_t flags, int64_t tcap,
                        void* ws, int64_t ws_bytes, int32_t C, const int32_t* cols, const int32_t* active,
                        void* X, void* stream);
