@@ -158,3 +158,28 @@ def test_polar_columns_history_rerank_equals_flushed_rerank(monkeypatch, precisi
     for a, b, d in zip(hist, flushed, capped):
         _same(a, b, bound)
         _same(d, b, bound)
+
+
+@pytest.mark.parametrize('freeze', [True, False])
+def test_history_with_zero_columns_and_graph_flag(freeze):
+    """Zero right-hand sides (converged before the first iteration), the hipGraph flag (ignored by
+    history solves) and freezing on or off: the same counts and norms as the flushed solve, and
+    zero shifted solutions for the zero columns."""
+    G = _graph(2000, 7)
+    W = _rhs(G, 8, 9)
+    W[:, [1, 6]] = 0.0
+    W = torch.from_numpy(W).float().cuda()
+    tau = np.array([0.0, 0.1, 1.0, 10.0])
+    act = torch.arange(G.n, dtype=torch.int32, device='cuda')
+    op = pg.Op(G, precision='fp32')
+    kw = dict(track=True, block=True, replace=True, freeze=freeze)
+    X, h, it = pg.msolve(op, W, tau, act, 1e-8, **kw)
+    _, h2, it2, H = pg.msolve(op, W, tau, act, 1e-8, graph=True, history=512, **kw)
+    assert np.array_equal(it, it2) and it[1] == 0 and it[6] == 0 and it.max() > 20
+    assert torch.equal(h, h2)
+    Xh = H.cols(torch.arange(8, dtype=torch.int32, device='cuda'), act).double().permute(1, 2, 0)
+    Xd = X.double()
+    assert float(Xh[:, :, [1, 6]].abs().max()) == 0.0
+    keep = [0, 2, 3, 4, 5, 7]
+    scale = Xd[:, :, keep].abs().amax(dim=1, keepdim=True)
+    assert float(((Xh[:, :, keep] - Xd[:, :, keep]).abs() / scale).max()) < 1e-5
