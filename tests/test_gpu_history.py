@@ -62,12 +62,12 @@ def test_history_solve_matches_flushed_solve(precision, replace):
     Xh = H.cols(torch.arange(R, dtype=torch.int32, device='cuda'), act).double()   # (R, p, nA)
     err = (Xh.permute(1, 2, 0) - Xd).abs() / scale
     assert float(err.max()) < bound
-    # two chosen rows of every column (absolute row ids among the active rows)
+    # three chosen rows of every column (absolute row ids among the active rows)
     rng = np.random.default_rng(2)
-    slots = rng.integers(0, len(act_h), size=(2, R))
+    slots = rng.integers(0, len(act_h), size=(3, R))
     rows = torch.from_numpy(act_h[slots].astype(np.int32)).cuda()
-    E = H.rows(rows).double()                                     # (p, 2, R)
-    want = torch.stack([Xd[:, torch.from_numpy(slots[k]).cuda(), torch.arange(R, device='cuda')] for k in range(2)], 1)
+    E = H.rows(rows).double()                                     # (p, 3, R)
+    want = torch.stack([Xd[:, torch.from_numpy(slots[k]).cuda(), torch.arange(R, device='cuda')] for k in range(3)], 1)
     assert float(((E - want).abs() / scale.squeeze(1)[:, None, :]).max()) < bound
 
 
