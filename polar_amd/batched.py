@@ -603,6 +603,7 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
     if progress is not None:
         progress('bank', dict(seconds=time.perf_counter() - t0, spmv=int(spmv.sum())))
 
+    hist_ok = True                     # cleared once a history solve outruns its history
     for rnd in range(k):
         # every group's proxy and top-B on the device, then one read-back of all shortlists
         Brs = [min(B, int(live[g].sum())) for g in range(Gn)]
@@ -632,7 +633,7 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
         Wb.index_put_((ib, ic), torch.full((Rt,), -1.0, dtype=dtype, device=dev))
         cg = np.concatenate([np.full(len(sh), g, np.int32) for g, sh in enumerate(shorts)])
         H = None
-        hcap = pg.history_cap(n, Rt, dtype, block) if history is not False else 0
+        hcap = pg.history_cap(n, Rt, dtype, block) if history is not False and hist_ok else 0
         if hcap:
             # FP32: keep the residuals, form X_j only at the shortlisted endpoints and the winners
             try:
@@ -641,6 +642,7 @@ def polar_columns(G, cand, k, s_list, q=64, B=8, eps=1e-2, tol=1e-7, obj='frac',
             except RuntimeError as ex:
                 if 'iteration cap' not in str(ex):
                     raise
+                hist_ok = False                # the later re-rank solves need about as many
         if H is None:
             Ub, hh, it = msolve_batched(op, segs, Wb, tau_all, active, tol, track=obj != 'logdet', graph=graph,
                                         block=block, replace=replace, col_group=cg)[:3]

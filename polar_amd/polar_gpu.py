@@ -310,19 +310,31 @@ def _col_bytes(n, p, nA, dtype, block=None, replace=False):
     return esz * (nvec * n + p * nA) + ((16 + esz) * n if replace else 0) + 64
 
 
-HISTORY_BYTES = 16 << 30      # residual history budget of one re-rank solve
+HISTORY_BYTES = 64 << 30      # residual history budget of one re-rank solve ...
+HISTORY_SHARE = 0.25          # ... and at most this share of the device memory still available
 HISTORY_STEPS = 1024          # iterations a history solve keeps at most
 
 
-def history_cap(n, R, dtype, block=None):
+def _available_bytes():
+    """Device memory free for new tensors: the driver's free memory plus what torch's caching
+    allocator holds unused."""
+    free, _ = torch.cuda.mem_get_info()
+    return free + torch.cuda.memory_reserved() - torch.cuda.memory_allocated()
+
+
+def history_cap(n, R, dtype, block=None, avail=None):
     """Iterations a re-rank solve of R columns on n rows keeps in history mode: with the blocked
     update (the FP32 default, FP64 with block=True) when at least 256 iterations fit in
-    HISTORY_BYTES, else 0 (the solve forms its shifted blocks as usual).  The pinned FP64 path never
-    uses it: its per-iteration update is the oracle's arithmetic."""
+    min(HISTORY_BYTES, HISTORY_SHARE x available device memory), else 0 (the solve forms its
+    shifted blocks as usual).  The pinned FP64 path never uses it: its per-iteration update is the
+    oracle's arithmetic.  avail: available bytes (None: asked from the device)."""
     if not _blocked(block, dtype):
         return 0
     esz = 8 if dtype == torch.float64 else 4
-    cap = min(HISTORY_STEPS, HISTORY_BYTES // max(1, n * R * esz))
+    if avail is None:
+        avail = _available_bytes()
+    budget = min(HISTORY_BYTES, int(HISTORY_SHARE * avail))
+    cap = min(HISTORY_STEPS, budget // max(1, n * R * esz))
     return int(cap) if cap >= 256 else 0
 
 
@@ -785,6 +797,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
         return X, Y, h, t, tr
 
     Ubuf = None
+    hist_ok = True                      # cleared once a history solve outruns its history
     X, Y, h, t, tr = bank()
     if not naive and not joint:
         Zbox[0] = None                      # Z is not needed again
@@ -820,7 +833,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
             need_track = obj != 'logdet'
             ph.mark('select')
             H = None
-            hcap = history_cap(n, Br, dtype, block) if history is not False else 0
+            hcap = history_cap(n, Br, dtype, block) if history is not False and hist_ok else 0
             if hcap:
                 # FP32: keep the residuals, form X_j only at the shortlisted endpoints and the winner
                 try:
@@ -829,6 +842,7 @@ def polar(G, cand, k, s, q=128, B=16, eps=1e-2, tol=1e-7, obj='frac', sketch=Non
                 except RuntimeError as ex:
                     if 'iteration cap' not in str(ex):
                         raise
+                    hist_ok = False             # the later re-rank solves need about as many
             if H is None:
                 if Ubuf is None:                # one re-rank block for all rounds
                     Ubuf = torch.empty((p, nA, max(B, 1)), dtype=dtype, device=dev)

@@ -12,7 +12,8 @@ winner's column.
   with marginals and trace estimates within 1e-6 (FP32; 1e-11 in FP64 with the blocked update),
   and a history too short for the solve falls back to the flushed solve with the same results.
 * The mode is used with the blocked update only (never on the pinned FP64 path) and where at least
-  256 iterations fit in the history budget.
+  256 iterations fit in the history budget (a share of the available device memory); once a
+  history solve of a greedy run outruns its history, the run's later re-rank solves are flushed.
 """
 import numpy as np
 import pytest
@@ -119,11 +120,18 @@ def _same(a, b, bound=1e-6):
 
 def test_history_cap_rule():
     f32, f64 = torch.float32, torch.float64
-    assert pg.history_cap(100_000, 24, f32) == pg.HISTORY_STEPS
-    assert pg.history_cap(100_000, 24, f64) == 0                      # the pinned FP64 path
-    assert pg.history_cap(100_000, 24, f64, block=True) == pg.HISTORY_BYTES // (100_000 * 24 * 8)
-    assert pg.history_cap(100_000, 24, f32, block=False) == 0
-    assert pg.history_cap(4_000_000, 16, f32) == 0                    # fewer than 256 steps fit
+    big = 250e9
+    assert pg.history_cap(100_000, 24, f32) == pg.HISTORY_STEPS       # the device's own free memory
+    assert pg.history_cap(100_000, 24, f32, avail=big) == pg.HISTORY_STEPS
+    assert pg.history_cap(100_000, 24, f64, avail=big) == 0           # the pinned FP64 path
+    assert pg.history_cap(100_000, 24, f64, block=True, avail=big) == pg.HISTORY_STEPS
+    assert pg.history_cap(100_000, 24, f32, block=False, avail=big) == 0
+    # c3's re-rank (n=1M, 16 columns): the budget, not the step limit, sets the cap
+    budget = min(pg.HISTORY_BYTES, int(pg.HISTORY_SHARE * big))
+    assert pg.history_cap(1_000_000, 16, f32, avail=big) == budget // (1_000_000 * 16 * 4) > 900
+    # c5's re-rank (n=4M, 16 columns) beside its caches: fewer than 256 steps fit
+    assert pg.history_cap(4_000_000, 16, f32, avail=60e9) == 0
+    assert pg.history_cap(1_000_000, 16, f32, avail=8e9) == 0
 
 
 def test_polar_history_rerank_equals_flushed_rerank(monkeypatch):
