@@ -2650,14 +2650,17 @@ constexpr int kHistBatch = 16;
 // there and writes the segment's weights.
 __global__ __launch_bounds__(256) void k_hist_coef(const uint8_t* __restrict__ hm, const double* __restrict__ haj,
                                                    const double* __restrict__ hbj, const double* __restrict__ hzn,
-                                                   const int* __restrict__ hist_T, int p, int ld, int64_t tcap,
-                                                   const int* __restrict__ cols, int C, double* __restrict__ coef) {
+                                                   const int* __restrict__ hist_T, const int64_t* __restrict__ iters,
+                                                   int p, int ld, int64_t tcap, const int* __restrict__ cols, int C,
+                                                   double* __restrict__ coef) {
   const int lane = threadIdx.x & 63;
   const int pair = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (pair >= C * p) return;               // the whole wave
   const int i = pair / p, j = pair - i * p;
   const int c = cols ? cols[i] : i;
-  const int Tn = hist_T[0];
+  // a column is active in its first iters[c] steps only (it never becomes active again), so its
+  // weights w[t] vanish for t > iters[c]: only w[0 .. iters[c]] are formed and read
+  const int Tn = (int)min((int64_t)hist_T[0], iters[c]);
   double* w = coef + ((int64_t)c * p + j) * (tcap + 1);
   const int seg = (Tn + 63) / 64;
   const int t0 = min(Tn, lane * seg), t1 = min(Tn, t0 + seg);
@@ -2722,15 +2725,16 @@ __global__ __launch_bounds__(256) void k_hist_coef(const uint8_t* __restrict__ h
 // t = l (mod 64), the lanes' FP64 sums added across the wave.
 template <typename T>
 __global__ __launch_bounds__(256) void k_hist_rows(const T* __restrict__ ring, int64_t n, int ld, int p, int64_t tcap,
-                                                   const int* __restrict__ hist_T, const double* __restrict__ coef,
-                                                   int R, int K, const int32_t* __restrict__ rows, T* __restrict__ E) {
+                                                   const int* __restrict__ hist_T, const int64_t* __restrict__ iters,
+                                                   const double* __restrict__ coef, int R, int K,
+                                                   const int32_t* __restrict__ rows, T* __restrict__ E) {
   constexpr int kU = 4;
   const int lane = threadIdx.x & 63;
   const int idx = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (idx >= p * K * R) return;            // the whole wave
   const int c = idx % R, k = (idx / R) % K, j = idx / (R * K);
   const int64_t row = rows[k * R + c];
-  const int Tn = hist_T[0];
+  const int Tn = (int)min((int64_t)hist_T[0], iters[c]);     // the column's last nonzero weight
   const double* w = coef + ((int64_t)c * p + j) * (tcap + 1);
   double acc = 0.0;
   for (int tb = lane; tb <= Tn; tb += 64 * kU) {
@@ -2763,9 +2767,9 @@ constexpr int kHistMaxC = 4;
 constexpr int kHistPad = 2;
 template <typename T, int PM>
 __global__ __launch_bounds__(256) void k_hist_cols(const T* __restrict__ ring, int64_t n, int ld, int p, int64_t tcap,
-                                                   const int* __restrict__ hist_T, const double* __restrict__ coef,
-                                                   const int* __restrict__ cols, int C,
-                                                   const int32_t* __restrict__ active, int64_t nA,
+                                                   const int* __restrict__ hist_T, const int64_t* __restrict__ iters,
+                                                   const double* __restrict__ coef, const int* __restrict__ cols,
+                                                   int C, const int32_t* __restrict__ active, int64_t nA,
                                                    T* __restrict__ X) {
   __shared__ double sw_[kHistMaxC][PM * kHistStage + kHistPad];
   auto sw = [&](int ci, int j, int tt) -> double& { return sw_[ci][j * kHistStage + tt]; };
@@ -2773,21 +2777,30 @@ __global__ __launch_bounds__(256) void k_hist_cols(const T* __restrict__ ring, i
   const int i = (int)(q % C);
   const int64_t a = q / C;
   const int c = cols[i];
-  const int Tn = hist_T[0];
+  // each column's weights end at its iteration count (k_hist_coef); the block walks to the largest
+  int Tc[kHistMaxC];
+  int Te = 0;
+#pragma unroll
+  for (int ci = 0; ci < kHistMaxC; ++ci) {
+    Tc[ci] = ci < C ? (int)min((int64_t)hist_T[0], iters[cols[ci]]) : 0;
+    Te = max(Te, Tc[ci]);
+  }
+  const int Ti = Tc[0] * (i == 0) + Tc[1] * (i == 1) + Tc[2] * (i == 2) + Tc[3] * (i == 3);
   double acc[PM];
 #pragma unroll
   for (int j = 0; j < PM; ++j) acc[j] = 0.0;
   const bool ok = a < nA;
   const int64_t row = ok ? active[a] : 0;
-  for (int t0 = 0; t0 <= Tn; t0 += kHistStage) {
+  for (int t0 = 0; t0 <= Te; t0 += kHistStage) {
     __syncthreads();
     for (int e = threadIdx.x; e < C * p * kHistStage; e += blockDim.x) {
       const int ci = e / (p * kHistStage), j = (e / kHistStage) % p, tt = e % kHistStage;
-      sw(ci, j, tt) = t0 + tt <= Tn ? coef[((int64_t)cols[ci] * p + j) * (tcap + 1) + t0 + tt] : 0.0;
+      const int tcol = (int)min((int64_t)hist_T[0], iters[cols[ci]]);
+      sw(ci, j, tt) = t0 + tt <= tcol ? coef[((int64_t)cols[ci] * p + j) * (tcap + 1) + t0 + tt] : 0.0;
     }
     __syncthreads();
     if (ok) {
-      const int nt = min(kHistStage, Tn + 1 - t0);
+      const int nt = min(kHistStage, Ti + 1 - t0);
       for (int u0 = 0; u0 < nt; u0 += 8) {
         double v[8];
 #pragma unroll
@@ -2830,10 +2843,10 @@ static int history_rows_impl(const polar_op* op, int64_t R, int p, int64_t nA, i
   const CGState& s = m.s;
   // one wave per (column, pole) pair, then one wave per value: 4 waves per block
   hipLaunchKernelGGL(k_hist_coef, dim3((int)((R * p + 3) / 4)), dim3(256), 0, st, s.hm, s.haj, s.hbj, s.hzn,
-                     s.hist_T, p, (int)R, tcap, (const int*)nullptr, (int)R, m.coef);
+                     s.hist_T, (const int64_t*)s.iters, p, (int)R, tcap, (const int*)nullptr, (int)R, m.coef);
   POLAR_LAUNCH_CHECK();
   hipLaunchKernelGGL((k_hist_rows<T>), dim3((int)((tot + 3) / 4)), dim3(256), 0, st, (const T*)m.ring, op->n,
-                     (int)R, p, tcap, s.hist_T, (const double*)m.coef, (int)R, K, rows, E);
+                     (int)R, p, tcap, s.hist_T, (const int64_t*)s.iters, (const double*)m.coef, (int)R, K, rows, E);
   POLAR_LAUNCH_CHECK();
   return 0;
 }
@@ -2850,15 +2863,15 @@ static int history_cols_impl(const polar_op* op, int64_t R, int p, int64_t nA, i
   if (blocks > INT32_MAX) return set_error(POLAR_E_ARG, "too many rows x columns");
   const CGState& s = m.s;
   hipLaunchKernelGGL(k_hist_coef, dim3((C * p + 3) / 4), dim3(256), 0, st, s.hm, s.haj, s.hbj, s.hzn, s.hist_T,
-                     p, (int)R, tcap, cols, C, m.coef);
+                     (const int64_t*)s.iters, p, (int)R, tcap, cols, C, m.coef);
   POLAR_LAUNCH_CHECK();
   const dim3 grid((unsigned)blocks);
   if (p <= 32)
     hipLaunchKernelGGL((k_hist_cols<T, 32>), grid, dim3(256), 0, st, (const T*)m.ring, op->n, (int)R, p, tcap,
-                       s.hist_T, (const double*)m.coef, cols, C, active, nA, X);
+                       s.hist_T, (const int64_t*)s.iters, (const double*)m.coef, cols, C, active, nA, X);
   else
     hipLaunchKernelGGL((k_hist_cols<T, 64>), grid, dim3(256), 0, st, (const T*)m.ring, op->n, (int)R, p, tcap,
-                       s.hist_T, (const double*)m.coef, cols, C, active, nA, X);
+                       s.hist_T, (const int64_t*)s.iters, (const double*)m.coef, cols, C, active, nA, X);
   POLAR_LAUNCH_CHECK();
   return 0;
 }
