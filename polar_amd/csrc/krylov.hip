@@ -2786,6 +2786,7 @@ __global__ __launch_bounds__(256) void k_hist_cols(const T* __restrict__ ring, i
     Te = max(Te, Tc[ci]);
   }
   // this lane's column (selected without a dynamic index, so that Tc stays in registers)
+  static_assert(kHistMaxC == 4, "the selection below covers 4 columns");
   const int Ti = Tc[0] * (i == 0) + Tc[1] * (i == 1) + Tc[2] * (i == 2) + Tc[3] * (i == 3);
   double acc[PM];
 #pragma unroll
