@@ -2807,7 +2807,8 @@ __global__ __launch_bounds__(256) void k_hist_cols(const T* __restrict__ ring, i
         double v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-          if (u0 + u < nt) v[u] = (double)ring[(int64_t)(t0 + u0 + u) * n * ld + row * ld + c];
+          if (u0 + u < nt)       // the history is read once: non-temporal loads
+            v[u] = (double)__builtin_nontemporal_load(ring + (int64_t)(t0 + u0 + u) * n * ld + row * ld + c);
 #pragma unroll
         for (int u = 0; u < 8; ++u)
           if (u0 + u < nt) {
