@@ -125,9 +125,10 @@ def test_history_cap_rule():
     f32, f64 = torch.float32, torch.float64
     big = 250e9
     assert pg.history_cap(100_000, 24, f32) == pg.HISTORY_STEPS       # the device's own free memory
-    # c3's re-rank (n=1M, 16 columns) needs about 420 iterations: the free memory of an idle
-    # 288 GB device keeps more than twice that
-    assert pg.history_cap(1_000_000, 16, f32) > 840
+    # c3's re-rank (n=1M, 16 columns) needs about 420 iterations: on an idle device (90% of its
+    # memory available, whatever earlier tests hold now) the budget keeps more than twice that
+    idle = 0.9 * torch.cuda.mem_get_info()[1]
+    assert pg.history_cap(1_000_000, 16, f32, avail=idle) > 840
     assert pg.history_cap(100_000, 24, f32, avail=big) == pg.HISTORY_STEPS
     assert pg.history_cap(100_000, 24, f64, avail=big) == 0           # the pinned FP64 path
     assert pg.history_cap(100_000, 24, f64, block=True, avail=big) == pg.HISTORY_STEPS
