@@ -213,11 +213,14 @@ class Op:
                 ptr = np.zeros(len(rows) + 1, dtype=np.int32)
                 ptr[1:] = np.cumsum([len(nbr[r]) for r in rows])
                 cols = np.array([c for r in rows for c, _ in nbr[r]], dtype=np.int32)
-                arrs = [torch.from_numpy(rows).to(dev), torch.from_numpy(ptr).to(dev), torch.from_numpy(cols).to(dev)]
+                parts = [rows, ptr, cols]
                 if any(g is not None for g in self.grp):
-                    grp = np.array([-1 if g is None else g for r in rows for _, g in nbr[r]], dtype=np.int32)
-                    arrs.append(torch.from_numpy(grp).to(dev))
-                self._patch = tuple(arrs)
+                    parts.append(np.array([-1 if g is None else g for r in rows for _, g in nbr[r]], dtype=np.int32))
+                # one upload for all the patch arrays (views of one device buffer): each pageable
+                # copy waits for the stream, and the greedy rebuilds the patch every round
+                buf = torch.from_numpy(np.concatenate(parts)).to(dev)
+                offs = np.cumsum([0] + [len(x) for x in parts])
+                self._patch = tuple(buf[offs[i]:offs[i + 1]] for i in range(len(parts)))
             else:
                 self._patch = ()
         return self._patch
