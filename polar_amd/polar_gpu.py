@@ -949,9 +949,16 @@ def random_set(cand, k, seed):
 
 
 def true_marginals(G, cand, seq, s, obj='frac', eps=1e-9, tol=1e-10, kmax=None, poles=None,
-                   precision='fp64'):
+                   precision='fp64', together=64):
     """Certified marginals along a sequence (polar_gpu.py:273-291): single-RHS multi-shift CG on
-    the GPU with active=[a, b] and g = sum_j om_j ||u_j||^2 / (1 + b^T u_j)."""
+    the GPU with active=[a, b] and g = sum_j om_j ||u_j||^2 / (1 + b^T u_j).
+
+    The reference solves position t after adding the edges seq[:t], one position at a time.  Here
+    `together` positions at a time run as column groups of one operator: position t's column sees
+    the added edges seq[:t] (edges before the chunk untagged, later ones as patch entries tagged
+    with the groups of the positions after them, in addition order), so each column does the
+    arithmetic of its own sequential solve, in one solve instead of `together`.  together=1 is the
+    sequential loop."""
     L.require_gpu()
     dev = _device()
     kmax = kmax or len(seq)
@@ -960,21 +967,51 @@ def true_marginals(G, cand, seq, s, obj='frac', eps=1e-9, tol=1e-10, kmax=None, 
     else:
         tau, om = np.asarray(poles[0], float), np.asarray(poles[1], float)
         qerr = float('nan')
-    op = Op(G, precision=precision)
+    track = obj != 'logdet'
     cnt = Counter()
     out = []
-    for e in seq:
-        a, b = int(cand[e, 0]), int(cand[e, 1])
-        w = torch.zeros((G.n, 1), dtype=op.dtype, device=dev)
-        w[a, 0] = 1.0
-        w[b, 0] = -1.0
-        U, hh, _ = msolve(op, w, tau, np.asarray([a, b]), tol, track=obj != 'logdet', counter=cnt)[:3]
-        bu = (U[:, 0, 0].double() - U[:, 1, 0].double()).cpu().numpy()
-        if obj == 'logdet':
-            g = float(np.log1p(bu[0]))
+    ab = np.array([[int(cand[e, 0]), int(cand[e, 1])] for e in seq], dtype=np.int64).reshape(-1, 2)
+    step = max(1, int(together))
+    for c0 in range(0, len(ab), step):
+        c1 = min(len(ab), c0 + step)
+        m = c1 - c0
+        op = Op(G, precision=precision)
+        for i in range(c0):                                   # edges every position of the chunk sees
+            op.add(ab[i, 0], ab[i, 1])
+        if m == 1:
+            a, b = int(ab[c0, 0]), int(ab[c0, 1])
+            w = torch.zeros((G.n, 1), dtype=op.dtype, device=dev)
+            w[a, 0] = 1.0
+            w[b, 0] = -1.0
+            U, hh, _ = msolve(op, w, tau, np.asarray([a, b]), tol, track=track, counter=cnt)[:3]
+            bu = (U[:, 0, 0].double() - U[:, 1, 0].double()).cpu().numpy()[:, None]
+            hh = hh.cpu().numpy() if track else None
         else:
-            g = float(np.sum(om * hh[:, 0].cpu().numpy() / (1 + bu)))
-        out.append(g)
-        op.add(a, b)
+            from . import batched
+            for i in range(c0, c1 - 1):                       # edge i: the positions after it
+                for t in range(i + 1, c1):
+                    op.add(ab[i, 0], ab[i, 1], group=t - c0)
+            act_h = np.unique(ab[c0:c1].ravel())
+            slot = np.searchsorted(act_h, ab[c0:c1])
+            cols = torch.arange(m, device=dev)
+            W = torch.zeros((G.n, m), dtype=op.dtype, device=dev)
+            W[torch.from_numpy(ab[c0:c1, 0]).to(dev), cols] = 1.0
+            W[torch.from_numpy(ab[c0:c1, 1]).to(dev), cols] = -1.0
+            act = torch.from_numpy(act_h.astype(np.int32)).to(dev)
+            X, hb, it = batched.msolve_batched(op, batched.single_segment(G, len(act_h)), W, np.tile(tau, (m, 1)),
+                                               act, tol, track=track, col_group=np.arange(m, dtype=np.int32))[:3]
+            sa = torch.from_numpy(slot[:, 0]).to(dev)
+            sb = torch.from_numpy(slot[:, 1]).to(dev)
+            bu = (X[:, sa, cols].double() - X[:, sb, cols].double()).cpu().numpy()      # (p, m)
+            hh = hb[:, 0, :].cpu().numpy() if track else None
+            it_sum = int(np.asarray(it).sum())
+            cnt.spmv += it_sum
+            cnt.rhs += m
+            cnt.solves.append(it_sum)
+        for t in range(m):
+            if obj == 'logdet':
+                out.append(float(np.log1p(bu[0, t])))
+            else:
+                out.append(float(np.sum(om * hh[:, t] / (1 + bu[:, t]))))
     rho = tol * float(np.max((G.Lam(kmax) + tau) / (G.sigma + tau)))
     return np.array(out), dict(quadrature_bound=qerr, residual_rho=rho, p=len(tau), spmv=cnt.spmv)

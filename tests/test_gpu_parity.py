@@ -332,6 +332,22 @@ def test_true_marginals_vs_eigenvalues():
     assert cert['p'] == cert_o['p'] and abs(cert['residual_rho'] - cert_o['residual_rho']) < 1e-20
 
 
+@pytest.mark.parametrize('obj', ['frac', 'logdet'])
+def test_true_marginals_positions_together(obj):
+    """The positions of the sequence as column groups of one solve (each sees the edges before it)
+    equal the sequential loop: same values and spmv, in chunks that do not divide the length."""
+    n = 700
+    e = graphs.barabasi_albert(n, 3, 6)
+    G = pg.Graph(n, e, 0.4)
+    cand = graphs.candidates(n, e, 300, 5)
+    seq = pg.random_set(cand, 11, 2)
+    ref, c_ref = pg.true_marginals(G, cand, seq, 0.5, obj=obj, together=1)
+    for together in (4, 64):
+        g, c = pg.true_marginals(G, cand, seq, 0.5, obj=obj, together=together)
+        assert c['spmv'] == c_ref['spmv'] and c['p'] == c_ref['p']
+        assert _rel(g, ref) < 1e-13, together
+
+
 @pytest.mark.parametrize('precision', ['fp64', 'fp32'])
 def test_msolve_column_compaction(precision):
     """Half the columns are zero (inactive from the start) and the rest mix dense random columns
