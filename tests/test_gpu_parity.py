@@ -332,20 +332,22 @@ def test_true_marginals_vs_eigenvalues():
     assert cert['p'] == cert_o['p'] and abs(cert['residual_rho'] - cert_o['residual_rho']) < 1e-20
 
 
-@pytest.mark.parametrize('obj', ['frac', 'logdet'])
-def test_true_marginals_positions_together(obj):
+@pytest.mark.parametrize('obj,precision', [('frac', 'fp64'), ('logdet', 'fp64'), ('frac', 'fp32')])
+def test_true_marginals_positions_together(obj, precision):
     """The positions of the sequence as column groups of one solve (each sees the edges before it)
-    equal the sequential loop: same values and spmv, in chunks that do not divide the length."""
+    equal the sequential loop: same values and spmv, in chunks that do not divide the length (FP32
+    with the blocked update: the same CG scalars, values within its rounding)."""
     n = 700
     e = graphs.barabasi_albert(n, 3, 6)
     G = pg.Graph(n, e, 0.4)
     cand = graphs.candidates(n, e, 300, 5)
     seq = pg.random_set(cand, 11, 2)
-    ref, c_ref = pg.true_marginals(G, cand, seq, 0.5, obj=obj, together=1)
+    kw = dict(obj=obj, precision=precision, tol=1e-10 if precision == 'fp64' else 1e-6)
+    ref, c_ref = pg.true_marginals(G, cand, seq, 0.5, together=1, **kw)
     for together in (4, 64):
-        g, c = pg.true_marginals(G, cand, seq, 0.5, obj=obj, together=together)
+        g, c = pg.true_marginals(G, cand, seq, 0.5, together=together, **kw)
         assert c['spmv'] == c_ref['spmv'] and c['p'] == c_ref['p']
-        assert _rel(g, ref) < 1e-13, together
+        assert _rel(g, ref) < (1e-13 if precision == 'fp64' else 1e-5), together
 
 
 @pytest.mark.parametrize('precision', ['fp64', 'fp32'])
