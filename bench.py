@@ -211,6 +211,30 @@ def flush_bench(pg, torch, graphs):
                 seed_bytes_with_replace=n * R * (4 + 4 + 4 + 8 + 8))
 
 
+def true_marginals_bench(pg, torch, configs, k=20, reps=2):
+    """The certified evaluator (polar_gpu.true_marginals, eps=1e-9, tol=1e-10, FP64) on the c2
+    graph along a random k-edge sequence of its pool: the positions as column groups of one solve
+    against the sequential loop (together=1); best of `reps` runs each."""
+    c = configs.make('c2', 1.0)
+    G = pg.Graph(c['n'], c['edges'], c['sigma'])
+    seq = pg.random_set(c['cand'], k, 1)
+    out = {}
+    vals = {}
+    for together in (1, 64):
+        best = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            g, cert = pg.true_marginals(G, c['cand'], seq, 0.5, together=together)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        vals[together] = g
+        out[f'together_{together}'] = dict(seconds=best, spmv=cert['spmv'], p=cert['p'])
+    out.update(k=k, n=G.n, max_rel_diff=float(np.max(np.abs(vals[64] - vals[1]) / np.abs(vals[1]))))
+    return out
+
+
 def _rel_max(a, b):
     a, b = np.asarray(a, float), np.asarray(b, float)
     n = min(len(a), len(b))
@@ -402,6 +426,9 @@ def main():
                     help='only the n=4M SpMM leg and the 1 GiB copy probe (for counter runs)')
     ap.add_argument('--flush-only', action='store_true',
                     help='only the window-flush solve (every pole changed; for kernel traces)')
+    ap.add_argument('--tm-only', action='store_true',
+                    help='only the certified-marginals evaluator on the c2 graph (positions together '
+                         'against one at a time)')
     a = ap.parse_args()
     global GRAPH, REPLACE, SCHEDULE
     SCHEDULE = a.schedule
@@ -411,7 +438,7 @@ def main():
         print(f'bench.py measures one GPU only (asked for --gpus {a.gpus}); the pole-sharded multi-GPU '
               'layout is a design note, not built.', file=sys.stderr)
         sys.exit(2)
-    cpu_proc = None if (a.skip_cpu or a.spmm_only or a.flush_only) else start_cpu_child(a)
+    cpu_proc = None if (a.skip_cpu or a.spmm_only or a.flush_only or a.tm_only) else start_cpu_child(a)
 
     import torch
     from polar_amd import _lib as L
@@ -430,6 +457,9 @@ def main():
     if a.flush_only:
         from polar_amd import graphs
         print(json.dumps(dict(flush=flush_bench(pg, torch, graphs))), flush=True)
+        return
+    if a.tm_only:
+        print(json.dumps(dict(true_marginals=true_marginals_bench(pg, torch, configs))), flush=True)
         return
 
     t = time.time()
