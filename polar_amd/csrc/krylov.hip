@@ -963,7 +963,7 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
                                                    const T* __restrict__ r, T* __restrict__ d,
                                                    const int32_t* __restrict__ active,
                                                    T* __restrict__ P, T* __restrict__ Xo, int64_t sj,
-                                                   int64_t si, int nb_d, CGState s, int cur) {
+                                                   int64_t si, int nb_d, int ps, CGState s, int cur) {
   if (s.flag[cur] == 0) return;
   Geo g(L);
   const int c0 = g.li * CPL;
@@ -996,7 +996,17 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
     }
     return;
   }
-  const SegPart sp = seg_part(s.bS, s.S, blockIdx.x - nb_d, gridDim.x - nb_d);
+  // ps > 1 (one segment, few active rows): the shifted blocks are ps copies of the row partition,
+  // copy q taking the poles j with j % ps == q; each (pole, row, column) update is unchanged.
+  SegPart sp;
+  int part = 0;
+  if (ps > 1) {
+    const int nbr = (int)(gridDim.x - nb_d) / ps, bb = (int)blockIdx.x - nb_d;
+    part = bb / nbr;
+    sp = SegPart{0, bb - part * nbr, nbr};
+  } else {
+    sp = seg_part(s.bS, s.S, blockIdx.x - nb_d, gridDim.x - nb_d);
+  }
   if (!seg_live(s, cur, sp.g)) return;     // uniform over the block, before any barrier
   int64_t a0, a1;
   seg_range(s.seg_a, s.S, sp.g, nA, a0, a1);
@@ -1021,20 +1031,20 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, int64_t nA, int p,
       s_m[q] = s.m[src];
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < p; j += kBlock) {
+    for (int j = part + threadIdx.x * ps; j < p; j += kBlock * ps) {
       bool any = false;
       for (int c = 0; c < R; ++c) any |= s_m[j * R + c] != 0;
       if (any) s_jl[atomicAdd(&s_nj, 1)] = j;   // pole order is irrelevant: poles are independent
     }
     __syncthreads();
   }
-  const int nj = staged ? s_nj : p;
+  const int nj = staged ? s_nj : (p - part + ps - 1) / ps;
   for (int64_t i = a0 + (int64_t)sp.rank * gpb + g.wave * g.G + g.gi; i < a1; i += (int64_t)sp.nb * gpb) {
     if (c0 >= R || nj == 0) break;
     T rn[CPL];
     vload<T, CPL>(r + (int64_t)active[i] * s.ld + c0, rn);
     for (int jj = 0; jj < nj; ++jj) {
-      const int j = staged ? s_jl[jj] : jj;
+      const int j = staged ? s_jl[jj] : part + jj * ps;
       bool mj[CPL];
       bool anym = false;
       T aj[CPL], bj[CPL], zn[CPL];
@@ -1530,6 +1540,7 @@ struct Segs {
 
 constexpr int kBatchGrid = 4096;   // default block budget of a batched kernel with partial sums
 constexpr int kUpdGrid = 65536;    // block budget of the stream kernels without partial sums
+constexpr int kSplitGrid = 512;    // k_update's shifted part splits its poles below this many blocks
 
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
@@ -1938,12 +1949,14 @@ static int run_iter(const polar_op* op, const Plan& pl, double tau0, const T* ri
   if (ev && g_prof_on == 1) POLAR_HIP(hipEventRecord(ev[5], st));
   // blocked: the window flush updates X, P; history: no X, P at all
   const int nb_d = pl.nb_d, nb_s = (s.T > 1 || s.hm) ? 0 : pl.nb_s;
+  // few active rows on one segment: spread the poles over copies of the shifted blocks
+  const int ps = (s.S <= 1 && nb_s > 0) ? (int)std::min<int64_t>(std::max(p, 1), std::max(1, kSplitGrid / nb_s)) : 1;
   if (p * R <= 1024)
-    hipLaunchKernelGGL((k_update<T, CPL, true>), dim3(nb_d + nb_s), dim3(kBlock), 0, st, n, nA, p, R, pl.L,
-                       (const T*)rout, d, active, P, Xo, sj, si, nb_d, s, cur);
+    hipLaunchKernelGGL((k_update<T, CPL, true>), dim3(nb_d + nb_s * ps), dim3(kBlock), 0, st, n, nA, p, R, pl.L,
+                       (const T*)rout, d, active, P, Xo, sj, si, nb_d, ps, s, cur);
   else
-    hipLaunchKernelGGL((k_update<T, CPL, false>), dim3(nb_d + nb_s), dim3(kBlock), 0, st, n, nA, p, R, pl.L,
-                       (const T*)rout, d, active, P, Xo, sj, si, nb_d, s, cur);
+    hipLaunchKernelGGL((k_update<T, CPL, false>), dim3(nb_d + nb_s * ps), dim3(kBlock), 0, st, n, nA, p, R, pl.L,
+                       (const T*)rout, d, active, P, Xo, sj, si, nb_d, ps, s, cur);
   POLAR_LAUNCH_CHECK();
   if (ev && g_prof_on == 1) POLAR_HIP(hipEventRecord(ev[6], st));
   return 0;
