@@ -184,6 +184,29 @@ def test_msolve_matches_oracle(freeze, R):
     assert np.max(np.abs(teg.cpu().numpy() - teo)) <= 1e-13 * np.max(np.abs(teo))
 
 
+@pytest.mark.parametrize('R', [1, 20])
+def test_msolve_many_poles_few_rows(R):
+    """342 poles on 40 active rows: the shifted update spreads the poles over copies of its row
+    blocks (pole j to copy j mod ps), with the per-pole scalars staged in LDS (R=1, 342 pole-column
+    pairs) and read from global memory (R=20, 6840 pairs). The solutions, h and the iteration
+    counts must still equal the oracle's (polar_gpu.py:100-125)."""
+    n, e, G0, G, _, _ = _small_problem(seed=3)
+    tau, om, _ = pg.fixed_rule(0.5, 342, 0.1, G.Lam(5))
+    assert len(tau) == 342
+    rng = np.random.default_rng(40 + R)
+    W = rng.standard_normal((n, R))
+    active = np.sort(rng.choice(n, 40, replace=False))
+    op0 = O.Op(G0, arith='cr')
+    Xo, ho, ito = O.msolve(op0, W, tau, active, 1e-10, track=True, freeze=True)
+    Xg, hg, itg = pg.msolve(pg.Op(G), torch.from_numpy(W).cuda(), tau, active, 1e-10, track=True,
+                            freeze=True)
+    assert np.array_equal(ito, itg), (ito, itg)
+    Xg = Xg.cpu().numpy()
+    assert Xg.shape == Xo.shape == (len(tau), len(active), R)
+    assert np.max(np.abs(Xg - Xo)) <= 1e-13 * np.max(np.abs(Xo))
+    assert np.max(np.abs(hg.cpu().numpy() - ho)) <= 1e-13 * np.max(np.abs(ho))
+
+
 def test_msolve_iteration_cap():
     n, e, G0, G, tau, om = _small_problem()
     W = np.random.default_rng(0).standard_normal((n, 4))
